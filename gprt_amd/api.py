@@ -1,0 +1,245 @@
+"""ctypes bindings of libgprt_hip.so (declared in include/gprt_hip.h).
+
+The names, argument order and meaning are GPRT's host API (gprt/gprt_host.h) for the
+acceleration-structure path; see the header for the reference line each one replaces. The
+library is the product: this module only loads it and declares signatures. It raises at import
+time of `load()` if the shared library has not been built — there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgprt_hip.so")
+
+GPRT_ERROR_RAISE = 0
+GPRT_ERROR_RECORD = 1
+
+GPRT_TRIANGLES = 2
+GPRT_BUILD_MODE_FAST_BUILD_NO_UPDATE = 1
+GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE = 2
+GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE = 3
+GPRT_BUILD_MODE_FAST_TRACE_AND_UPDATE = 4
+
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("primID", "<i4"), ("u", "<f4"), ("v", "<f4")])
+
+
+class GPRTBuildParams(C.Structure):
+    """gprt/gprt_host.h:1423-1469 (same members, order and defaults)."""
+    _fields_ = [("buildMode", C.c_int), ("allowCompaction", C.c_bool), ("minimizeMemory", C.c_bool),
+                ("hasMotionBlur", C.c_bool), ("test0", C.c_bool), ("test1", C.c_bool)]
+
+    def __init__(self, buildMode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE):
+        super().__init__(buildMode, False, False, False, False, False)
+
+
+_P = C.c_void_p
+_SIGS = {
+    # name: (restype, argtypes)
+    "gprtVersion": (C.c_char_p, []),
+    "gprtSetErrorMode": (None, [C.c_int]),
+    "gprtGetLastError": (C.c_char_p, []),
+    "gprtClearLastError": (None, []),
+    "gprtContextCreate": (_P, [C.POINTER(C.c_int32), C.c_int]),
+    "gprtContextDestroy": (None, [_P]),
+    "gprtHostBufferCreate": (_P, [_P, C.c_size_t, C.c_size_t, _P, C.c_size_t]),
+    "gprtDeviceBufferCreate": (_P, [_P, C.c_size_t, C.c_size_t, _P, C.c_size_t]),
+    "gprtSharedBufferCreate": (_P, [_P, C.c_size_t, C.c_size_t, _P, C.c_size_t]),
+    "gprtBufferClear": (None, [_P]),
+    "gprtBufferDestroy": (None, [_P]),
+    "gprtBufferGetSize": (C.c_size_t, [_P, C.c_int]),
+    "gprtBufferGetHostPointer": (_P, [_P, C.c_int]),
+    "gprtBufferMap": (None, [_P, C.c_int]),
+    "gprtBufferUnmap": (None, [_P, C.c_int]),
+    "gprtBufferGetDevicePointer": (_P, [_P, C.c_int]),
+    "gprtBufferResize": (None, [_P, _P, C.c_size_t, C.c_size_t, C.c_bool, C.c_int]),
+    "gprtBufferCopy": (None, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int]),
+    "gprtBufferExclusiveSum": (C.c_uint32, [_P, _P, _P, _P]),
+    "gprtBufferPartition": (C.c_uint32, [_P, _P, C.c_bool, _P, _P]),
+    "gprtBufferSelect": (C.c_uint32, [_P, _P, C.c_bool, _P, _P]),
+    "gprtBufferSort": (None, [_P, _P, _P]),
+    "gprtBufferSortPayload": (None, [_P, _P, _P, _P]),
+    "gprtBufferSortKV": (None, [_P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "gprtSetSortConfig": (None, [_P, C.c_uint32, C.c_uint32]),
+    "gprtGeomTypeCreate": (_P, [_P, C.c_int, C.c_size_t]),
+    "gprtGeomTypeDestroy": (None, [_P]),
+    "gprtGeomCreate": (_P, [_P, _P]),
+    "gprtGeomDestroy": (None, [_P]),
+    "gprtTrianglesSetVertices": (None, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gprtTrianglesSetIndices": (None, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gprtTriangleAccelCreate": (_P, [_P, _P, C.c_uint]),
+    "gprtAccelDestroy": (None, [_P]),
+    "gprtAccelBuild": (None, [_P, _P, GPRTBuildParams]),
+    "gprtAccelUpdate": (None, [_P, _P]),
+    "gprtAccelGetSize": (C.c_size_t, [_P, C.c_int]),
+    "gprtAccelGetDeviceAddress": (_P, [_P, C.c_int]),
+    "gprtAccelGetPrimCount": (C.c_uint32, [_P]),
+    "gprtAccelTraceClosestHit": (None, [_P, _P, _P, _P, C.c_uint32]),
+    "gprtAccelExportLBVH": (None, [_P, _P, _P, _P, _P]),
+    "gprtComputeSynchronize": (C.c_uint64, [_P]),
+    "gprtDeviceSynchronize": (C.c_uint64, [_P]),
+    "gprtBeginProfile": (None, [_P]),
+    "gprtEndProfile": (C.c_float, [_P]),
+    "gprtKernelTimerEnable": (None, [_P, C.c_bool]),
+    "gprtKernelTimerReset": (None, [_P]),
+    "gprtKernelTimerQuery": (C.c_double, [_P, C.c_char_p, C.POINTER(C.c_uint32)]),
+    "gprtKernelTimerReport": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None) -> C.CDLL:
+    """Load libgprt_hip.so and declare every signature. Raises if it is missing or incomplete."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libgprt_hip.so not built ({p}); run __graft_entry__.build() first")
+    lib = C.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)  # AttributeError if the symbol is missing
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def symbols() -> list[str]:
+    return list(_SIGS)
+
+
+# ------------------------------------------------------------------------------------------------
+# Thin helpers (numpy <-> GPRT buffers)
+# ------------------------------------------------------------------------------------------------
+class Context:
+    """A GPRT context on one HIP device (gprtContextCreate / gprtContextDestroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        dev = (C.c_int32 * 1)(device)
+        self.h = self.lib.gprtContextCreate(dev, 1)
+        if not self.h:
+            raise RuntimeError("gprtContextCreate failed: %s" % last_error())
+
+    def close(self):
+        if self.h:
+            self.lib.gprtContextDestroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_buffer(self, data=None, nbytes: int | None = None) -> "Buffer":
+        if data is not None:
+            arr = np.ascontiguousarray(data)
+            h = self.lib.gprtDeviceBufferCreate(self.h, arr.itemsize, arr.size,
+                                                arr.ctypes.data_as(C.c_void_p), 16)
+            size = arr.nbytes
+        else:
+            size = int(nbytes or 0)
+            h = self.lib.gprtDeviceBufferCreate(self.h, 1, size, None, 16)
+        if not h:
+            raise RuntimeError("gprtDeviceBufferCreate failed: %s" % last_error())
+        return Buffer(self, h)
+
+    def sync(self):
+        self.lib.gprtComputeSynchronize(self.h)
+
+
+class Buffer:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h, self.lib = ctx, h, ctx.lib
+
+    @property
+    def nbytes(self) -> int:
+        return int(self.lib.gprtBufferGetSize(self.h, 0))
+
+    def read(self, dtype, count: int | None = None) -> np.ndarray:
+        """Map (waits for the stream), copy out, unmap."""
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        self.lib.gprtBufferMap(self.h, 0)
+        ptr = self.lib.gprtBufferGetHostPointer(self.h, 0)
+        out = np.empty(n, dtype=dt)
+        if n:
+            C.memmove(out.ctypes.data, ptr, n * dt.itemsize)
+        self.lib.gprtBufferUnmap(self.h, 0)
+        return out
+
+    def write(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.lib.gprtBufferMap(self.h, 0)
+        ptr = self.lib.gprtBufferGetHostPointer(self.h, 0)
+        C.memmove(ptr, arr.ctypes.data, arr.nbytes)
+        self.lib.gprtBufferUnmap(self.h, 0)
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtBufferDestroy(self.h)
+            self.h = None
+
+
+def last_error() -> str | None:
+    e = load().gprtGetLastError()
+    return e.decode() if e else None
+
+
+class TriangleAccel:
+    """gprtGeomTypeCreate + gprtGeomCreate + gprtTriangles* + gprtTriangleAccelCreate."""
+
+    def __init__(self, ctx: Context, verts: np.ndarray, idx: np.ndarray):
+        self.ctx, self.lib = ctx, ctx.lib
+        verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        self.nv, self.nt = len(verts), len(idx)
+        self.vbuf = ctx.device_buffer(verts)
+        self.ibuf = ctx.device_buffer(idx)
+        self.gtype = self.lib.gprtGeomTypeCreate(ctx.h, GPRT_TRIANGLES, 0)
+        self.geom = self.lib.gprtGeomCreate(ctx.h, self.gtype)
+        self.lib.gprtTrianglesSetVertices(self.geom, self.vbuf.h, self.nv, 12, 0)
+        self.lib.gprtTrianglesSetIndices(self.geom, self.ibuf.h, self.nt, 12, 0)
+        self.h = self.lib.gprtTriangleAccelCreate(ctx.h, self.geom, 0)
+
+    def build(self, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE):
+        self.lib.gprtAccelBuild(self.ctx.h, self.h, GPRTBuildParams(mode))
+
+    def update(self):
+        self.lib.gprtAccelUpdate(self.ctx.h, self.h)
+
+    def export(self):
+        """Reference-layout LBVH: (keys u64[N], nodes int32[2N-1,4], aabbs float32[2N-1,6])."""
+        nb = self.ctx.device_buffer(nbytes=16)
+        ab = self.ctx.device_buffer(nbytes=16)
+        kb = self.ctx.device_buffer(nbytes=16)
+        self.lib.gprtAccelExportLBVH(self.ctx.h, self.h, nb.h, ab.h, kb.h)
+        n = self.nt
+        keys = kb.read(np.uint64, n)
+        nodes = nb.read(np.int32, 4 * (2 * n - 1)).reshape(-1, 4)
+        aabbs = ab.read(np.float32, 6 * (2 * n - 1)).reshape(-1, 6)
+        for b in (nb, ab, kb):
+            b.destroy()
+        return keys, nodes, aabbs
+
+    def trace(self, rays_buf: Buffer, hits_buf: Buffer, count: int):
+        self.lib.gprtAccelTraceClosestHit(self.ctx.h, self.h, rays_buf.h, hits_buf.h, count)
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtAccelDestroy(self.h)
+            self.lib.gprtGeomDestroy(self.geom)
+            self.lib.gprtGeomTypeDestroy(self.gtype)
+            self.vbuf.destroy()
+            self.ibuf.destroy()
+            self.h = None

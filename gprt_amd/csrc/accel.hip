@@ -1,0 +1,225 @@
+// C-ABI of the triangle geometry / accel family: gprtGeomTypeCreate, gprtGeomCreate,
+// gprtTrianglesSetVertices/Indices, gprtTriangleAccelCreate, gprtAccelBuild/Update/Destroy
+// (gprt/gprt_host.h:406-498, :1308, :1421-1557; gprt/gprt.cpp:3149-3575, :9152-9262) and the
+// ray-batch trace and LBVH export extensions.
+#include "bvh.hpp"
+#include "internal.hpp"
+
+using namespace gprt;
+
+GPRT_API GPRTGeomType gprtGeomTypeCreate(GPRTContext _ctx, GPRTGeomKind kind, size_t recordSize) {
+  Context *ctx = (Context *)_ctx;
+  if (!ctx) {
+    fail("gprtGeomTypeCreate: null context");
+    return nullptr;
+  }
+  if (kind != GPRT_TRIANGLES) {
+    fail("gprtGeomTypeCreate: only GPRT_TRIANGLES geometry has a software accel in this build (kind %d)", (int)kind);
+    return nullptr;
+  }
+  GeomType *t = new GeomType{ctx, kind, recordSize};
+  return (GPRTGeomType)t;
+}
+
+GPRT_API void gprtGeomTypeDestroy(GPRTGeomType t) { delete (GeomType *)t; }
+
+GPRT_API GPRTGeom gprtGeomCreate(GPRTContext _ctx, GPRTGeomType _type) {
+  Context *ctx = (Context *)_ctx;
+  GeomType *type = (GeomType *)_type;
+  if (!ctx || !type) {
+    fail("gprtGeomCreate: null argument");
+    return nullptr;
+  }
+  Geom *g = new Geom();
+  g->ctx = ctx;
+  g->type = type;
+  return (GPRTGeom)g;
+}
+
+GPRT_API void gprtGeomDestroy(GPRTGeom g) { delete (Geom *)g; }
+
+GPRT_API void gprtTrianglesSetVertices(GPRTGeom _g, GPRTBuffer _v, uint32_t count, uint32_t stride, uint32_t offset) {
+  Geom *g = (Geom *)_g;
+  Buffer *v = (Buffer *)_v;
+  if (!g || !v) {
+    fail("gprtTrianglesSetVertices: null argument");
+    return;
+  }
+  if (stride < 12 || (stride & 3) || (offset & 3)) {
+    fail("gprtTrianglesSetVertices: stride %u / offset %u must be 4-byte aligned with stride >= 12", stride, offset);
+    return;
+  }
+  if (count && (size_t)offset + (size_t)(count - 1) * stride + 12 > v->size) {
+    fail("gprtTrianglesSetVertices: %u vertices (stride %u, offset %u) exceed the buffer (%zu B)", count, stride,
+         offset, v->size);
+    return;
+  }
+  g->vbuf = v;
+  g->vcount = count;
+  g->vstride = stride;
+  g->voffset = offset;
+}
+
+GPRT_API void gprtTrianglesSetIndices(GPRTGeom _g, GPRTBuffer _i, uint32_t count, uint32_t stride, uint32_t offset) {
+  Geom *g = (Geom *)_g;
+  Buffer *ib = (Buffer *)_i;
+  if (!g || !ib) {
+    fail("gprtTrianglesSetIndices: null argument");
+    return;
+  }
+  if (stride < 12 || (stride & 3) || (offset & 3)) {
+    fail("gprtTrianglesSetIndices: stride %u / offset %u must be 4-byte aligned with stride >= 12", stride, offset);
+    return;
+  }
+  if (count && (size_t)offset + (size_t)(count - 1) * stride + 12 > ib->size) {
+    fail("gprtTrianglesSetIndices: %u triangles (stride %u, offset %u) exceed the buffer (%zu B)", count, stride,
+         offset, ib->size);
+    return;
+  }
+  g->ibuf = ib;
+  g->icount = count;
+  g->istride = stride;
+  g->ioffset = offset;
+}
+
+GPRT_API GPRTAccel gprtTriangleAccelCreate(GPRTContext _ctx, GPRTGeom _g, unsigned int flags) {
+  Context *ctx = (Context *)_ctx;
+  Geom *g = (Geom *)_g;
+  if (!ctx || !g) {
+    fail("gprtTriangleAccelCreate: null argument");
+    return nullptr;
+  }
+  if (g->type->kind != GPRT_TRIANGLES) {
+    fail("gprtTriangleAccelCreate: geometry is not GPRT_TRIANGLES");
+    return nullptr;
+  }
+  Accel *a = new Accel();
+  a->ctx = ctx;
+  a->geom = g;
+  a->flags = flags;
+  return (GPRTAccel)a;
+}
+
+GPRT_API void gprtAccelDestroy(GPRTAccel _a) {
+  Accel *a = (Accel *)_a;
+  if (!a) return;
+  hipStreamSynchronize(a->ctx->stream);
+  lbvh_release(a->bvh);
+  delete a;
+}
+
+static bool validate_geom(Geom *g, const char *fn) {
+  if (!g->vbuf || !g->ibuf) {
+    fail("%s: triangle geometry needs vertices and indices", fn);
+    return false;
+  }
+  if (g->icount == 0 || g->vcount == 0) {
+    fail("%s: geometry has no triangles or no vertices", fn);
+    return false;
+  }
+  if (g->icount >= (1u << 30)) {
+    fail("%s: %u triangles exceeds the 2^30 limit", fn, g->icount);
+    return false;
+  }
+  if (g->vbuf->kind == BufKind::Host || g->ibuf->kind == BufKind::Host) {
+    // Host buffers are device-visible (mapped pinned memory); allowed but slow.
+  }
+  return true;
+}
+
+GPRT_API void gprtAccelBuild(GPRTContext _ctx, GPRTAccel _a, GPRTBuildParams params) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  if (!ctx || !a) {
+    fail("gprtAccelBuild: null argument");
+    return;
+  }
+  if (!validate_geom(a->geom, "gprtAccelBuild")) return;
+  if (params.buildMode == GPRT_BUILD_MODE_UNINITIALIZED) params.buildMode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
+  a->mode = params.buildMode;
+  if (lbvh_build(ctx, ctx->stream, a)) a->built = true;
+}
+
+GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  if (!ctx || !a) {
+    fail("gprtAccelUpdate: null argument");
+    return;
+  }
+  if (!a->built) {
+    fail("gprtAccelUpdate: accel has not been built");
+    return;
+  }
+  // gprt/gprt.cpp:3404-3408: update requires an *_AND_UPDATE build mode.
+  if (a->mode != GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE && a->mode != GPRT_BUILD_MODE_FAST_TRACE_AND_UPDATE) {
+    fail("Attempting to update an accel that was not built with an *_AND_UPDATE build mode");
+    return;
+  }
+  if (!validate_geom(a->geom, "gprtAccelUpdate")) return;
+  if (a->geom->icount != a->bvh.n) {
+    fail("gprtAccelUpdate: triangle count changed (%u -> %u); rebuild instead", a->bvh.n, a->geom->icount);
+    return;
+  }
+  lbvh_refit(ctx, ctx->stream, a);
+}
+
+GPRT_API size_t gprtAccelGetSize(GPRTAccel _a, int) {
+  Accel *a = (Accel *)_a;
+  return a ? a->bvh.blockBytes : 0;
+}
+
+GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel _a, int) {
+  Accel *a = (Accel *)_a;
+  return a ? a->bvh.nodes : nullptr;
+}
+
+GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel _a) {
+  Accel *a = (Accel *)_a;
+  return a ? a->bvh.n : 0;
+}
+
+GPRT_API void gprtAccelTraceClosestHit(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _rays, GPRTBuffer _hits,
+                                       uint32_t count) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits;
+  if (!ctx || !a || !rays || !hits) {
+    fail("gprtAccelTraceClosestHit: null argument");
+    return;
+  }
+  if (!a->built) {
+    fail("gprtAccelTraceClosestHit: accel has not been built");
+    return;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size) {
+    fail("gprtAccelTraceClosestHit: %u rays exceed the ray buffer (%zu B)", count, rays->size);
+    return;
+  }
+  if ((size_t)count * sizeof(GPRTHit) > hits->size) {
+    fail("gprtAccelTraceClosestHit: %u hits exceed the hit buffer (%zu B)", count, hits->size);
+    return;
+  }
+  trace_closest(ctx, ctx->stream, a, rays->d, hits->d, count);
+}
+
+GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _nodes, GPRTBuffer _aabbs,
+                                  GPRTBuffer _keys) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  Buffer *nodes = (Buffer *)_nodes, *aabbs = (Buffer *)_aabbs, *keys = (Buffer *)_keys;
+  if (!ctx || !a || !nodes || !aabbs || !keys) {
+    fail("gprtAccelExportLBVH: null argument");
+    return;
+  }
+  if (!a->built) {
+    fail("gprtAccelExportLBVH: accel has not been built");
+    return;
+  }
+  const size_t n = a->bvh.n;
+  const size_t total = 2 * n - 1;
+  if (nodes->size < total * 16 && !buffer_realloc(nodes, total * 16, false)) return;
+  if (aabbs->size < total * 24 && !buffer_realloc(aabbs, total * 24, false)) return;
+  if (keys->size < n * 8 && !buffer_realloc(keys, n * 8, false)) return;
+  lbvh_export(ctx, ctx->stream, a, (int32_t *)nodes->d, (float *)aabbs->d, (uint64_t *)keys->d);
+}

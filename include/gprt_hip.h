@@ -1,0 +1,228 @@
+/*
+ * gprt_hip.h — C-ABI of the MI355X (gfx950) software acceleration-structure path.
+ *
+ * The entry points keep the names, argument order and meaning of GPRT's public host API
+ * (gprt/gprt_host.h) for the path they replace: buffers, radix sort, exclusive scan /
+ * partition / select, triangle accel create / build / update, synchronisation and profiling.
+ * Every exported symbol is `extern "C"`, takes opaque handles, plain pointers and sizes, and
+ * carries no C++ or torch types, so cgo / ctypes / JNI can bind it directly.
+ *
+ * Each declaration cites the reference declaration it stands in for (file:line in GPRT).
+ * Declarations under "Extensions" have no reference counterpart; each says why it exists.
+ *
+ * Error convention (gprt/gprt.cpp:205-251): an invalid argument or a failed HIP call prints
+ * "#gprt error: <message>" and raises SIGINT, as the reference's LOG_ERROR does.
+ * gprtSetErrorMode(GPRT_ERROR_RECORD) switches to a recording mode in which the message is kept
+ * for gprtGetLastError() and the call returns without touching the GPU (used by tests).
+ */
+#ifndef GPRT_HIP_H
+#define GPRT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+#define GPRT_API extern "C" __attribute__((visibility("default")))
+#define GPRT_IF_CPP(a) a
+#else
+#define GPRT_API __attribute__((visibility("default")))
+#define GPRT_IF_CPP(a)
+#endif
+
+/* Opaque handles: gprt/gprt_host.h:104-117 (GPRT_DEFINE_HANDLE). */
+typedef struct GPRTContext_T *GPRTContext;
+typedef struct GPRTBuffer_T *GPRTBuffer;
+typedef struct GPRTGeom_T *GPRTGeom;
+typedef struct GPRTGeomType_T *GPRTGeomType;
+typedef struct GPRTAccel_T *GPRTAccel;
+
+/* gprt/gprt_host.h:286-288 */
+typedef enum {
+  GPRT_UNKNOWN,
+  GPRT_AABBS,
+  GPRT_TRIANGLES,
+  GPRT_SPHERES,
+  GPRT_LSS,
+  GPRT_SOLIDS,
+  GPRT_PARTICLES
+} GPRTGeomKind;
+
+/* gprt/gprt_host.h:349-355 */
+typedef enum {
+  GPRT_BUILD_MODE_UNINITIALIZED,
+  GPRT_BUILD_MODE_FAST_BUILD_NO_UPDATE,
+  GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE,
+  GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE,
+  GPRT_BUILD_MODE_FAST_TRACE_AND_UPDATE,
+} GPRTBuildMode;
+
+/* gprt/gprt_host.h:1423-1469 — same members, same order, same defaults. */
+typedef struct GPRTBuildParams {
+  GPRTBuildMode buildMode GPRT_IF_CPP(= GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE);
+  bool allowCompaction GPRT_IF_CPP(= false);
+  bool minimizeMemory GPRT_IF_CPP(= false);
+  bool hasMotionBlur GPRT_IF_CPP(= false);
+  bool test0 GPRT_IF_CPP(= false);
+  bool test1 GPRT_IF_CPP(= false);
+} GPRTBuildParams;
+
+/* ---------------------------------------------------------------------------------------------
+ * Context (gprt/gprt_host.h:992-995). One HIP device and one in-order stream per context.
+ * numDevices must be 1: the reference has no multi-GPU path either (gprt/gprt.cpp:5373-5374).
+ * ------------------------------------------------------------------------------------------- */
+GPRT_API GPRTContext gprtContextCreate(int32_t *requestedDeviceIDs GPRT_IF_CPP(= nullptr),
+                                       int numDevices GPRT_IF_CPP(= 1));
+GPRT_API void gprtContextDestroy(GPRTContext context);
+
+/* ---------------------------------------------------------------------------------------------
+ * Buffers (gprt/gprt_host.h:1899-2210). `size` is the element size in bytes, `count` the number
+ * of elements, exactly as in the reference. Device and shared buffers live in HBM and are read
+ * and written by the host through map/unmap, which first waits for the context's stream (the
+ * reference's map does not, gprt/gprt.cpp:6774; waiting here removes that hazard).
+ * ------------------------------------------------------------------------------------------- */
+GPRT_API GPRTBuffer gprtHostBufferCreate(GPRTContext context, size_t size, size_t count GPRT_IF_CPP(= 1),
+                                         const void *init GPRT_IF_CPP(= nullptr),
+                                         size_t alignment GPRT_IF_CPP(= 16)); /* gprt_host.h:1899 */
+GPRT_API GPRTBuffer gprtDeviceBufferCreate(GPRTContext context, size_t size, size_t count GPRT_IF_CPP(= 1),
+                                           const void *init GPRT_IF_CPP(= nullptr),
+                                           size_t alignment GPRT_IF_CPP(= 16)); /* gprt_host.h:1946 */
+GPRT_API GPRTBuffer gprtSharedBufferCreate(GPRTContext context, size_t size, size_t count GPRT_IF_CPP(= 1),
+                                           const void *init GPRT_IF_CPP(= nullptr),
+                                           size_t alignment GPRT_IF_CPP(= 16)); /* gprt_host.h:1995 */
+GPRT_API void gprtBufferClear(GPRTBuffer buffer);                            /* gprt_host.h:2037 */
+GPRT_API void gprtBufferDestroy(GPRTBuffer buffer);                          /* gprt_host.h:2053 */
+GPRT_API size_t gprtBufferGetSize(GPRTBuffer buffer, int deviceID GPRT_IF_CPP(= 0)); /* :2067 */
+GPRT_API void *gprtBufferGetHostPointer(GPRTBuffer buffer, int deviceID GPRT_IF_CPP(= 0)); /* :2082 */
+GPRT_API void gprtBufferMap(GPRTBuffer buffer, int deviceID GPRT_IF_CPP(= 0));     /* :2108 */
+GPRT_API void gprtBufferUnmap(GPRTBuffer buffer, int deviceID GPRT_IF_CPP(= 0));   /* :2116 */
+GPRT_API void *gprtBufferGetDevicePointer(GPRTBuffer buffer, int deviceID GPRT_IF_CPP(= 0)); /* :2128 */
+GPRT_API void gprtBufferResize(GPRTContext context, GPRTBuffer buffer, size_t size, size_t count,
+                               bool preserveContents, int deviceID GPRT_IF_CPP(= 0)); /* :2150 */
+GPRT_API void gprtBufferCopy(GPRTContext context, GPRTBuffer src, GPRTBuffer dst, size_t srcOffset,
+                             size_t dstOffset, size_t size, size_t count, int srcDeviceID GPRT_IF_CPP(= 0),
+                             int dstDeviceID GPRT_IF_CPP(= 0)); /* :2187 */
+
+/* ---------------------------------------------------------------------------------------------
+ * Scan family (gprt/gprt_host.h:2246, :2295, :2344; stub in gprt/gprt.cpp:8739-8818).
+ * Input is 32-bit; the element count is bytes/4 (gprt/gprt.cpp:8750 in the commented-out path).
+ * ExclusiveSum wraps modulo 2^32 (no 2^30 limit: look-back words are 64-bit) and returns the
+ * total. Partition places selected items first in input order and unselected items at the tail
+ * in reverse order (gprt/gprt_scan.slang:89-92); Select compacts the selected items. Selection
+ * is "v >= 0" when selectPositive, else "v < 0" (tests/t03-bufferScans/hostCode.cpp:117).
+ * scratch may be null (internal allocation) or undersized (resized in place), as documented at
+ * gprt/gprt_host.h:2238-2241. The return value requires a host wait (the reference maps scratch).
+ * ------------------------------------------------------------------------------------------- */
+GPRT_API uint32_t gprtBufferExclusiveSum(GPRTContext context, GPRTBuffer input, GPRTBuffer output,
+                                         GPRTBuffer scratch GPRT_IF_CPP(= nullptr));
+GPRT_API uint32_t gprtBufferPartition(GPRTContext context, GPRTBuffer input, bool selectPositive,
+                                      GPRTBuffer output, GPRTBuffer scratch GPRT_IF_CPP(= nullptr));
+GPRT_API uint32_t gprtBufferSelect(GPRTContext context, GPRTBuffer input, bool selectPositive, GPRTBuffer output,
+                                   GPRTBuffer scratch GPRT_IF_CPP(= nullptr));
+
+/* ---------------------------------------------------------------------------------------------
+ * Radix sort (gprt/gprt_host.h:2384, :2416; gprt/gprt.cpp:8821-9073). Stable LSD sort, ascending,
+ * in place; the payload follows its key. As in the reference (gprt/gprt.cpp:8837) these two
+ * entry points treat keys as 64-bit (count = bytes/8) and the payload as 64-bit; the key and
+ * payload buffers must be the same size (gprt/gprt.cpp:8829-8835). Enqueued; returns at once.
+ * ------------------------------------------------------------------------------------------- */
+GPRT_API void gprtBufferSort(GPRTContext context, GPRTBuffer buffer, GPRTBuffer scratch GPRT_IF_CPP(= nullptr));
+GPRT_API void gprtBufferSortPayload(GPRTContext context, GPRTBuffer keys, GPRTBuffer values,
+                                    GPRTBuffer scratch GPRT_IF_CPP(= nullptr));
+
+/* ---------------------------------------------------------------------------------------------
+ * Triangle geometry and accel (gprt/gprt_host.h:406, :476, :498, :1308, :1421, :1506, :1529,
+ * :1553, :1557, :1580). The accel is a binary LBVH: 30-bit Morton keys of triangle centroids
+ * normalised to the scene box, packed as morton<<32|primID, sorted, Karras split search, and a
+ * bottom-up refit with per-node arrival counters. Build and update are enqueued on the stream.
+ * ------------------------------------------------------------------------------------------- */
+GPRT_API GPRTGeomType gprtGeomTypeCreate(GPRTContext context, GPRTGeomKind kind, size_t recordSize);
+GPRT_API void gprtGeomTypeDestroy(GPRTGeomType geomType);
+GPRT_API GPRTGeom gprtGeomCreate(GPRTContext context, GPRTGeomType type);
+GPRT_API void gprtGeomDestroy(GPRTGeom geometry);
+GPRT_API void gprtTrianglesSetVertices(GPRTGeom triangles, GPRTBuffer vertices, uint32_t count,
+                                       uint32_t stride GPRT_IF_CPP(= 12), uint32_t offset GPRT_IF_CPP(= 0));
+GPRT_API void gprtTrianglesSetIndices(GPRTGeom triangles, GPRTBuffer indices, uint32_t count,
+                                      uint32_t stride GPRT_IF_CPP(= 12), uint32_t offset GPRT_IF_CPP(= 0));
+GPRT_API GPRTAccel gprtTriangleAccelCreate(GPRTContext context, GPRTGeom geom, unsigned int flags GPRT_IF_CPP(= 0));
+GPRT_API void gprtAccelDestroy(GPRTAccel accel);
+GPRT_API void gprtAccelBuild(GPRTContext context, GPRTAccel accel,
+                             GPRTBuildParams buildParams GPRT_IF_CPP(= GPRTBuildParams()));
+GPRT_API void gprtAccelUpdate(GPRTContext context, GPRTAccel accel);
+GPRT_API size_t gprtAccelGetSize(GPRTAccel accel, int deviceID GPRT_IF_CPP(= 0));
+GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel accel, int deviceID GPRT_IF_CPP(= 0));
+
+/* ---------------------------------------------------------------------------------------------
+ * Synchronisation and profiling (gprt/gprt_host.h:2588-2595). Begin/EndProfile time the work
+ * enqueued between the two calls on the context's stream, in milliseconds.
+ * ------------------------------------------------------------------------------------------- */
+GPRT_API uint64_t gprtComputeSynchronize(GPRTContext context);
+GPRT_API uint64_t gprtDeviceSynchronize(GPRTContext context);
+GPRT_API void gprtBeginProfile(GPRTContext context);
+GPRT_API float gprtEndProfile(GPRTContext context);
+
+/* =============================================================================================
+ * Extensions (no reference counterpart)
+ * =========================================================================================== */
+
+/* Error handling for callers that cannot take SIGINT (language bindings, test runners). */
+#define GPRT_ERROR_RAISE 0  /* default: print and raise(SIGINT), as gprt/gprt.cpp:247-251 */
+#define GPRT_ERROR_RECORD 1 /* print, record the message, return without GPU work */
+GPRT_API void gprtSetErrorMode(int mode);
+GPRT_API const char *gprtGetLastError(void); /* NULL if no error since the last clear */
+GPRT_API void gprtClearLastError(void);
+
+/* Width-explicit sorts. The reference's templates advertise uint32_t buffers while its
+ * implementation reads 64-bit keys (gprt/gprt_host.h:2390 vs gprt/gprt.cpp:8837); these make the
+ * width explicit. keyBits = 32 or 64; valueBits = 0 (no payload), 32 or 64. count = element count
+ * (keys need not fill the buffer). Stable, ascending, in place, enqueued. */
+GPRT_API void gprtBufferSortKV(GPRTContext context, GPRTBuffer keys, GPRTBuffer values, uint32_t count,
+                               uint32_t keyBits, uint32_t valueBits, GPRTBuffer scratch);
+
+/* Radix sort knobs used by every sort and accel build on this context (the autotuner's search
+ * space): digitBits in {4, 6, 8}, tileKeys (keys per workgroup tile) in {2048, 4096, 8192}.
+ * Results do not depend on them. */
+GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_t tileKeys);
+
+/* Ray batch closest hit against a built triangle accel. rays: `count` records of
+ * {float ox, oy, oz, tmin; float dx, dy, dz, tmax} (32 B); hits: `count` records of
+ * {float t; int32 primID; float u, v} (16 B), primID = -1 on a miss. A hit is accepted when
+ * tmin < t < tbest, or t == tbest with a smaller primID, so the result is the lexicographic
+ * minimum of (t, primID) over all triangles the watertight ray–triangle test accepts and does not
+ * depend on traversal order. The reference traces through vkCmdTraceRaysKHR
+ * (gprt/gprt.cpp:9404); this is the software form of that closest-hit query. */
+typedef struct GPRTRay {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+} GPRTRay;
+typedef struct GPRTHit {
+  float t;
+  int32_t primID;
+  float u, v;
+} GPRTHit;
+GPRT_API void gprtAccelTraceClosestHit(GPRTContext context, GPRTAccel accel, GPRTBuffer rays, GPRTBuffer hits,
+                                       uint32_t count);
+
+/* Export of the built LBVH in the reference's development layout (gprt/development/gprt_lbvh.h:28-55):
+ * nodes int4[2N-1] {left, right, parent, leaf primID or -1} with inner nodes [0, N-2] and leaf
+ * i at N-1+i; aabbs float3[2(2N-1)] {min, max} per node; keys uint64[N] sorted
+ * (morton30<<32 | primID). Buffers are resized as needed. Used by parity tests. */
+GPRT_API void gprtAccelExportLBVH(GPRTContext context, GPRTAccel accel, GPRTBuffer nodes, GPRTBuffer aabbs,
+                                  GPRTBuffer keys);
+GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel accel);
+
+/* Per-kernel device timing. While enabled, every kernel the library launches is bracketed by
+ * HIP events on the stream it runs on; gprtKernelTimerQuery returns the summed milliseconds
+ * and launch count of the kernels whose name contains `name` (after a host wait). */
+GPRT_API void gprtKernelTimerEnable(GPRTContext context, bool enable);
+GPRT_API void gprtKernelTimerReset(GPRTContext context);
+GPRT_API double gprtKernelTimerQuery(GPRTContext context, const char *name, uint32_t *launches);
+/* Writes up to `cap` bytes of "name,launches,total_ms\n" lines; returns the bytes needed. */
+GPRT_API size_t gprtKernelTimerReport(GPRTContext context, char *out, size_t cap);
+
+/* Library version string, for logs. */
+GPRT_API const char *gprtVersion(void);
+
+#endif /* GPRT_HIP_H */

@@ -1,0 +1,187 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by __graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline
+leg as the timed CPU path. The product (gprt_amd/) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("primID", "<i4"), ("u", "<f4"), ("v", "<f4")])
+
+
+def build(force: bool = False) -> None:
+    args = ["make", "-C", _HERE] + (["-B"] if force else [])
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sz = C.c_size_t
+        sigs = {
+            "orc_stable_sort": (None, [P, P, sz, C.c_int, C.c_int]),
+            "orc_exclusive_sum": (C.c_uint32, [P, P, sz]),
+            "orc_partition": (C.c_uint32, [P, P, sz, C.c_int]),
+            "orc_select": (C.c_uint32, [P, P, sz, C.c_int]),
+            "orc_separate_bits": (C.c_uint32, [C.c_uint32]),
+            "orc_morton3": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_scene_bounds": (None, [P, P, sz, P]),
+            "orc_morton_keys": (None, [P, P, sz, P, P]),
+            "orc_karras": (None, [P, sz, P]),
+            "orc_topdown": (None, [P, sz, P]),
+            "orc_refit": (None, [P, P, P, sz, P]),
+            "orc_refit_bruteforce": (None, [P, P, P, sz, P]),
+            "orc_lbvh_build": (None, [P, P, sz, P, P, P]),
+            "orc_intersect": (C.c_int, [P, P, P, P, P, P, P]),
+            "orc_plane_t_fp64": (C.c_double, [P, P, P, P]),
+            "orc_trace_bruteforce": (None, [P, sz, P, P, sz, P, C.c_int]),
+            "orc_trace_bvh": (None, [P, sz, P, P, P, P, sz, P, P, C.c_int]),
+            "orc_max_threads": (C.c_int, []),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def stable_sort(keys: np.ndarray, vals: np.ndarray | None = None):
+    k = np.ascontiguousarray(keys).copy()
+    v = None if vals is None else np.ascontiguousarray(vals).copy()
+    lib().orc_stable_sort(_p(k), None if v is None else _p(v), len(k), k.itemsize * 8,
+                          0 if v is None else v.itemsize * 8)
+    return k, v
+
+
+def exclusive_sum(x: np.ndarray):
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.empty_like(x)
+    total = lib().orc_exclusive_sum(_p(x), _p(out), len(x))
+    return out, int(total)
+
+
+def partition(x: np.ndarray, select_positive: bool):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    out = np.zeros_like(x)
+    total = lib().orc_partition(_p(x), _p(out), len(x), int(select_positive))
+    return out, int(total)
+
+
+def select(x: np.ndarray, select_positive: bool):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    out = np.zeros_like(x)
+    total = lib().orc_select(_p(x), _p(out), len(x), int(select_positive))
+    return out, int(total)
+
+
+def scene_bounds(verts, idx):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    out = np.zeros(6, dtype=np.float32)
+    lib().orc_scene_bounds(_p(verts), _p(idx), len(idx), _p(out))
+    return out
+
+
+def morton_keys(verts, idx, bounds):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    bounds = np.ascontiguousarray(bounds, dtype=np.float32)
+    keys = np.zeros(len(idx), dtype=np.uint64)
+    lib().orc_morton_keys(_p(verts), _p(idx), len(idx), _p(bounds), _p(keys))
+    return keys
+
+
+def karras(sorted_keys):
+    k = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+    nodes = np.zeros((2 * len(k) - 1, 4), dtype=np.int32)
+    lib().orc_karras(_p(k), len(k), _p(nodes))
+    return nodes
+
+
+def topdown(sorted_keys):
+    k = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+    nodes = np.zeros((2 * len(k) - 1, 4), dtype=np.int32)
+    lib().orc_topdown(_p(k), len(k), _p(nodes))
+    return nodes
+
+
+def refit(verts, idx, nodes, bruteforce: bool = False):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    n = (len(nodes) + 1) // 2
+    aabbs = np.zeros((len(nodes), 6), dtype=np.float32)
+    fn = lib().orc_refit_bruteforce if bruteforce else lib().orc_refit
+    fn(_p(verts), _p(idx), _p(nodes), n, _p(aabbs))
+    return aabbs
+
+
+def lbvh_build(verts, idx):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    n = len(idx)
+    keys = np.zeros(n, dtype=np.uint64)
+    nodes = np.zeros((2 * n - 1, 4), dtype=np.int32)
+    aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
+    lib().orc_lbvh_build(_p(verts), _p(idx), n, _p(keys), _p(nodes), _p(aabbs))
+    return keys, nodes, aabbs
+
+
+def intersect(ray8, v0, v1, v2):
+    r = np.ascontiguousarray(ray8, dtype=np.float32)
+    a, b, c = (np.ascontiguousarray(v, dtype=np.float32) for v in (v0, v1, v2))
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    ok = lib().orc_intersect(_p(r), _p(a), _p(b), _p(c), C.byref(t), C.byref(u), C.byref(v))
+    return bool(ok), t.value, u.value, v.value
+
+
+def plane_t_fp64(ray8, v0, v1, v2) -> float:
+    r = np.ascontiguousarray(ray8, dtype=np.float32)
+    a, b, c = (np.ascontiguousarray(v, dtype=np.float32) for v in (v0, v1, v2))
+    return float(lib().orc_plane_t_fp64(_p(r), _p(a), _p(b), _p(c)))
+
+
+def _rays_f32(rays) -> np.ndarray:
+    return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+
+
+def trace_bruteforce(rays, verts, idx, threads: int = 0):
+    r = _rays_f32(rays)
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    hits = np.zeros(len(r), dtype=HIT_DTYPE)
+    lib().orc_trace_bruteforce(_p(r), len(r), _p(verts), _p(idx), len(idx), _p(hits),
+                               threads or os.cpu_count() or 1)
+    return hits
+
+
+def trace_bvh(rays, verts, idx, nodes, aabbs, threads: int = 1):
+    r = _rays_f32(rays)
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    aabbs = np.ascontiguousarray(aabbs, dtype=np.float32)
+    n = (len(nodes) + 1) // 2
+    hits = np.zeros(len(r), dtype=HIT_DTYPE)
+    counters = np.zeros(2, dtype=np.uint64)
+    lib().orc_trace_bvh(_p(r), len(r), _p(verts), _p(idx), _p(nodes), _p(aabbs), n, _p(hits), _p(counters),
+                        threads)
+    return hits, counters

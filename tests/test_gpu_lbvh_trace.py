@@ -1,0 +1,173 @@
+"""GPU parity of the LBVH build and closest-hit traversal against the CPU oracle (C-ABI calls).
+
+LBVH: sorted keys (morton30<<32 | primID), topology in the reference's int4 node layout
+(gprt/development/gprt_lbvh.h:28-55) and every node box must be bit-identical to the oracle's
+restatement of gprt/development/gprt_lbvh.hlsl. Trace: primID exact and t / u / v bit-identical
+to the oracle's brute force (every triangle tested) on the same FP32 rays.
+"""
+import numpy as np
+import pytest
+
+from gprt_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(ctx, verts, idx, mode=api.GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE):
+    acc = api.TriangleAccel(ctx, verts, idx)
+    acc.build(mode)
+    assert api.last_error() is None, api.last_error()
+    return acc
+
+
+def _check_lbvh(ctx, orc, verts, idx):
+    acc = _build(ctx, verts, idx)
+    keys, nodes, aabbs = acc.export()
+    rk, rn, ra = orc.lbvh_build(verts, idx)
+    np.testing.assert_array_equal(keys, rk)
+    np.testing.assert_array_equal(nodes, rn)
+    np.testing.assert_array_equal(aabbs.view(np.uint32), ra.view(np.uint32))
+    return acc
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 100, 4097, 16384])
+def test_lbvh_uniform(ctx, orc, n):
+    verts, idx = scenes.uniform_scene(n, seed=1 if n == 16384 else 50 + n)
+    _check_lbvh(ctx, orc, verts, idx).destroy()
+
+
+def test_lbvh_blobs(ctx, orc):
+    verts, idx = scenes.blob_scene(200_000, seed=2)
+    _check_lbvh(ctx, orc, verts, idx).destroy()
+
+
+def test_lbvh_morton_collisions(ctx, orc):
+    # 3000 copies of 5 triangles: identical centroids, keys differ only in primID
+    base, _ = scenes.uniform_scene(5, seed=3)
+    verts = np.tile(base.reshape(5, 3, 3), (600, 1, 1)).reshape(-1, 3)
+    idx = np.arange(len(verts), dtype=np.uint32).reshape(-1, 3)
+    _check_lbvh(ctx, orc, verts, idx).destroy()
+
+
+def test_lbvh_shared_vertices_and_flat_axis(ctx, orc):
+    # indexed grid in the z=0.5 plane: shared vertices, zero extent on z (0/0 quantisation)
+    g = 40
+    ys, xs = np.meshgrid(np.arange(g + 1), np.arange(g + 1), indexing="ij")
+    verts = np.stack([xs.ravel() / g, ys.ravel() / g, np.full(xs.size, 0.5)], 1).astype(np.float32)
+    tris = []
+    for y in range(g):
+        for x in range(g):
+            a = y * (g + 1) + x
+            tris += [[a, a + 1, a + g + 1], [a + 1, a + g + 2, a + g + 1]]
+    idx = np.array(tris, dtype=np.uint32)
+    _check_lbvh(ctx, orc, verts, idx).destroy()
+
+
+def test_lbvh_update_refit(ctx, orc):
+    verts, idx = scenes.blob_scene(50_000, seed=6)
+    acc = _build(ctx, verts, idx, api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    keys0, nodes0, _ = acc.export()
+    moved = (verts + scenes.rand01(7, 0, verts.size).reshape(verts.shape).astype(np.float32) * np.float32(2e-3)
+             - np.float32(1e-3)).astype(np.float32)
+    acc.vbuf.write(moved)
+    acc.update()
+    keys1, nodes1, aabbs1 = acc.export()
+    np.testing.assert_array_equal(nodes1, nodes0)  # topology kept
+    ref = orc.refit(moved, idx, nodes0)
+    np.testing.assert_array_equal(aabbs1.view(np.uint32), ref.view(np.uint32))
+    acc.destroy()
+
+
+def test_update_requires_update_mode(ctx):
+    verts, idx = scenes.uniform_scene(100, seed=4)
+    acc = _build(ctx, verts, idx)
+    acc.update()
+    assert "AND_UPDATE" in api.last_error()
+    ctx.lib.gprtClearLastError()
+    acc.destroy()
+
+
+def _trace(ctx, acc, rays):
+    rb = ctx.device_buffer(rays)
+    hb = ctx.device_buffer(nbytes=len(rays) * 16)
+    acc.trace(rb, hb, len(rays))
+    assert api.last_error() is None, api.last_error()
+    hits = hb.read(api.HIT_DTYPE)
+    rb.destroy()
+    hb.destroy()
+    return hits
+
+
+def _assert_hits_equal(g, r):
+    np.testing.assert_array_equal(g["primID"], r["primID"])
+    np.testing.assert_array_equal(g["t"].view(np.uint32), r["t"].view(np.uint32))
+    np.testing.assert_array_equal(g["u"].view(np.uint32), r["u"].view(np.uint32))
+    np.testing.assert_array_equal(g["v"].view(np.uint32), r["v"].view(np.uint32))
+
+
+def test_trace_uniform_primary_all_rays(ctx, orc):
+    # 16,384 triangles, 256x256 pinhole rays: every ray against brute force
+    verts, idx = scenes.uniform_scene(16384, seed=1)
+    acc = _build(ctx, verts, idx)
+    rays = scenes.pinhole_rays(256, 256)
+    g = _trace(ctx, acc, rays)
+    r = orc.trace_bruteforce(rays, verts, idx)
+    _assert_hits_equal(g, r)
+    assert (g["primID"] >= 0).sum() > 5000
+    # bounce rays from the primary hits
+    b = scenes.bounce_rays(rays, r, verts, idx, 8192, seed=3)
+    _assert_hits_equal(_trace(ctx, acc, b), orc.trace_bruteforce(b, verts, idx))
+    acc.destroy()
+
+
+def test_trace_random_rays_inside_scene(ctx, orc):
+    verts, idx = scenes.blob_scene(30_000, seed=8, offset=0.01)
+    acc = _build(ctx, verts, idx)
+    rays = scenes.random_rays(20_000, seed=9)
+    _assert_hits_equal(_trace(ctx, acc, rays), orc.trace_bruteforce(rays, verts, idx))
+    acc.destroy()
+
+
+def test_trace_blob_scene_sampled(ctx, orc):
+    # a 1M-triangle blob scene; a seeded 2,048-ray subsample (primary + bounce) against brute force
+    verts, idx = scenes.blob_scene(1 << 20, seed=2)
+    acc = _build(ctx, verts, idx)
+    prim = scenes.pinhole_rays(512, 512)
+    gp = _trace(ctx, acc, prim)
+    sel = np.random.default_rng(0).choice(len(prim), 1024, replace=False)
+    _assert_hits_equal(gp[sel], orc.trace_bruteforce(prim[sel], verts, idx))
+    b = scenes.bounce_rays(prim, gp, verts, idx, 1024, seed=3)
+    _assert_hits_equal(_trace(ctx, acc, b), orc.trace_bruteforce(b, verts, idx))
+    acc.destroy()
+
+
+def test_trace_t_error_vs_fp64(ctx, orc):
+    verts, idx = scenes.uniform_scene(16384, seed=1)
+    acc = _build(ctx, verts, idx)
+    rays = scenes.pinhole_rays(128, 128)
+    g = _trace(ctx, acc, rays)
+    r8 = rays.view(np.float32).reshape(-1, 8)
+    errs = []
+    for i in np.nonzero(g["primID"] >= 0)[0][:3000]:
+        v = verts[idx[g["primID"][i]]]
+        t64 = orc.plane_t_fp64(r8[i], v[0], v[1], v[2])
+        errs.append(abs(float(g["t"][i]) - t64) / abs(t64))
+    errs = np.array(errs)
+    print(f"hit t vs FP64 plane distance: max rel err {errs.max():.3e}, median {np.median(errs):.3e}")
+    assert np.median(errs) < 1e-6 and errs.max() < 1e-3
+    acc.destroy()
+
+
+def test_trace_single_triangle_and_misses(ctx, orc):
+    verts = np.array([[-1, -1, 1], [3, -1, 1], [-1, 3, 1]], dtype=np.float32)
+    idx = np.array([[0, 1, 2]], dtype=np.uint32)
+    acc = _build(ctx, verts, idx)
+    rays = np.zeros(3, dtype=api.RAY_DTYPE)
+    rays["oz"] = -1
+    rays["dz"] = [1, -1, 1]
+    rays["tmin"] = [0, 0, 2.0]  # third: hit exactly at tmin is excluded (open interval)
+    rays["tmax"] = 1e30
+    g = _trace(ctx, acc, rays)
+    assert g["primID"].tolist() == [0, -1, -1]
+    assert g["t"][0] == 2.0
+    acc.destroy()
