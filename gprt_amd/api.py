@@ -75,6 +75,7 @@ _SIGS = {
     "gprtTriangleAccelCreate": (_P, [_P, _P, C.c_uint]),
     "gprtAccelDestroy": (None, [_P]),
     "gprtAccelBuild": (None, [_P, _P, GPRTBuildParams]),
+    "gprtAccelBuildBatch": (None, [_P, C.POINTER(C.c_void_p), C.c_uint32, GPRTBuildParams]),
     "gprtAccelUpdate": (None, [_P, _P]),
     "gprtAccelGetSize": (C.c_size_t, [_P, C.c_int]),
     "gprtAccelGetDeviceAddress": (_P, [_P, C.c_int]),
@@ -243,3 +244,9 @@ class TriangleAccel:
             self.vbuf.destroy()
             self.ibuf.destroy()
             self.h = None
+
+
+def build_batch(ctx: Context, accels, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE):
+    """gprtAccelBuildBatch over a list of TriangleAccel objects."""
+    arr = (C.c_void_p * len(accels))(*[a.h for a in accels])
+    ctx.lib.gprtAccelBuildBatch(ctx.h, arr, len(accels), GPRTBuildParams(mode))

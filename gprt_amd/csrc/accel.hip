@@ -2,6 +2,9 @@
 // gprtTrianglesSetVertices/Indices, gprtTriangleAccelCreate, gprtAccelBuild/Update/Destroy
 // (gprt/gprt_host.h:406-498, :1308, :1421-1557; gprt/gprt.cpp:3149-3575, :9152-9262) and the
 // ray-batch trace and LBVH export extensions.
+#include <unordered_set>
+#include <vector>
+
 #include "bvh.hpp"
 #include "internal.hpp"
 
@@ -104,7 +107,7 @@ GPRT_API void gprtAccelDestroy(GPRTAccel _a) {
   Accel *a = (Accel *)_a;
   if (!a) return;
   hipStreamSynchronize(a->ctx->stream);
-  lbvh_release(a->bvh);
+  accel_detach(a);
   delete a;
 }
 
@@ -137,7 +140,35 @@ GPRT_API void gprtAccelBuild(GPRTContext _ctx, GPRTAccel _a, GPRTBuildParams par
   if (!validate_geom(a->geom, "gprtAccelBuild")) return;
   if (params.buildMode == GPRT_BUILD_MODE_UNINITIALIZED) params.buildMode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
   a->mode = params.buildMode;
-  if (lbvh_build(ctx, ctx->stream, a)) a->built = true;
+  // A single build detaches the accel from any batch store it shared.
+  if (a->store && a->store->refs > 1) accel_detach(a);
+  lbvh_build_batch(ctx, ctx->stream, &a, 1, false);
+}
+
+GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t count, GPRTBuildParams params) {
+  Context *ctx = (Context *)_ctx;
+  if (!ctx || !_accels) {
+    fail("gprtAccelBuildBatch: null argument");
+    return;
+  }
+  if (count == 0) return;
+  std::vector<Accel *> accels(count);
+  std::unordered_set<Accel *> seen;
+  for (uint32_t i = 0; i < count; ++i) {
+    accels[i] = (Accel *)_accels[i];
+    if (!accels[i]) {
+      fail("gprtAccelBuildBatch: accel %u is null", i);
+      return;
+    }
+    if (!seen.insert(accels[i]).second) {
+      fail("gprtAccelBuildBatch: accel %u appears twice", i);
+      return;
+    }
+    if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatch")) return;
+  }
+  if (params.buildMode == GPRT_BUILD_MODE_UNINITIALIZED) params.buildMode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
+  for (auto *a : accels) a->mode = params.buildMode;
+  lbvh_build_batch(ctx, ctx->stream, accels.data(), count, true);
 }
 
 GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
@@ -157,8 +188,8 @@ GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
     return;
   }
   if (!validate_geom(a->geom, "gprtAccelUpdate")) return;
-  if (a->geom->icount != a->bvh.n) {
-    fail("gprtAccelUpdate: triangle count changed (%u -> %u); rebuild instead", a->bvh.n, a->geom->icount);
+  if (a->geom->icount != a->n) {
+    fail("gprtAccelUpdate: triangle count changed (%u -> %u); rebuild instead", a->n, a->geom->icount);
     return;
   }
   lbvh_refit(ctx, ctx->stream, a);
@@ -166,17 +197,19 @@ GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
 
 GPRT_API size_t gprtAccelGetSize(GPRTAccel _a, int) {
   Accel *a = (Accel *)_a;
-  return a ? a->bvh.blockBytes : 0;
+  // Nodes + leaf triangle records of this accel (the traversal-resident part).
+  return (a && a->built) ? (size_t)(a->n > 1 ? a->n - 1 : 1) * sizeof(BVHNode) + (size_t)a->n * sizeof(TriRecord)
+                         : 0;
 }
 
 GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel _a, int) {
   Accel *a = (Accel *)_a;
-  return a ? a->bvh.nodes : nullptr;
+  return a ? a->nodes : nullptr;
 }
 
 GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel _a) {
   Accel *a = (Accel *)_a;
-  return a ? a->bvh.n : 0;
+  return a ? a->n : 0;
 }
 
 GPRT_API void gprtAccelTraceClosestHit(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _rays, GPRTBuffer _hits,
@@ -216,7 +249,7 @@ GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _no
     fail("gprtAccelExportLBVH: accel has not been built");
     return;
   }
-  const size_t n = a->bvh.n;
+  const size_t n = a->n;
   const size_t total = 2 * n - 1;
   if (nodes->size < total * 16 && !buffer_realloc(nodes, total * 16, false)) return;
   if (aabbs->size < total * 24 && !buffer_realloc(aabbs, total * 24, false)) return;

@@ -99,24 +99,47 @@ struct Geom {
   uint32_t icount = 0, istride = 12, ioffset = 0;
 };
 
-// Device storage of one triangle LBVH (layout documented in bvh.hpp).
-struct LBVHStorage {
-  uint32_t n = 0;          // primitives the storage is sized for
-  uint32_t capacity = 0;   // primitives allocated
-  void *block = nullptr;   // one hipMalloc holding every array below
+// Per-mesh descriptor read by the segmented LBVH kernels (device copy in BVHStore::dDesc).
+struct MeshDesc {
+  const char *verts;
+  const char *idx;
+  uint32_t vstride, nverts, istride, ntri;
+  uint32_t primOffset;   // first key / leaf / triangle slot of this mesh
+  uint32_t innerOffset;  // first inner node of this mesh (= primOffset - mesh index)
+  uint32_t tileOffset;   // first tile of this mesh
+  uint32_t ntiles;       // ceil(ntri / tileKeys)
+};
+
+// Device storage for the LBVHs of one or more meshes built together (one accel = one mesh).
+// Every kernel of the build is launched over the tile table, so a batch of thousands of uneven
+// meshes costs the same number of launches as a single mesh. Layout in bvh.hpp.
+struct BVHStore {
+  int refs = 0;
+  uint32_t meshes = 0, prims = 0, inner = 0, tiles = 0;
+  uint32_t primCap = 0, meshCap = 0, tileCap = 0;
+  SortConfig cfg;    // tile size / digit width the arrays were laid out for
+  int passes = 0;
+  void *block = nullptr;
   size_t blockBytes = 0;
-  uint32_t *mcodes = nullptr;     // [n] sorted morton codes
-  uint32_t *ids = nullptr;        // [n] primIDs in sorted order
-  void *nodes = nullptr;          // [n-1] BVHNode (64 B)
-  void *tris = nullptr;           // [n] TriRecord (48 B), leaf order
-  uint32_t *parentLeaf = nullptr; // [n]  parent<<1 | side
-  uint32_t *parentInner = nullptr;// [n-1] parent<<1 | side (root: 0xFFFFFFFF)
-  uint32_t *counters = nullptr;   // [n-1] refit arrival counters
-  float *header = nullptr;        // [16] scene box (0..5), root box (8..13)
-  uint32_t *keysTmp = nullptr;    // [n] sort ping-pong keys
-  uint32_t *idsTmp = nullptr;     // [n] sort ping-pong ids
-  void *sortScratch = nullptr;    // sort histograms / look-back state
-  size_t sortScratchBytes = 0;
+  MeshDesc *dDesc = nullptr;
+  uint32_t *dTileMesh = nullptr;
+  uint32_t *codes = nullptr, *ids = nullptr, *codesTmp = nullptr, *idsTmp = nullptr;
+  void *nodes = nullptr;  // BVHNode[inner]
+  void *tris = nullptr;   // TriRecord[prims]
+  uint32_t *parentLeaf = nullptr, *parentInner = nullptr;
+  float *headers = nullptr;  // [meshes][16]
+  // Zeroed at the start of every build (one contiguous range):
+  uint32_t *zeroBase = nullptr;
+  size_t zeroWords = 0;
+  uint32_t *counters = nullptr;  // [inner]
+  uint32_t *hist = nullptr;      // [meshes][passes][R]
+  uint32_t *status = nullptr;    // [passes][tiles][R]
+  uint32_t *tickets = nullptr;   // [passes]
+  // Host copies of the plan (to detect when a cached graph can be replayed).
+  std::vector<MeshDesc> desc;
+  std::vector<uint32_t> tileMesh;
+  hipGraphExec_t graph = nullptr;
+  bool graphValid = false;
 };
 
 struct Accel {
@@ -125,7 +148,13 @@ struct Accel {
   unsigned flags = 0;
   GPRTBuildMode mode = GPRT_BUILD_MODE_UNINITIALIZED;
   bool built = false;
-  LBVHStorage bvh;
+  BVHStore *store = nullptr;  // shared by every accel of one batch build
+  uint32_t mesh = 0;          // this accel's index in the store
+  uint32_t n = 0;             // triangles
+  // Views into the store for this mesh (used by trace / export / update).
+  void *nodes = nullptr;
+  void *tris = nullptr;
+  float *header = nullptr;
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -165,10 +194,12 @@ bool sort_run(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt
 // Fill a device range with a 32-bit pattern (graph-safe; no hipMemset nodes).
 void fill_u32(Context *ctx, hipStream_t s, void *dst, uint32_t value, size_t words);
 
-// LBVH
-bool lbvh_reserve(Context *ctx, LBVHStorage &st, uint32_t n);
-void lbvh_release(LBVHStorage &st);
-bool lbvh_build(Context *ctx, hipStream_t s, Accel *a);
+// LBVH (segmented: one store may hold many meshes)
+void store_release(BVHStore *st);
+void accel_detach(Accel *a);
+// Builds every accel in `accels` into one store with one launch sequence (captured into a
+// hipGraph and replayed when `useGraph` and the batch shape is unchanged).
+bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph);
 bool lbvh_refit(Context *ctx, hipStream_t s, Accel *a);
 bool lbvh_export(Context *ctx, hipStream_t s, Accel *a, int32_t *nodes, float *aabbs, uint64_t *keys);
 

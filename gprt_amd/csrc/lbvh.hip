@@ -2,19 +2,30 @@
 // bottom-up refit, following the stage order of GPRT's development LBVH
 // (gprt/development/gprt_lbvh.h:170-195; kernels gprt/development/gprt_lbvh.hlsl:215-428).
 //
-// Differences in form (results are the same binary radix tree and the same exact boxes):
-//  * Scene bounds: wave/block min-max reduction, then one ordered-uint atomic per block and
-//    axis, instead of six float CAS atomics per triangle on one address set (hlsl:215-226).
-//  * Keys: the reference sorts 32-bit Morton codes with primID payload; ties are broken by the
-//    primitive index (hlsl:56-58). Here keys are defined as morton30<<32 | primID. Because the
-//    primIDs enter the sort in ascending order, a stable sort of the 30 Morton bits with primID
-//    payload yields exactly the order of the 64-bit keys, at 4 passes of 8 bits instead of 8.
-//  * Karras: delta(i,j) = clz64(key_i ^ key_j), -1 outside [0, N-1] (hlsl:45-59); the split
+// Every kernel is launched over a tile table that covers one or many meshes: tile t belongs to
+// mesh tileMesh[t] and covers tileKeys consecutive primitives of it. A batch of thousands of
+// uneven meshes is therefore built by the same ~10 launches as a single mesh, and the sequence
+// can be captured once into a hipGraph and replayed.
+//
+// Differences in form from the reference (results: the same binary radix tree, the same boxes):
+//  * Scene bounds: per-tile wave/workgroup min-max reduction, then one ordered-uint atomic per
+//    tile and axis, instead of six float CAS atomics per triangle on one address set
+//    (hlsl:215-226).
+//  * Keys are defined as morton30<<32 | primID. The primIDs enter the sort in ascending order,
+//    so a stable sort of the 30 Morton bits with primID payload gives exactly the order of the
+//    64-bit keys, in 4 passes of 8 bits (u32 key + u32 payload) instead of 8 passes over u64.
+//  * The sort is a segmented one-sweep radix sort: per-mesh digit histograms are built by the
+//    key kernel, tiles never cross a mesh, the decoupled look-back stops at the mesh's first tile,
+//    and a mesh that fits in one tile is sorted entirely in LDS with no look-back at all.
+//  * Karras: delta(i,j) = clz64(key_i ^ key_j), -1 outside the mesh (hlsl:45-59); the split
 //    search is the binary search of hlsl:84-97 without its trailing t=1 iterations (they cannot
 //    move s once the search has converged).
 //  * Refit: one thread per leaf climbs while it is the second child to arrive at a node (atomic
-//    arrival counter, acquire/release at device scope), so every node is combined exactly once:
-//    O(N) work, versus every leaf doing six CAS atomics on every ancestor (hlsl:400-428).
+//    arrival counter, acquire/release at device scope): every node is combined exactly once,
+//    O(N), versus every leaf doing six CAS atomics on every ancestor (hlsl:400-428).
+#include <algorithm>
+#include <cstring>
+
 #include "bvh.hpp"
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -23,34 +34,59 @@ namespace gprt {
 
 namespace {
 
-struct GeomView {
-  const char *verts;
-  uint32_t vstride;
-  const char *idx;
-  uint32_t istride;
-  uint32_t nverts;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+constexpr int kMortonBits = 30;
+constexpr int kThreads = 256;  // non-sort kernels
+
+struct BuildArgs {
+  const MeshDesc *desc;
+  const uint32_t *tileMesh;
+  uint32_t tileKeys;
+  uint32_t tiles;
+  uint32_t *codes, *ids, *codesTmp, *idsTmp;
+  BVHNode *nodes;
+  TriRecord *tris;
+  uint32_t *parentLeaf, *parentInner, *counters;
+  float *headers;
+  uint32_t *hist, *status, *tickets;
+  int passes;
+  int digitBits;
 };
+
+struct TileRange {
+  uint32_t mesh, lt, lo, hi;
+};
+
+__device__ __forceinline__ TileRange tile_range(const BuildArgs &a, const MeshDesc &d, uint32_t t, uint32_t m) {
+  TileRange r;
+  r.mesh = m;
+  r.lt = t - d.tileOffset;
+  r.lo = r.lt * a.tileKeys;
+  r.hi = min(r.lo + a.tileKeys, d.ntri);
+  return r;
+}
 
 // Vertex indices are clamped to the vertex count, so a malformed index buffer yields a wrong
 // box rather than an out-of-bounds read.
-__device__ __forceinline__ float3 load_vertex(const GeomView &g, uint32_t i) {
-  i = i < g.nverts ? i : g.nverts - 1;
-  const float *p = (const float *)(g.verts + (size_t)i * g.vstride);
+__device__ __forceinline__ float3 load_vertex(const MeshDesc &d, uint32_t i) {
+  i = i < d.nverts ? i : d.nverts - 1;
+  const float *p = (const float *)(d.verts + (size_t)i * d.vstride);
   return make_float3(p[0], p[1], p[2]);
 }
 
-__device__ __forceinline__ uint3 load_tri(const GeomView &g, uint32_t prim) {
-  const uint32_t *p = (const uint32_t *)(g.idx + (size_t)prim * g.istride);
-  return make_uint3(p[0], p[1], p[2]);
+// primIDs are clamped to the mesh as well, so no gather can leave the caller's buffers.
+__device__ __forceinline__ void load_triangle(const MeshDesc &d, uint32_t prim, float3 &a, float3 &b, float3 &c) {
+  prim = prim < d.ntri ? prim : d.ntri - 1;
+  const uint32_t *p = (const uint32_t *)(d.idx + (size_t)prim * d.istride);
+  a = load_vertex(d, p[0]);
+  b = load_vertex(d, p[1]);
+  c = load_vertex(d, p[2]);
 }
 
-// Zero refit counters; set the scene box accumulators to the identities of min/max.
-__global__ void __launch_bounds__(256) k_build_init(uint32_t *__restrict__ counters, uint32_t nInner,
-                                                    uint32_t *__restrict__ header) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  if (i < 16) header[i] = (i < 3) ? 0xFFFFFFFFu : 0u;
-  for (; i < nInner; i += stride) counters[i] = 0u;
+__device__ __forceinline__ float unord(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -64,131 +100,369 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Scene box over all vertices of all triangles (gprt_lbvh.hlsl:121-131, :215-226).
-__global__ void __launch_bounds__(256) k_bounds(GeomView g, uint32_t n, uint32_t *__restrict__ header) {
-  float mn[3] = {INFINITY, INFINITY, INFINITY};
-  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+// ------------------------------------------------------------------------------------------------
+// Init: zero counters / histograms / look-back state; box accumulators to min/max identities.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_init(uint32_t *__restrict__ zero, size_t words, float *__restrict__ headers,
+                                              uint32_t meshes) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    const uint3 t = load_tri(g, (uint32_t)i);
-    const float3 a = load_vertex(g, t.x), b = load_vertex(g, t.y), c = load_vertex(g, t.z);
-    mn[0] = fmin_ord(mn[0], fmin_ord(a.x, fmin_ord(b.x, c.x)));
-    mn[1] = fmin_ord(mn[1], fmin_ord(a.y, fmin_ord(b.y, c.y)));
-    mn[2] = fmin_ord(mn[2], fmin_ord(a.z, fmin_ord(b.z, c.z)));
-    mx[0] = fmax_ord(mx[0], fmax_ord(a.x, fmax_ord(b.x, c.x)));
-    mx[1] = fmax_ord(mx[1], fmax_ord(a.y, fmax_ord(b.y, c.y)));
-    mx[2] = fmax_ord(mx[2], fmax_ord(a.z, fmax_ord(b.z, c.z)));
+  for (size_t j = i; j < (size_t)meshes * 16; j += stride) {
+    const uint32_t q = (uint32_t)(j & 15);
+    ((uint32_t *)headers)[j] = q < 3 ? 0xFFFFFFFFu : 0u;
   }
-  __shared__ float s[4][6];
+  for (; i < words; i += stride) zero[i] = 0u;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Scene box per mesh over all vertices of its triangles (gprt_lbvh.hlsl:121-131, :215-226).
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_bounds(BuildArgs a, uint32_t tileBase) {
+  const uint32_t t = tileBase + blockIdx.x;
+  const uint32_t m = a.tileMesh[t];
+  const MeshDesc d = a.desc[m];
+  const TileRange r = tile_range(a, d, t, m);
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kThreads) {
+    float3 p, q, s;
+    load_triangle(d, i, p, q, s);
+    mn[0] = fmin_ord(mn[0], fmin_ord(p.x, fmin_ord(q.x, s.x)));
+    mn[1] = fmin_ord(mn[1], fmin_ord(p.y, fmin_ord(q.y, s.y)));
+    mn[2] = fmin_ord(mn[2], fmin_ord(p.z, fmin_ord(q.z, s.z)));
+    mx[0] = fmax_ord(mx[0], fmax_ord(p.x, fmax_ord(q.x, s.x)));
+    mx[1] = fmax_ord(mx[1], fmax_ord(p.y, fmax_ord(q.y, s.y)));
+    mx[2] = fmax_ord(mx[2], fmax_ord(p.z, fmax_ord(q.z, s.z)));
+  }
+  __shared__ float sb[kThreads / 64][6];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    mn[a] = wave_min(mn[a]);
-    mx[a] = wave_max(mx[a]);
+  for (int c = 0; c < 3; ++c) {
+    mn[c] = wave_min(mn[c]);
+    mx[c] = wave_max(mx[c]);
   }
   if (l == 0) {
-    for (int a = 0; a < 3; ++a) {
-      s[w][a] = mn[a];
-      s[w][3 + a] = mx[a];
+    for (int c = 0; c < 3; ++c) {
+      sb[w][c] = mn[c];
+      sb[w][3 + c] = mx[c];
     }
   }
   __syncthreads();
   if (threadIdx.x < 6) {
-    const int a = threadIdx.x;
-    float v = s[0][a];
-    for (int ww = 1; ww < 4; ++ww) v = (a < 3) ? fmin_ord(v, s[ww][a]) : fmax_ord(v, s[ww][a]);
-    if (a < 3)
-      atomicMin(&header[a], ford(v));
+    const int c = threadIdx.x;
+    float v = sb[0][c];
+    for (int ww = 1; ww < kThreads / 64; ++ww) v = c < 3 ? fmin_ord(v, sb[ww][c]) : fmax_ord(v, sb[ww][c]);
+    uint32_t *h = (uint32_t *)a.headers + (size_t)m * 16;
+    if (c < 3)
+      atomicMin(&h[c], ford(v));
     else
-      atomicMax(&header[a], ford(v));
+      atomicMax(&h[c], ford(v));
   }
 }
 
-__device__ __forceinline__ float unord(uint32_t u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+// ------------------------------------------------------------------------------------------------
+// Morton keys of triangle centroids (gprt_lbvh.hlsl:139-146, :262-277) + per-mesh digit
+// histograms of every sort pass (only for meshes spanning more than one tile).
+// ------------------------------------------------------------------------------------------------
+template <int BITS>
+__global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBase) {
+  constexpr int R = 1 << BITS;
+  extern __shared__ uint32_t s_h[];  // [passes][R]
+  const uint32_t t = tileBase + blockIdx.x;
+  const uint32_t m = a.tileMesh[t];
+  const MeshDesc d = a.desc[m];
+  const TileRange r = tile_range(a, d, t, m);
+  const bool multi = d.ntiles > 1;
+  if (multi) {
+    for (int i = threadIdx.x; i < a.passes * R; i += kThreads) s_h[i] = 0;
+    __syncthreads();
+  }
+  const uint32_t *h = (const uint32_t *)a.headers + (size_t)m * 16;
+  const float lox = unord(h[0]), loy = unord(h[1]), loz = unord(h[2]);
+  const float ex = unord(h[3]) - lox, ey = unord(h[4]) - loy, ez = unord(h[5]) - loz;
+  for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kThreads) {
+    float3 p, q, s;
+    load_triangle(d, i, p, q, s);
+    const float cx = (p.x + q.x + s.x) / 3.0f;
+    const float cy = (p.y + q.y + s.y) / 3.0f;
+    const float cz = (p.z + q.z + s.z) / 3.0f;
+    const float qx = fminf(fmaxf((cx - lox) / ex * 1024.0f, 0.0f), 1023.0f);
+    const float qy = fminf(fmaxf((cy - loy) / ey * 1024.0f, 0.0f), 1023.0f);
+    const float qz = fminf(fmaxf((cz - loz) / ez * 1024.0f, 0.0f), 1023.0f);
+    const uint32_t code = morton30((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
+    a.codes[d.primOffset + i] = code;
+    a.ids[d.primOffset + i] = i;
+    if (multi) {
+      for (int ps = 0; ps < a.passes; ++ps) {
+        const int shift = ps * BITS;
+        const int bits = min(BITS, kMortonBits - shift);
+        atomicAdd(&s_h[ps * R + ((code >> shift) & ((1u << bits) - 1u))], 1u);
+      }
+    }
+  }
+  if (multi) {
+    __syncthreads();
+    uint32_t *gh = a.hist + (size_t)m * a.passes * R;
+    for (int i = threadIdx.x; i < a.passes * R; i += kThreads) {
+      const uint32_t c = s_h[i];
+      if (c) atomicAdd(&gh[i], c);
+    }
+  }
 }
 
-// Morton keys of triangle centroids (gprt_lbvh.hlsl:139-146, :262-277). Keys are written as the
-// 30-bit code with the primID as payload; ids enter the sort in ascending order.
-__global__ void __launch_bounds__(256) k_keys(GeomView g, uint32_t n, const uint32_t *__restrict__ header,
-                                              uint32_t *__restrict__ codes, uint32_t *__restrict__ ids) {
-  const float lox = unord(header[0]), loy = unord(header[1]), loz = unord(header[2]);
-  const float hix = unord(header[3]), hiy = unord(header[4]), hiz = unord(header[5]);
-  const float ex = hix - lox, ey = hiy - loy, ez = hiz - loz;
+// Exclusive scan of every pass's per-mesh histogram (meshes spanning more than one tile).
+template <int BITS>
+__global__ void __launch_bounds__(256) k_hist_scan(BuildArgs a) {
+  constexpr int R = 1 << BITS;
+  const uint32_t m = blockIdx.x;
+  if (a.desc[m].ntiles <= 1) return;
+  __shared__ uint32_t s_w[4];
+  const int tid = threadIdx.x;
+  uint32_t *gh = a.hist + (size_t)m * a.passes * R;
+  for (int p = 0; p < a.passes; ++p) {
+    const uint32_t c = tid < R ? gh[p * R + tid] : 0u;
+    const uint32_t incl = wave_incl_sum(c);
+    if ((tid & 63) == 63) s_w[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t off = incl - c;
+    for (int w = 0; w < (tid >> 6); ++w) off += s_w[w];
+    if (tid < R) gh[p * R + tid] = off;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Segmented one-sweep pass over (u32 Morton code, u32 primID).
+// ------------------------------------------------------------------------------------------------
+template <int BITS, int THREADS, int ITEMS>
+__global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
+  constexpr int R = 1 << BITS;
+  constexpr int W = THREADS / 64;
+  constexpr int TILE = THREADS * ITEMS;
+  constexpr int DPT = (R + THREADS - 1) / THREADS;
+  __shared__ __attribute__((aligned(16))) uint32_t s_buf[TILE];
+  __shared__ uint32_t s_whist[W][R];
+  __shared__ uint32_t s_tdigit[R];
+  __shared__ uint32_t s_gbase[R];
+  __shared__ uint32_t s_wtot[W];
+  __shared__ uint32_t s_tile;
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  const int l = tid & 63;
+  const int shift = pass * BITS;
+  const int bits = min(BITS, kMortonBits - shift);
+  const uint32_t dmask = (1u << bits) - 1u;
+  const uint32_t *__restrict__ src = (pass & 1) ? a.codesTmp : a.codes;
+  uint32_t *__restrict__ dst = (pass & 1) ? a.codes : a.codesTmp;
+  const uint32_t *__restrict__ vsrc = (pass & 1) ? a.idsTmp : a.ids;
+  uint32_t *__restrict__ vdst = (pass & 1) ? a.ids : a.idsTmp;
+
+  if (tid == 0) s_tile = ticket(&a.tickets[pass]);
+  for (int i = tid; i < W * R; i += THREADS) (&s_whist[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t t = s_tile;
+  const uint32_t m = a.tileMesh[t];
+  const MeshDesc d = a.desc[m];
+  const uint32_t lt = t - d.tileOffset;
+  const uint32_t lo = lt * TILE;
+  const uint32_t valid = min((uint32_t)TILE, d.ntri - lo);
+  const uint32_t base = d.primOffset + lo;
+  const uint32_t segEnd = d.primOffset + d.ntri;
+  const bool single = d.ntiles == 1;
+
+  uint32_t k[ITEMS], dg[ITEMS], pos[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t q = (uint32_t)(w * ITEMS * 64 + j * 64 + l);
+    k[j] = q < valid ? src[base + q] : 0xFFFFFFFFu;
+    dg[j] = (k[j] >> shift) & dmask;
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t mm = ~0ull;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const bool bit = (dg[j] >> b) & 1u;
+      const uint64_t bb = wave_ballot(bit);
+      mm &= bit ? bb : ~bb;
+    }
+    const uint32_t rnk = mask_rank(mm);
+    const uint32_t c = (uint32_t)__popcll(mm);
+    const uint32_t pre = s_whist[w][dg[j]];
+    if (rnk == 0) s_whist[w][dg[j]] = pre + c;
+    pos[j] = pre + rnk;
+  }
+  __syncthreads();
+
+  uint32_t cnt[DPT];
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int dd = tid * DPT + q;
+    cnt[q] = 0;
+    if (dd < R) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (int ww = 0; ww < W; ++ww) {
+        const uint32_t c = s_whist[ww][dd];
+        s_whist[ww][dd] = sum;
+        sum += c;
+      }
+      cnt[q] = sum;
+      if (!single)
+        st_agent(&a.status[((size_t)pass * a.tiles + t) * R + dd], (lt == 0 ? kFlagInc : kFlagAgg) | sum);
+    }
+  }
+  {
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) tsum += cnt[q];
+    const uint32_t incl = wave_incl_sum(tsum);
+    if (l == 63) s_wtot[w] = incl;
+    __syncthreads();
+    uint32_t off = incl - tsum;
+    for (int ww = 0; ww < w; ++ww) off += s_wtot[ww];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int dd = tid * DPT + q;
+      if (dd < R) s_tdigit[dd] = off;
+      off += cnt[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int dd = tid * DPT + q;
+    if (dd >= R) continue;
+    if (single) {
+      s_gbase[dd] = base;
+      continue;
+    }
+    uint32_t excl = 0;
+    if (lt > 0) {
+      int64_t tt = (int64_t)t - 1;
+      const int64_t first = d.tileOffset;
+      while (tt >= first) {
+        const uint32_t s = ld_agent(&a.status[((size_t)pass * a.tiles + tt) * R + dd]);
+        const uint32_t flag = s & ~kValueMask;
+        if (flag == 0u) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += s & kValueMask;
+        if (flag == kFlagInc) break;
+        --tt;
+      }
+      st_agent(&a.status[((size_t)pass * a.tiles + t) * R + dd], kFlagInc | (excl + cnt[q]));
+    }
+    const uint32_t gstart = a.hist[((size_t)m * a.passes + pass) * R + dd];
+    s_gbase[dd] = d.primOffset + gstart + excl - s_tdigit[dd];
+  }
+  // s_tdigit / s_gbase of every digit must be visible before any lane reads another thread's.
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    pos[j] = s_tdigit[dg[j]] + s_whist[w][dg[j]] + pos[j];
+    s_buf[pos[j]] = k[j];
+  }
+  __syncthreads();
+  uint32_t gaddr[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t i = (uint32_t)(j * THREADS + tid);
+    gaddr[j] = 0xFFFFFFFFu;
+    if (i < valid) {
+      const uint32_t key = s_buf[i];
+      const uint32_t g = s_gbase[(key >> shift) & dmask] + i;
+      if (g >= d.primOffset && g < segEnd) {  // guards the write; always true when consistent
+        dst[g] = key;
+        gaddr[j] = g;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t q = (uint32_t)(w * ITEMS * 64 + j * 64 + l);
+    s_buf[pos[j]] = q < valid ? vsrc[base + q] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t i = (uint32_t)(j * THREADS + tid);
+    if (i < valid && gaddr[j] != 0xFFFFFFFFu) vdst[gaddr[j]] = s_buf[i];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_copy_back(BuildArgs a, uint32_t prims) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    const uint3 t = load_tri(g, (uint32_t)i);
-    const float3 a = load_vertex(g, t.x), b = load_vertex(g, t.y), c = load_vertex(g, t.z);
-    const float cx = (a.x + b.x + c.x) / 3.0f;
-    const float cy = (a.y + b.y + c.y) / 3.0f;
-    const float cz = (a.z + b.z + c.z) / 3.0f;
-    const float px = (cx - lox) / ex, py = (cy - loy) / ey, pz = (cz - loz) / ez;
-    const float qx = fminf(fmaxf(px * 1024.0f, 0.0f), 1023.0f);
-    const float qy = fminf(fmaxf(py * 1024.0f, 0.0f), 1023.0f);
-    const float qz = fminf(fmaxf(pz * 1024.0f, 0.0f), 1023.0f);
-    codes[i] = morton30((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
-    ids[i] = (uint32_t)i;
+  for (; i < prims; i += stride) {
+    a.codes[i] = a.codesTmp[i];
+    a.ids[i] = a.idsTmp[i];
   }
 }
 
-__device__ __forceinline__ uint64_t key_at(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
-                                           int64_t j) {
-  return ((uint64_t)codes[j] << 32) | ids[j];
-}
-
+// ------------------------------------------------------------------------------------------------
+// Karras 2012 split search (gprt_lbvh.hlsl:64-103, :287-338). One thread per inner node.
+// ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ int delta64(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
                                        uint64_t ki, int64_t j, int64_t n) {
   if (j < 0 || j >= n) return -1;
-  return __clzll(ki ^ key_at(codes, ids, j));
+  return __clzll(ki ^ (((uint64_t)codes[j] << 32) | ids[j]));
 }
 
-// Karras 2012 split search (gprt_lbvh.hlsl:64-103, :287-338). One thread per inner node.
-__global__ void __launch_bounds__(256) k_karras(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
-                                                uint32_t n, BVHNode *__restrict__ nodes,
-                                                uint32_t *__restrict__ parentLeaf,
-                                                uint32_t *__restrict__ parentInner) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t N = n;
-  if (i >= N - 1) return;
-  const uint64_t ki = key_at(codes, ids, i);
-  const int d = delta64(codes, ids, ki, i + 1, N) >= delta64(codes, ids, ki, i - 1, N) ? 1 : -1;
-  const int dmin = delta64(codes, ids, ki, i - d, N);
-  int64_t lmax = 2;
-  while (delta64(codes, ids, ki, i + lmax * d, N) > dmin) lmax *= 2;
-  int64_t l = 0;
-  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta64(codes, ids, ki, i + (l + t) * d, N) > dmin) l += t;
-  const int64_t j = i + l * d;
-  const int dnode = delta64(codes, ids, ki, j, N);
-  int64_t s = 0;
-  int64_t t = l;
-  do {
-    t = (t + 1) >> 1;
-    if (delta64(codes, ids, ki, i + (s + t) * d, N) > dnode) s += t;
-  } while (t > 1);
-  const int64_t split = i + s * d + (d < 0 ? -1 : 0);
-  const int64_t first = d > 0 ? i : j;
-  const int64_t last = d > 0 ? j : i;
-  const int64_t left = split, right = split + 1;
-  int c0, c1;
-  if (left == first) {
-    c0 = ~(int)left;
-    parentLeaf[left] = ((uint32_t)i << 1) | 0u;
-  } else {
-    c0 = (int)left;
-    parentInner[left] = ((uint32_t)i << 1) | 0u;
+__global__ void __launch_bounds__(kThreads) k_karras(BuildArgs a, uint32_t tileBase) {
+  const uint32_t t = tileBase + blockIdx.x;
+  const uint32_t m = a.tileMesh[t];
+  const MeshDesc d = a.desc[m];
+  const TileRange r = tile_range(a, d, t, m);
+  const int64_t N = d.ntri;
+  const uint32_t *__restrict__ codes = a.codes + d.primOffset;
+  const uint32_t *__restrict__ ids = a.ids + d.primOffset;
+  BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
+  uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
+  uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
+  const uint32_t hiInner = min(r.hi, d.ntri - 1);
+  for (uint32_t ii = r.lo + threadIdx.x; ii < hiInner; ii += kThreads) {
+    const int64_t i = ii;
+    const uint64_t ki = ((uint64_t)codes[i] << 32) | ids[i];
+    const int dir = delta64(codes, ids, ki, i + 1, N) >= delta64(codes, ids, ki, i - 1, N) ? 1 : -1;
+    const int dmin = delta64(codes, ids, ki, i - dir, N);
+    int64_t lmax = 2;
+    while (delta64(codes, ids, ki, i + lmax * dir, N) > dmin) lmax *= 2;
+    int64_t len = 0;
+    for (int64_t s = lmax >> 1; s >= 1; s >>= 1)
+      if (delta64(codes, ids, ki, i + (len + s) * dir, N) > dmin) len += s;
+    const int64_t j = i + len * dir;
+    const int dnode = delta64(codes, ids, ki, j, N);
+    int64_t s = 0, step = len;
+    do {
+      step = (step + 1) >> 1;
+      if (delta64(codes, ids, ki, i + (s + step) * dir, N) > dnode) s += step;
+    } while (step > 1);
+    const int64_t split = i + s * dir + (dir < 0 ? -1 : 0);
+    const int64_t first = dir > 0 ? i : j;
+    const int64_t last = dir > 0 ? j : i;
+    if (split < first || split >= last) continue;  // only possible with repeated keys
+    int c0, c1;
+    if (split == first) {
+      c0 = ~(int)split;
+      parentLeaf[split] = ((uint32_t)i << 1) | 0u;
+    } else {
+      c0 = (int)split;
+      parentInner[split] = ((uint32_t)i << 1) | 0u;
+    }
+    if (split + 1 == last) {
+      c1 = ~(int)(split + 1);
+      parentLeaf[split + 1] = ((uint32_t)i << 1) | 1u;
+    } else {
+      c1 = (int)(split + 1);
+      parentInner[split + 1] = ((uint32_t)i << 1) | 1u;
+    }
+    nodes[i].d = make_int4(c0, c1, 0, 0);
+    if (i == 0) parentInner[0] = 0xFFFFFFFFu;
   }
-  if (right == last) {
-    c1 = ~(int)right;
-    parentLeaf[right] = ((uint32_t)i << 1) | 1u;
-  } else {
-    c1 = (int)right;
-    parentInner[right] = ((uint32_t)i << 1) | 1u;
-  }
-  nodes[i].d = make_int4(c0, c1, 0, 0);
-  if (i == 0) parentInner[0] = 0xFFFFFFFFu;
 }
 
 __device__ __forceinline__ void store_child_box(BVHNode *nodes, uint32_t node, uint32_t side, const float b[6]) {
@@ -209,60 +483,70 @@ __device__ __forceinline__ void load_child_box(const BVHNode *nodes, uint32_t no
   b[5] = z.y;
 }
 
-// Leaf records + bottom-up refit with arrival counters. One thread per sorted leaf position.
-__global__ void __launch_bounds__(256) k_refit(GeomView g, const uint32_t *__restrict__ ids, uint32_t n,
-                                               BVHNode *__restrict__ nodes, TriRecord *__restrict__ tris,
-                                               const uint32_t *__restrict__ parentLeaf,
-                                               const uint32_t *__restrict__ parentInner,
-                                               uint32_t *__restrict__ counters, float *__restrict__ header) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t prim = ids[k];
-  const uint3 t = load_tri(g, prim);
-  const float3 a = load_vertex(g, t.x), b = load_vertex(g, t.y), c = load_vertex(g, t.z);
-  TriRecord rec;
-  rec.v0 = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
-  rec.v1 = make_float4(b.x, b.y, b.z, 0.0f);
-  rec.v2 = make_float4(c.x, c.y, c.z, 0.0f);
-  tris[k] = rec;
-  float box[6];
-  box[0] = fmin_ord(a.x, fmin_ord(b.x, c.x));
-  box[1] = fmin_ord(a.y, fmin_ord(b.y, c.y));
-  box[2] = fmin_ord(a.z, fmin_ord(b.z, c.z));
-  box[3] = fmax_ord(a.x, fmax_ord(b.x, c.x));
-  box[4] = fmax_ord(a.y, fmax_ord(b.y, c.y));
-  box[5] = fmax_ord(a.z, fmax_ord(b.z, c.z));
-  if (n == 1) {
-    for (int q = 0; q < 6; ++q) header[8 + q] = box[q];
-    return;
-  }
-  uint32_t link = parentLeaf[k];
-  // A radix tree over distinct 64-bit keys is at most 65 levels deep; the bound and the index
-  // checks only guard against a malformed topology.
-  for (int level = 0; level < 72; ++level) {
-    const uint32_t node = link >> 1, side = link & 1u;
-    if (node >= n - 1) return;
-    store_child_box(nodes, node, side, box);
-    const uint32_t old =
-        __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old != 1u) return;  // first arrival: the sibling's thread finishes this node
-    float sib[6];
-    load_child_box(nodes, node, side ^ 1u, sib);
-    box[0] = fmin_ord(box[0], sib[0]);
-    box[1] = fmin_ord(box[1], sib[1]);
-    box[2] = fmin_ord(box[2], sib[2]);
-    box[3] = fmax_ord(box[3], sib[3]);
-    box[4] = fmax_ord(box[4], sib[4]);
-    box[5] = fmax_ord(box[5], sib[5]);
-    link = parentInner[node];
-    if (link == 0xFFFFFFFFu) {
-      for (int q = 0; q < 6; ++q) header[8 + q] = box[q];
-      return;
+// ------------------------------------------------------------------------------------------------
+// Leaf records + bottom-up refit with arrival counters. One thread per leaf.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_refit(BuildArgs a, uint32_t tileBase) {
+  const uint32_t t = tileBase + blockIdx.x;
+  const uint32_t m = a.tileMesh[t];
+  const MeshDesc d = a.desc[m];
+  const TileRange r = tile_range(a, d, t, m);
+  const uint32_t n = d.ntri;
+  BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
+  const uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
+  const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
+  uint32_t *__restrict__ counters = a.counters + d.innerOffset;
+  float *__restrict__ header = a.headers + (size_t)m * 16;
+  for (uint32_t k = r.lo + threadIdx.x; k < r.hi; k += kThreads) {
+    const uint32_t prim = a.ids[d.primOffset + k];
+    float3 p, q, s;
+    load_triangle(d, prim, p, q, s);
+    TriRecord rec;
+    rec.v0 = make_float4(p.x, p.y, p.z, __uint_as_float(prim));
+    rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
+    rec.v2 = make_float4(s.x, s.y, s.z, 0.0f);
+    a.tris[d.primOffset + k] = rec;
+    float box[6];
+    box[0] = fmin_ord(p.x, fmin_ord(q.x, s.x));
+    box[1] = fmin_ord(p.y, fmin_ord(q.y, s.y));
+    box[2] = fmin_ord(p.z, fmin_ord(q.z, s.z));
+    box[3] = fmax_ord(p.x, fmax_ord(q.x, s.x));
+    box[4] = fmax_ord(p.y, fmax_ord(q.y, s.y));
+    box[5] = fmax_ord(p.z, fmax_ord(q.z, s.z));
+    if (n == 1) {
+      for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
+      continue;
+    }
+    uint32_t link = parentLeaf[k];
+    // A radix tree over distinct 64-bit keys is at most 65 levels deep; the bound and the index
+    // checks only guard against a malformed topology.
+    for (int level = 0; level < 72; ++level) {
+      const uint32_t node = link >> 1, side = link & 1u;
+      if (node >= n - 1) break;
+      store_child_box(nodes, node, side, box);
+      const uint32_t old =
+          __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (old != 1u) break;  // first arrival: the sibling's thread finishes this node
+      float sib[6];
+      load_child_box(nodes, node, side ^ 1u, sib);
+      box[0] = fmin_ord(box[0], sib[0]);
+      box[1] = fmin_ord(box[1], sib[1]);
+      box[2] = fmin_ord(box[2], sib[2]);
+      box[3] = fmax_ord(box[3], sib[3]);
+      box[4] = fmax_ord(box[4], sib[4]);
+      box[5] = fmax_ord(box[5], sib[5]);
+      link = parentInner[node];
+      if (link == 0xFFFFFFFFu) {
+        for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
+        break;
+      }
     }
   }
 }
 
-// Reference-layout export (gprt/development/gprt_lbvh.h:28-55).
+// ------------------------------------------------------------------------------------------------
+// Reference-layout export of one mesh (gprt/development/gprt_lbvh.h:28-55).
+// ------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_export(const BVHNode *__restrict__ nodes, const uint32_t *__restrict__ codes,
                                                 const uint32_t *__restrict__ ids,
                                                 const uint32_t *__restrict__ parentLeaf,
@@ -307,16 +591,9 @@ __global__ void __launch_bounds__(256) k_export(const BVHNode *__restrict__ node
   for (int q = 0; q < 6; ++q) outBoxes[(size_t)x * 6 + q] = b[q];
 }
 
-GeomView geom_view(Geom *g) {
-  GeomView v;
-  v.verts = (const char *)g->vbuf->d + g->voffset;
-  v.vstride = g->vstride;
-  v.idx = (const char *)g->ibuf->d + g->ioffset;
-  v.istride = g->istride;
-  v.nverts = g->vcount;
-  return v;
-}
-
+// ------------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------------
 unsigned grid_for(size_t n, unsigned block, unsigned cap) {
   size_t b = (n + block - 1) / block;
   if (b > cap) b = cap;
@@ -324,114 +601,311 @@ unsigned grid_for(size_t n, unsigned block, unsigned cap) {
   return (unsigned)b;
 }
 
+BuildArgs args_of(BVHStore *st) {
+  BuildArgs a;
+  a.desc = st->dDesc;
+  a.tileMesh = st->dTileMesh;
+  a.tileKeys = (uint32_t)st->cfg.tileKeys;
+  a.tiles = st->tiles;
+  a.codes = st->codes;
+  a.ids = st->ids;
+  a.codesTmp = st->codesTmp;
+  a.idsTmp = st->idsTmp;
+  a.nodes = (BVHNode *)st->nodes;
+  a.tris = (TriRecord *)st->tris;
+  a.parentLeaf = st->parentLeaf;
+  a.parentInner = st->parentInner;
+  a.counters = st->counters;
+  a.headers = st->headers;
+  a.hist = st->hist;
+  a.status = st->status;
+  a.tickets = st->tickets;
+  a.passes = st->passes;
+  a.digitBits = st->cfg.digitBits;
+  return a;
+}
+
+template <int BITS>
+void launch_sort_passes(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a) {
+  for (int p = 0; p < st->passes; ++p) {
+    if (st->cfg.tileKeys == 2048)
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 8>, dim3(st->tiles), dim3(256), 0, a, p);
+    else if (st->cfg.tileKeys == 4096)
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 16>, dim3(st->tiles), dim3(256), 0, a, p);
+    else
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 512, 16>, dim3(st->tiles), dim3(512), 0, a, p);
+  }
+}
+
+// The whole build of every mesh in the store: no host synchronisation inside, so it can be
+// captured into a hipGraph.
+void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
+  const BuildArgs a = args_of(st);
+  const int R = 1 << st->cfg.digitBits;
+  launch(ctx, s, "lbvh_init", k_init, dim3(grid_for(std::max<size_t>(st->zeroWords, st->meshes * 16ull), 256, 4096)),
+         dim3(256), 0, st->zeroBase, st->zeroWords, st->headers, st->meshes);
+  launch(ctx, s, "lbvh_bounds", k_bounds, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
+  const size_t histLds = (size_t)st->passes * R * 4;
+  switch (st->cfg.digitBits) {
+    case 4:
+      launch(ctx, s, "lbvh_keys", k_keys<4>, dim3(st->tiles), dim3(kThreads), histLds, a, 0u);
+      launch(ctx, s, "lbvh_hist_scan", k_hist_scan<4>, dim3(st->meshes), dim3(256), 0, a);
+      launch_sort_passes<4>(ctx, s, st, a);
+      break;
+    case 6:
+      launch(ctx, s, "lbvh_keys", k_keys<6>, dim3(st->tiles), dim3(kThreads), histLds, a, 0u);
+      launch(ctx, s, "lbvh_hist_scan", k_hist_scan<6>, dim3(st->meshes), dim3(256), 0, a);
+      launch_sort_passes<6>(ctx, s, st, a);
+      break;
+    default:
+      launch(ctx, s, "lbvh_keys", k_keys<8>, dim3(st->tiles), dim3(kThreads), histLds, a, 0u);
+      launch(ctx, s, "lbvh_hist_scan", k_hist_scan<8>, dim3(st->meshes), dim3(256), 0, a);
+      launch_sort_passes<8>(ctx, s, st, a);
+      break;
+  }
+  if (st->passes & 1)
+    launch(ctx, s, "lbvh_copy_back", k_copy_back, dim3(grid_for(st->prims, 256, 8192)), dim3(256), 0, a, st->prims);
+  launch(ctx, s, "lbvh_karras", k_karras, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
+  launch(ctx, s, "lbvh_refit", k_refit, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
+}
+
+bool same_plan(const BVHStore *st, const std::vector<MeshDesc> &desc, const SortConfig &cfg) {
+  if (st->cfg.digitBits != cfg.digitBits || st->cfg.tileKeys != cfg.tileKeys) return false;
+  if (st->desc.size() != desc.size()) return false;
+  return std::memcmp(st->desc.data(), desc.data(), desc.size() * sizeof(MeshDesc)) == 0;
+}
+
+void free_block(BVHStore *st) {
+  if (st->graph) {
+    hipGraphExecDestroy(st->graph);
+    st->graph = nullptr;
+  }
+  st->graphValid = false;
+  if (st->block) hipFree(st->block);
+  st->block = nullptr;
+  st->blockBytes = 0;
+}
+
+// Lays the arrays of `st` out for the given plan (reusing the allocation when it is big enough).
+bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::vector<uint32_t> &tileMesh,
+                  const SortConfig &cfg, uint32_t prims) {
+  const uint32_t M = (uint32_t)desc.size(), T = (uint32_t)tileMesh.size();
+  const uint32_t I = prims - M;  // every mesh has ntri - 1 inner nodes
+  const int R = 1 << cfg.digitBits;
+  const int passes = (kMortonBits + cfg.digitBits - 1) / cfg.digitBits;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t oDesc = off;
+  off += al((size_t)M * sizeof(MeshDesc));
+  const size_t oTile = off;
+  off += al((size_t)T * 4);
+  const size_t oCodes = off;
+  off += al((size_t)prims * 4);
+  const size_t oIds = off;
+  off += al((size_t)prims * 4);
+  const size_t oCodesT = off;
+  off += al((size_t)prims * 4);
+  const size_t oIdsT = off;
+  off += al((size_t)prims * 4);
+  const size_t oNodes = off;
+  off += al((size_t)std::max<uint32_t>(I, 1) * sizeof(BVHNode));
+  const size_t oTris = off;
+  off += al((size_t)prims * sizeof(TriRecord));
+  const size_t oPL = off;
+  off += al((size_t)prims * 4);
+  const size_t oPI = off;
+  off += al((size_t)std::max<uint32_t>(I, 1) * 4);
+  const size_t oHdr = off;
+  off += al((size_t)M * 16 * 4);
+  const size_t oZero = off;
+  const size_t cntWords = std::max<uint32_t>(I, 1);
+  const size_t histWords = (size_t)M * passes * R;
+  const size_t statWords = (size_t)passes * T * R;
+  const size_t tickWords = 64;
+  const size_t zeroWords = cntWords + histWords + statWords + tickWords;
+  off += al(zeroWords * 4);
+  if (off > st->blockBytes) {
+    free_block(st);
+    if (!hip_ok(hipMalloc(&st->block, off), "hipMalloc(lbvh)")) return false;
+    st->blockBytes = off;
+  }
+  char *b = (char *)st->block;
+  st->meshes = M;
+  st->prims = prims;
+  st->inner = I;
+  st->tiles = T;
+  st->cfg = cfg;
+  st->passes = passes;
+  st->dDesc = (MeshDesc *)(b + oDesc);
+  st->dTileMesh = (uint32_t *)(b + oTile);
+  st->codes = (uint32_t *)(b + oCodes);
+  st->ids = (uint32_t *)(b + oIds);
+  st->codesTmp = (uint32_t *)(b + oCodesT);
+  st->idsTmp = (uint32_t *)(b + oIdsT);
+  st->nodes = b + oNodes;
+  st->tris = b + oTris;
+  st->parentLeaf = (uint32_t *)(b + oPL);
+  st->parentInner = (uint32_t *)(b + oPI);
+  st->headers = (float *)(b + oHdr);
+  st->zeroBase = (uint32_t *)(b + oZero);
+  st->zeroWords = zeroWords;
+  st->counters = st->zeroBase;
+  st->hist = st->counters + cntWords;
+  st->status = st->hist + histWords;
+  st->tickets = st->status + statWords;
+  st->desc = desc;
+  st->tileMesh = tileMesh;
+  return true;
+}
+
+void set_views(Accel *a) {
+  BVHStore *st = a->store;
+  const MeshDesc &d = st->desc[a->mesh];
+  a->n = d.ntri;
+  a->nodes = (char *)st->nodes + (size_t)d.innerOffset * sizeof(BVHNode);
+  a->tris = (char *)st->tris + (size_t)d.primOffset * sizeof(TriRecord);
+  a->header = st->headers + (size_t)a->mesh * 16;
+}
+
 }  // namespace
 
-bool lbvh_reserve(Context *ctx, LBVHStorage &st, uint32_t n) {
+void store_release(BVHStore *st) {
+  if (!st) return;
+  free_block(st);
+  delete st;
+}
+
+void accel_detach(Accel *a) {
+  if (!a->store) return;
+  BVHStore *st = a->store;
+  a->store = nullptr;
+  a->built = false;
+  if (--st->refs == 0) {
+    hipStreamSynchronize(a->ctx->stream);
+    store_release(st);
+  }
+}
+
+bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph) {
   const SortConfig cfg = ctx->sortCfg;
-  const size_t nn0 = n ? n : 1;
-  if (n <= st.capacity && st.block && sort_scratch_bytes(nn0, 32, 32, 0, 30, cfg) <= st.sortScratchBytes) {
-    st.n = n;
-    return true;
+  const uint32_t tileKeys = (uint32_t)cfg.tileKeys;
+  std::vector<MeshDesc> desc(count);
+  std::vector<uint32_t> tileMesh;
+  uint64_t prims = 0, tiles = 0;
+  for (uint32_t m = 0; m < count; ++m) {
+    Geom *g = accels[m]->geom;
+    MeshDesc &d = desc[m];
+    std::memset(&d, 0, sizeof(d));
+    d.verts = (const char *)g->vbuf->d + g->voffset;
+    d.idx = (const char *)g->ibuf->d + g->ioffset;
+    d.vstride = g->vstride;
+    d.nverts = g->vcount;
+    d.istride = g->istride;
+    d.ntri = g->icount;
+    d.primOffset = (uint32_t)prims;
+    d.innerOffset = (uint32_t)(prims - m);
+    d.tileOffset = (uint32_t)tiles;
+    d.ntiles = (g->icount + tileKeys - 1) / tileKeys;
+    prims += g->icount;
+    tiles += d.ntiles;
   }
-  lbvh_release(st);
-  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t nn = n ? n : 1;
-  const size_t inner = nn > 1 ? nn - 1 : 1;
-  const size_t sortBytes = sort_scratch_bytes(nn, 32, 32, 0, 30, cfg);
-  size_t off = 0;
-  const size_t oCodes = off;
-  off += al(nn * 4);
-  const size_t oIds = off;
-  off += al(nn * 4);
-  const size_t oNodes = off;
-  off += al(inner * sizeof(BVHNode));
-  const size_t oTris = off;
-  off += al(nn * sizeof(TriRecord));
-  const size_t oPL = off;
-  off += al(nn * 4);
-  const size_t oPI = off;
-  off += al(inner * 4);
-  const size_t oCnt = off;
-  off += al(inner * 4);
-  const size_t oHdr = off;
-  off += al(kHeaderFloats * 4);
-  const size_t oKT = off;
-  off += al(nn * 4);
-  const size_t oIT = off;
-  off += al(nn * 4);
-  const size_t oSS = off;
-  off += al(sortBytes);
-  if (!hip_ok(hipMalloc(&st.block, off), "hipMalloc(lbvh)")) return false;
-  char *b = (char *)st.block;
-  st.blockBytes = off;
-  st.capacity = (uint32_t)nn;
-  st.mcodes = (uint32_t *)(b + oCodes);
-  st.ids = (uint32_t *)(b + oIds);
-  st.nodes = b + oNodes;
-  st.tris = b + oTris;
-  st.parentLeaf = (uint32_t *)(b + oPL);
-  st.parentInner = (uint32_t *)(b + oPI);
-  st.counters = (uint32_t *)(b + oCnt);
-  st.header = (float *)(b + oHdr);
-  st.keysTmp = (uint32_t *)(b + oKT);
-  st.idsTmp = (uint32_t *)(b + oIT);
-  st.sortScratch = b + oSS;
-  st.sortScratchBytes = sortBytes;
-  st.n = n;
-  return true;
-}
+  if (prims >= (1ull << 30)) {
+    fail("gprtAccelBuild: %llu triangles in one build exceeds the 2^30 limit", (unsigned long long)prims);
+    return false;
+  }
+  tileMesh.resize(tiles);
+  for (uint32_t m = 0; m < count; ++m)
+    for (uint32_t t = 0; t < desc[m].ntiles; ++t) tileMesh[desc[m].tileOffset + t] = m;
 
-void lbvh_release(LBVHStorage &st) {
-  if (st.block) hipFree(st.block);
-  st = LBVHStorage();
-}
-
-bool lbvh_build(Context *ctx, hipStream_t s, Accel *a) {
-  Geom *g = a->geom;
-  const uint32_t n = g->icount;
-  if (!lbvh_reserve(ctx, a->bvh, n)) return false;
-  LBVHStorage &st = a->bvh;
-  const GeomView gv = geom_view(g);
-  const uint32_t inner = n > 1 ? n - 1 : 0;
-  launch(ctx, s, "lbvh_init", k_build_init, dim3(grid_for(inner > 16 ? inner : 16, 256, 2048)), dim3(256), 0,
-         st.counters, inner, (uint32_t *)st.header);
-  launch(ctx, s, "lbvh_bounds", k_bounds, dim3(grid_for(n, 256 * 8, (unsigned)ctx->cuCount * 8)), dim3(256), 0, gv,
-         n, (uint32_t *)st.header);
-  launch(ctx, s, "lbvh_keys", k_keys, dim3(grid_for(n, 256, 1u << 20)), dim3(256), 0, gv, n,
-         (const uint32_t *)st.header, st.mcodes, st.ids);
-  if (n > 1) {
-    if (!sort_run(ctx, s, st.mcodes, st.ids, st.keysTmp, st.idsTmp, n, 32, 32, 0, 30, st.sortScratch,
-                  ctx->sortCfg))
+  // Reuse the store when exactly these accels own it and the plan is unchanged: the cached graph
+  // then replays the whole build with one launch.
+  BVHStore *st = accels[0]->store;
+  bool reuse = st && st->refs == (int)count;
+  for (uint32_t m = 0; reuse && m < count; ++m) reuse = accels[m]->store == st && accels[m]->mesh == m;
+  const bool samePlan = reuse && same_plan(st, desc, cfg);
+  if (!reuse) {
+    for (uint32_t m = 0; m < count; ++m) accel_detach(accels[m]);
+    st = new BVHStore();
+    st->refs = (int)count;
+    for (uint32_t m = 0; m < count; ++m) {
+      accels[m]->store = st;
+      accels[m]->mesh = m;
+    }
+  }
+  if (!samePlan) {
+    if (st->graph) {
+      hipGraphExecDestroy(st->graph);
+      st->graph = nullptr;
+    }
+    st->graphValid = false;
+    if (!store_layout(st, desc, tileMesh, cfg, (uint32_t)prims)) return false;
+    if (!hip_ok(hipMemcpyAsync(st->dDesc, st->desc.data(), desc.size() * sizeof(MeshDesc), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync(mesh table)"))
       return false;
-    launch(ctx, s, "lbvh_karras", k_karras, dim3(grid_for(inner, 256, 1u << 22)), dim3(256), 0,
-           (const uint32_t *)st.mcodes, (const uint32_t *)st.ids, n, (BVHNode *)st.nodes, st.parentLeaf,
-           st.parentInner);
+    if (!hip_ok(hipMemcpyAsync(st->dTileMesh, st->tileMesh.data(), tileMesh.size() * 4, hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync(tile table)"))
+      return false;
   }
-  launch(ctx, s, "lbvh_refit", k_refit, dim3(grid_for(n, 256, 1u << 22)), dim3(256), 0, gv,
-         (const uint32_t *)st.ids, n, (BVHNode *)st.nodes, (TriRecord *)st.tris, (const uint32_t *)st.parentLeaf,
-         (const uint32_t *)st.parentInner, st.counters, st.header);
+  for (uint32_t m = 0; m < count; ++m) set_views(accels[m]);
+
+  if (useGraph) {
+    if (!st->graphValid) {
+      // The table uploads above must land before capture starts.
+      if (!hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize(pre-capture)")) return false;
+      const bool timer = ctx->timer.enabled;
+      ctx->timer.enabled = false;
+      hipGraph_t graph = nullptr;
+      if (!hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) {
+        ctx->timer.enabled = timer;
+        return false;
+      }
+      enqueue_build(ctx, s, st);
+      const bool ok = hip_ok(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+      ctx->timer.enabled = timer;
+      if (!ok) return false;
+      if (!hip_ok(hipGraphInstantiate(&st->graph, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+        hipGraphDestroy(graph);
+        return false;
+      }
+      hipGraphDestroy(graph);
+      st->graphValid = true;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timer.enabled) {
+      e0 = ctx->timer.get();
+      e1 = ctx->timer.get();
+      hipEventRecord(e0, s);
+    }
+    if (!hip_ok(hipGraphLaunch(st->graph, s), "hipGraphLaunch")) return false;
+    if (ctx->timer.enabled) {
+      hipEventRecord(e1, s);
+      ctx->timer.pending.push_back({"lbvh_batch_graph", e0, e1});
+    }
+  } else {
+    enqueue_build(ctx, s, st);
+  }
+  for (uint32_t m = 0; m < count; ++m) accels[m]->built = true;
   return true;
 }
 
-bool lbvh_refit(Context *ctx, hipStream_t s, Accel *a) {
-  LBVHStorage &st = a->bvh;
-  const uint32_t n = st.n;
-  const GeomView gv = geom_view(a->geom);
-  const uint32_t inner = n > 1 ? n - 1 : 0;
-  if (inner) fill_u32(ctx, s, st.counters, 0u, inner);
-  launch(ctx, s, "lbvh_refit", k_refit, dim3(grid_for(n, 256, 1u << 22)), dim3(256), 0, gv,
-         (const uint32_t *)st.ids, n, (BVHNode *)st.nodes, (TriRecord *)st.tris, (const uint32_t *)st.parentLeaf,
-         (const uint32_t *)st.parentInner, st.counters, st.header);
+bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
+  BVHStore *st = acc->store;
+  const MeshDesc &d = st->desc[acc->mesh];
+  if (d.ntri > 1) fill_u32(ctx, s, st->counters + d.innerOffset, 0u, d.ntri - 1);
+  const BuildArgs a = args_of(st);
+  launch(ctx, s, "lbvh_refit", k_refit, dim3(d.ntiles), dim3(kThreads), 0, a, d.tileOffset);
   return true;
 }
 
-bool lbvh_export(Context *ctx, hipStream_t s, Accel *a, int32_t *nodes, float *aabbs, uint64_t *keys) {
-  LBVHStorage &st = a->bvh;
-  const uint32_t n = st.n;
+bool lbvh_export(Context *ctx, hipStream_t s, Accel *acc, int32_t *nodes, float *aabbs, uint64_t *keys) {
+  BVHStore *st = acc->store;
+  const MeshDesc &d = st->desc[acc->mesh];
+  const uint32_t n = d.ntri;
   launch(ctx, s, "lbvh_export", k_export, dim3(grid_for(2 * (size_t)n - 1, 256, 1u << 22)), dim3(256), 0,
-         (const BVHNode *)st.nodes, (const uint32_t *)st.mcodes, (const uint32_t *)st.ids,
-         (const uint32_t *)st.parentLeaf, (const uint32_t *)st.parentInner, (const float *)st.header, n,
-         (int4 *)nodes, aabbs, keys);
+         (const BVHNode *)acc->nodes, (const uint32_t *)(st->codes + d.primOffset),
+         (const uint32_t *)(st->ids + d.primOffset), (const uint32_t *)(st->parentLeaf + d.primOffset),
+         (const uint32_t *)(st->parentInner + d.innerOffset), (const float *)acc->header, n, (int4 *)nodes, aabbs,
+         keys);
   return true;
 }
 

@@ -120,7 +120,6 @@ __global__ void __launch_bounds__(kTraceBlock)
 
 bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count) {
   if (count == 0) return true;
-  LBVHStorage &st = a->bvh;
   const unsigned blocks = (unsigned)ctx->cuCount * 8;
   const size_t need = (size_t)blocks * kTraceBlock * (kMaxStack - kLdsStack) * 4 + 256;
   if (need > ctx->traceBytes) {
@@ -136,8 +135,8 @@ bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void
   uint32_t *work = (uint32_t *)ctx->traceScratch;
   void *spill = (char *)ctx->traceScratch + 256;
   fill_u32(ctx, s, work, 0u, 1);
-  launch(ctx, s, "trace_closest", k_trace, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)st.nodes,
-         (const TriRecord *)st.tris, st.n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, (uint32_t *)spill);
+  launch(ctx, s, "trace_closest", k_trace, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
+         (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, (uint32_t *)spill);
   return true;
 }
 

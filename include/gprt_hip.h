@@ -186,6 +186,16 @@ GPRT_API void gprtBufferSortKV(GPRTContext context, GPRTBuffer keys, GPRTBuffer 
  * Results do not depend on them. */
 GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_t tileKeys);
 
+/* Batched build: every accel in `accels` (each over its own triangle geometry) is built by one
+ * launch sequence over a shared tile table — bounds, keys, segmented radix sort, Karras and refit
+ * of all meshes together — captured into a hipGraph on the first call and replayed by later calls
+ * with the same accels and geometry shapes (vertex positions may change between calls). The
+ * accels share one device store until they are rebuilt alone or destroyed. Each accel's result
+ * is identical to gprtAccelBuild on it. The reference builds every accel separately with a
+ * device synchronisation per build (gprt/gprt.cpp:3216, :3271, :3285). */
+GPRT_API void gprtAccelBuildBatch(GPRTContext context, GPRTAccel *accels, uint32_t count,
+                                  GPRTBuildParams buildParams GPRT_IF_CPP(= GPRTBuildParams()));
+
 /* Ray batch closest hit against a built triangle accel. rays: `count` records of
  * {float ox, oy, oz, tmin; float dx, dy, dz, tmax} (32 B); hits: `count` records of
  * {float t; int32 primID; float u, v} (16 B), primID = -1 on a miss. A hit is accepted when
