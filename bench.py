@@ -104,7 +104,7 @@ def main():
     lib.gprtKernelTimerEnable(ctx.h, False)
 
     K = args.steps
-    build_names = ["lbvh_", "sort_", "fill_u32"]
+    build_names = ["lbvh_"]
     trace_ms_tot, trace_launches = kernel_ms(lib, ctx, "trace_closest")
     build_ms_tot = 0.0
     buf = C.create_string_buffer(1 << 16)
@@ -116,21 +116,23 @@ def main():
     for name, rec in per_kernel.items():
         if any(name.startswith(p) for p in build_names):
             build_ms_tot += rec["total_ms"]
-    # fill_u32 launches belong to both phases (trace counter reset is one tiny launch per step)
     build_ms = build_ms_tot / K
     trace_ms = trace_ms_tot / K
     ms_per_step = wall * 1e3 / K
     build_mprims = N_TRIS / (build_ms * 1e3)
     trace_mrays = n_rays / (trace_ms * 1e3)
 
+    for name, rec in sorted(per_kernel.items(), key=lambda kv: -kv[1]["total_ms"]):
+        log(f"  {name:24s} {rec['launches']:5d} launches {rec['total_ms'] / K:9.3f} ms/step "
+            f"{rec['total_ms'] / rec['launches']:9.4f} ms/launch")
     hits = hb.read(api.HIT_DTYPE)
     hit_rate = float((hits["primID"] >= 0).mean())
 
     # Roofline of the dominant build kernel: one onesweep pass over u32 key + u32 payload.
-    sw_ms, sw_n = kernel_ms(lib, ctx, "sort_onesweep")
+    sw_ms, sw_n = kernel_ms(lib, ctx, "lbvh_sort_pass")
     sw_avg_ms = sw_ms / max(sw_n, 1)
     sw_bytes = N_TRIS * 16
-    roof_sort = {"kernel": "sort_onesweep", "bound": "hbm", "achieved": sw_bytes / (sw_avg_ms * 1e-3) / 1e9,
+    roof_sort = {"kernel": "lbvh_sort_pass", "bound": "hbm", "achieved": sw_bytes / (sw_avg_ms * 1e-3) / 1e9,
                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
                  "avg_launch_ms": sw_avg_ms, "bytes_per_launch": sw_bytes}
     roof_sort["frac"] = roof_sort["achieved"] / HBM_PEAK_GBS

@@ -51,6 +51,8 @@ struct BuildArgs {
   uint32_t *parentLeaf, *parentInner, *counters;
   float *headers;
   uint32_t *hist, *status, *tickets;
+  uint32_t *entries, *entryCount;  // refit edges into nodes spanning tiles: {mesh, link}
+  uint32_t entryCap;
   int passes;
   int digitBits;
 };
@@ -460,7 +462,8 @@ __global__ void __launch_bounds__(kThreads) k_karras(BuildArgs a, uint32_t tileB
       c1 = (int)(split + 1);
       parentInner[split + 1] = ((uint32_t)i << 1) | 1u;
     }
-    nodes[i].d = make_int4(c0, c1, 0, 0);
+    // d.z keeps the other end of the node's leaf range [min(i,j), max(i,j)] for the refit.
+    nodes[i].d = make_int4(c0, c1, (int)j, 0);
     if (i == 0) parentInner[0] = 0xFFFFFFFFu;
   }
 }
@@ -483,10 +486,25 @@ __device__ __forceinline__ void load_child_box(const BVHNode *nodes, uint32_t no
   b[5] = z.y;
 }
 
+__device__ __forceinline__ void merge_box(float box[6], const float sib[6]) {
+  box[0] = fmin_ord(box[0], sib[0]);
+  box[1] = fmin_ord(box[1], sib[1]);
+  box[2] = fmin_ord(box[2], sib[2]);
+  box[3] = fmax_ord(box[3], sib[3]);
+  box[4] = fmax_ord(box[4], sib[4]);
+  box[5] = fmax_ord(box[5], sib[5]);
+}
+
 // ------------------------------------------------------------------------------------------------
-// Leaf records + bottom-up refit with arrival counters. One thread per leaf.
+// Refit, phase 1 (one workgroup per tile): leaf records and every inner node whose leaf range
+// lies inside the tile. Arrival counters for those nodes live in LDS and the boxes are ordered
+// with workgroup-scope acquire/release, which costs no cache maintenance. A climb that reaches a
+// node whose range leaves the tile writes its box into that node and records the edge for
+// phase 2.
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_refit(BuildArgs a, uint32_t tileBase) {
+template <int TILE>
+__global__ void __launch_bounds__(kThreads) k_refit_local(BuildArgs a, uint32_t tileBase) {
+  __shared__ uint32_t s_cnt[TILE];
   const uint32_t t = tileBase + blockIdx.x;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
@@ -495,8 +513,9 @@ __global__ void __launch_bounds__(kThreads) k_refit(BuildArgs a, uint32_t tileBa
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
   const uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
   const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
-  uint32_t *__restrict__ counters = a.counters + d.innerOffset;
   float *__restrict__ header = a.headers + (size_t)m * 16;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)TILE; i += kThreads) s_cnt[i] = 0u;
+  __syncthreads();
   for (uint32_t k = r.lo + threadIdx.x; k < r.hi; k += kThreads) {
     const uint32_t prim = a.ids[d.primOffset + k];
     float3 p, q, s;
@@ -524,22 +543,64 @@ __global__ void __launch_bounds__(kThreads) k_refit(BuildArgs a, uint32_t tileBa
       const uint32_t node = link >> 1, side = link & 1u;
       if (node >= n - 1) break;
       store_child_box(nodes, node, side, box);
+      const uint32_t other = (uint32_t)nodes[node].d.z;
+      const uint32_t first = min(node, other), last = max(node, other);
+      if (first < r.lo || last >= r.hi) {
+        // The node's range leaves this tile: phase 2 finishes it.
+        const uint32_t e = atomicAdd(a.entryCount, 1u);
+        if (e < a.entryCap) {
+          a.entries[2 * e] = m;
+          a.entries[2 * e + 1] = link;
+        }
+        break;
+      }
       const uint32_t old =
-          __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(&s_cnt[node - r.lo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old != 1u) break;  // first arrival: the sibling's thread finishes this node
       float sib[6];
       load_child_box(nodes, node, side ^ 1u, sib);
-      box[0] = fmin_ord(box[0], sib[0]);
-      box[1] = fmin_ord(box[1], sib[1]);
-      box[2] = fmin_ord(box[2], sib[2]);
-      box[3] = fmax_ord(box[3], sib[3]);
-      box[4] = fmax_ord(box[4], sib[4]);
-      box[5] = fmax_ord(box[5], sib[5]);
+      merge_box(box, sib);
       link = parentInner[node];
       if (link == 0xFFFFFFFFu) {
         for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
         break;
       }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Refit, phase 2: the nodes whose range spans several tiles. Each recorded edge arrives at its
+// node through a device-scope acquire/release counter; the second arrival merges both child
+// boxes and climbs. Only these few nodes pay for cross-XCD ordering.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_refit_top(BuildArgs a) {
+  const uint32_t count = min(*a.entryCount, a.entryCap);
+  for (uint32_t e = blockIdx.x * kThreads + threadIdx.x; e < count; e += gridDim.x * kThreads) {
+    const uint32_t m = a.entries[2 * e];
+    uint32_t link = a.entries[2 * e + 1];
+    const MeshDesc d = a.desc[m];
+    const uint32_t n = d.ntri;
+    BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
+    const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
+    uint32_t *__restrict__ counters = a.counters + d.innerOffset;
+    float *__restrict__ header = a.headers + (size_t)m * 16;
+    for (int level = 0; level < 72; ++level) {
+      const uint32_t node = link >> 1, side = link & 1u;
+      if (node >= n - 1) break;
+      const uint32_t old =
+          __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (old != 1u) break;
+      float box[6], sib[6];
+      load_child_box(nodes, node, side, box);
+      load_child_box(nodes, node, side ^ 1u, sib);
+      merge_box(box, sib);
+      link = parentInner[node];
+      if (link == 0xFFFFFFFFu) {
+        for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
+        break;
+      }
+      store_child_box(nodes, link >> 1, link & 1u, box);
     }
   }
 }
@@ -620,9 +681,23 @@ BuildArgs args_of(BVHStore *st) {
   a.hist = st->hist;
   a.status = st->status;
   a.tickets = st->tickets;
+  a.entries = st->entries;
+  a.entryCount = st->tickets + 32;
+  a.entryCap = st->prims;
   a.passes = st->passes;
   a.digitBits = st->cfg.digitBits;
   return a;
+}
+
+void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a, uint32_t tileBase, uint32_t tiles) {
+  if (st->cfg.tileKeys == 2048)
+    launch(ctx, s, "lbvh_refit_local", k_refit_local<2048>, dim3(tiles), dim3(kThreads), 0, a, tileBase);
+  else if (st->cfg.tileKeys == 4096)
+    launch(ctx, s, "lbvh_refit_local", k_refit_local<4096>, dim3(tiles), dim3(kThreads), 0, a, tileBase);
+  else
+    launch(ctx, s, "lbvh_refit_local", k_refit_local<8192>, dim3(tiles), dim3(kThreads), 0, a, tileBase);
+  launch(ctx, s, "lbvh_refit_top", k_refit_top, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tiles, 1024))),
+         dim3(kThreads), 0, a);
 }
 
 template <int BITS>
@@ -666,7 +741,7 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
   if (st->passes & 1)
     launch(ctx, s, "lbvh_copy_back", k_copy_back, dim3(grid_for(st->prims, 256, 8192)), dim3(256), 0, a, st->prims);
   launch(ctx, s, "lbvh_karras", k_karras, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
-  launch(ctx, s, "lbvh_refit", k_refit, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
+  launch_refit(ctx, s, st, a, 0u, st->tiles);
 }
 
 bool same_plan(const BVHStore *st, const std::vector<MeshDesc> &desc, const SortConfig &cfg) {
@@ -717,6 +792,8 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   off += al((size_t)std::max<uint32_t>(I, 1) * 4);
   const size_t oHdr = off;
   off += al((size_t)M * 16 * 4);
+  const size_t oEnt = off;
+  off += al((size_t)prims * 8);
   const size_t oZero = off;
   const size_t cntWords = std::max<uint32_t>(I, 1);
   const size_t histWords = (size_t)M * passes * R;
@@ -747,6 +824,7 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->parentLeaf = (uint32_t *)(b + oPL);
   st->parentInner = (uint32_t *)(b + oPI);
   st->headers = (float *)(b + oHdr);
+  st->entries = (uint32_t *)(b + oEnt);
   st->zeroBase = (uint32_t *)(b + oZero);
   st->zeroWords = zeroWords;
   st->counters = st->zeroBase;
@@ -892,8 +970,9 @@ bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   BVHStore *st = acc->store;
   const MeshDesc &d = st->desc[acc->mesh];
   if (d.ntri > 1) fill_u32(ctx, s, st->counters + d.innerOffset, 0u, d.ntri - 1);
+  fill_u32(ctx, s, st->tickets + 32, 0u, 1);  // refit edge count
   const BuildArgs a = args_of(st);
-  launch(ctx, s, "lbvh_refit", k_refit, dim3(d.ntiles), dim3(kThreads), 0, a, d.tileOffset);
+  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles);
   return true;
 }
 
