@@ -82,6 +82,18 @@ _SIGS = {
     "gprtAccelGetPrimCount": (C.c_uint32, [_P]),
     "gprtAccelTraceClosestHit": (None, [_P, _P, _P, _P, C.c_uint32]),
     "gprtAccelExportLBVH": (None, [_P, _P, _P, _P, _P]),
+    "gprtProposalCreate": (_P, [_P, C.c_uint32, C.c_uint64]),
+    "gprtProposalDestroy": (None, [_P, _P]),
+    "gprtProposalBindAccel": (C.c_uint32, [_P, _P, _P, C.c_uint32]),
+    "gprtProposalParamCount": (C.c_size_t, []),
+    "gprtProposalSetParams": (None, [_P, _P, _P, C.c_size_t]),
+    "gprtProposalGetParams": (None, [_P, _P, _P, C.c_size_t]),
+    "gprtProposalTrain": (C.c_float, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
+    "gprtProposalInfer": (None, [_P, _P, _P, C.c_uint32, _P]),
+    "gprtProposalLogits": (None, [_P, _P, _P, C.c_uint32, _P]),
+    "gprtProposalGradients": (None, [_P, _P, _P, _P, C.c_uint32, _P, C.c_size_t]),
+    "gprtProposalExportTreelets": (None, [_P, _P, _P, _P]),
+    "gprtAccelTraceClosestHitEx": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P]),
     "gprtComputeSynchronize": (C.c_uint64, [_P]),
     "gprtDeviceSynchronize": (C.c_uint64, [_P]),
     "gprtBeginProfile": (None, [_P]),
@@ -250,3 +262,37 @@ def build_batch(ctx: Context, accels, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE)
     """gprtAccelBuildBatch over a list of TriangleAccel objects."""
     arr = (C.c_void_p * len(accels))(*[a.h for a in accels])
     ctx.lib.gprtAccelBuildBatch(ctx.h, arr, len(accels), GPRTBuildParams(mode))
+
+
+class Proposal:
+    """gprtProposal*: the learned treelet proposal bound to one accel."""
+
+    def __init__(self, ctx: Context, max_batch: int = 8192, seed: int = 1):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.h = self.lib.gprtProposalCreate(ctx.h, max_batch, seed)
+        if not self.h:
+            raise RuntimeError("gprtProposalCreate failed: %s" % last_error())
+        self.n_params = int(self.lib.gprtProposalParamCount())
+
+    def bind(self, accel: "TriangleAccel", cut_depth: int = 12) -> int:
+        return int(self.lib.gprtProposalBindAccel(self.ctx.h, self.h, accel.h, cut_depth))
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self.lib.gprtProposalGetParams(self.ctx.h, self.h, out.ctypes.data_as(C.c_void_p), self.n_params)
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.lib.gprtProposalSetParams(self.ctx.h, self.h, p.ctypes.data_as(C.c_void_p), len(p))
+
+    def train(self, rays_buf: Buffer, hits_buf: Buffer, count: int, epochs: int = 1, lr: float = 1e-3):
+        acc = C.c_float(0.0)
+        loss = self.lib.gprtProposalTrain(self.ctx.h, self.h, rays_buf.h, hits_buf.h, count, epochs, lr,
+                                          C.byref(acc))
+        return float(loss), float(acc.value)
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtProposalDestroy(self.ctx.h, self.h)
+            self.h = None

@@ -206,5 +206,8 @@ bool lbvh_export(Context *ctx, hipStream_t s, Accel *a, int32_t *nodes, float *a
 
 // Trace
 bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count);
+// proposal / classRoot / visits may be null. proposal[i] = treelet class of ray i (or -1).
+bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
+                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits);
 
 }  // namespace gprt

@@ -1,0 +1,778 @@
+// Learned treelet proposal for gfx950: a 102 -> 128 -> 128 -> 128 -> 4096 MLP in bf16 on MFMA
+// with FP32 accumulation, trained on the device by hand-written forward / backward / Adam
+// kernels, and a proposal-first closest-hit traversal whose result equals the plain one.
+//
+// The reference has no counterpart (its traversal is the Vulkan driver's); the closest relative
+// is the hand-written Adam of samples/s9-differentiable/hostCode.cpp:66-69, :375-397. Standard
+// Adam (bias-corrected first and second moments, beta1 0.9, beta2 0.999) is used here.
+//
+// Treelets: the LBVH is cut at depth 12; every node at depth 12, and every leaf shallower than
+// that, roots one treelet (at most 4096). A ray's label is the treelet holding the primitive the
+// plain traversal hit. At inference the top-1 treelet is traversed first to obtain a hit that
+// tightens tmax, then the exact traversal runs from the root with that (t, primID) as its
+// starting best. Because the candidate is a real hit and the closest-hit rule is the
+// lexicographic minimum of (t, primID), the result cannot differ from the plain traversal.
+//
+// Layouts (all weights transposed, Wt[out][in], so a K-slice of one output is contiguous):
+//   fp32 parameters, flat: W1t[128][128] b1[128] W2t[128][128] b2[128] W3t[128][128] b3[128]
+//                          W4t[4096][128] b4[4096]   (input columns 102..127 of W1t stay 0)
+//   bf16 copies of W1t..W4t for the MFMA kernels; Adam moments m, v in the fp32 layout.
+//   Input features per ray (102): o.xyz, d.xyz, then for each of those 6 values v and each band
+//   k = 0..7: sin(pi * 2^k * v), cos(pi * 2^k * v); padded with zeros to 128.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bvh.hpp"
+#include "device_common.hpp"
+#include "internal.hpp"
+#include "mlp.hpp"
+
+namespace gprt {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+
+union Frag {
+  bf16x8_t v;
+  uint4 u;
+  uint16_t h[8];
+};
+
+__device__ __forceinline__ f32x4_t mfma16(const Frag &a, const Frag &b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, c, 0, 0, 0);
+}
+
+// FP32 -> bf16 with round-to-nearest-even (NaN stays NaN).
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+  union {
+    float f;
+    uint32_t u;
+  } x;
+  x.f = f;
+  if ((x.u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((x.u >> 16) | 0x40u);
+  const uint32_t r = 0x7FFFu + ((x.u >> 16) & 1u);
+  return (uint16_t)((x.u + r) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ------------------------------------------------------------------------------------------------
+// Input encoding, produced directly as 64 packed words (two bf16 per word, low half first):
+// words 0..2 = (o.x,o.y) (o.z,d.x) (d.y,d.z); word 3 + 8c + k = (sin, cos)(2^k pi v_c); rest 0.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16); }
+
+__device__ __forceinline__ void encode_ray(const GPRTRay &r, uint32_t (&wds)[kMlpInPad / 2]) {
+  const float v[6] = {r.ox, r.oy, r.oz, r.dx, r.dy, r.dz};
+  wds[0] = pack2(v[0], v[1]);
+  wds[1] = pack2(v[2], v[3]);
+  wds[2] = pack2(v[4], v[5]);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    float x = v[c];  // 2^k * v is exact, so sin(pi * 2^k * v) needs no large-argument reduction
+#pragma unroll
+    for (int k = 0; k < kMlpBands; ++k) {
+      float sn, cs;
+      sincospif(x, &sn, &cs);
+      wds[3 + c * kMlpBands + k] = pack2(sn, cs);
+      x *= 2.0f;
+    }
+  }
+#pragma unroll
+  for (int i = 3 + 6 * kMlpBands; i < kMlpInPad / 2; ++i) wds[i] = 0u;
+}
+
+__device__ __forceinline__ void store_features(const uint32_t (&wds)[kMlpInPad / 2], uint16_t *dst) {
+#pragma unroll
+  for (int q = 0; q < kMlpInPad / 8; ++q)
+    *(uint4 *)(dst + 8 * q) = make_uint4(wds[4 * q], wds[4 * q + 1], wds[4 * q + 2], wds[4 * q + 3]);
+}
+
+__global__ void __launch_bounds__(256) k_encode(const GPRTRay *__restrict__ rays, uint32_t count,
+                                                uint16_t *__restrict__ X) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t wds[kMlpInPad / 2];
+  encode_ray(rays[i], wds);
+  store_features(wds, X + (size_t)i * kMlpInPad);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Generic bf16 GEMM on MFMA 16x16x32 with FP32 accumulation:
+//   C[m][n] = alpha * sum_k A(m,k) B(k,n) (+ bias[n]) (* (mask(m,n) > 0)) (relu) (+= C)
+// A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]: any transposition through strides.
+// Workgroup tile 64x64 (4 waves of 32x32 = 2x2 MFMA tiles), K step 32 staged in LDS.
+// ------------------------------------------------------------------------------------------------
+struct GemmArgs {
+  const uint16_t *A;
+  long long sam, sak;
+  const uint16_t *B;
+  long long sbk, sbn;
+  float *Cf;
+  uint16_t *Cb;
+  long long scm, scn;
+  int M, N, K;
+  const float *bias;
+  const uint16_t *mask;
+  long long smm, smn;
+  int relu;
+  int accumulate;
+  float alpha;
+};
+
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  f32x4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int r = tid >> 2, kk = (tid & 3) * 8;
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int m = m0 + r, k = k0 + kk + j, n = n0 + r;
+      As[r][kk + j] = (m < g.M && k < g.K) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : (uint16_t)0;
+      Bs[r][kk + j] = (n < g.N && k < g.K) ? g.B[(long long)k * g.sbk + (long long)n * g.sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+    Frag a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      a[i].u = *(const uint4 *)&As[wm + i * 16 + (l & 15)][8 * (l >> 4)];
+      b[i].u = *(const uint4 *)&Bs[wn + i * 16 + (l & 15)][8 * (l >> 4)];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + i * 16 + 4 * (l >> 4) + q;
+        const int n = n0 + wn + j * 16 + (l & 15);
+        if (m >= g.M || n >= g.N) continue;
+        float v = g.alpha * acc[i][j][q];
+        if (g.bias) v += g.bias[n];
+        if (g.mask && !(bf2f(g.mask[(long long)m * g.smm + (long long)n * g.smn]) > 0.f)) v = 0.f;
+        if (g.relu) v = fmaxf(v, 0.f);
+        const long long o = (long long)m * g.scm + (long long)n * g.scn;
+        if (g.Cf) {
+          if (g.accumulate)
+            g.Cf[o] += v;
+          else
+            g.Cf[o] = v;
+        }
+        if (g.Cb) g.Cb[o] = f2bf(v);
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Softmax cross-entropy gradient: dZ = (softmax(z) - onehot(y)) * scale; rows with y < 0 give 0.
+// One workgroup per row. loss += -log softmax(z)[y] * scale.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_softmax_ce(const float *__restrict__ Z, const int32_t *__restrict__ labels,
+                                                    uint16_t *__restrict__ dZ, float scale, float *__restrict__ loss,
+                                                    uint32_t *__restrict__ correct) {
+  const uint32_t row = blockIdx.x;
+  const int tid = threadIdx.x;
+  const float *z = Z + (size_t)row * kMlpOut;
+  uint16_t *dz = dZ + (size_t)row * kMlpOut;
+  const int y = labels[row];
+  __shared__ float s_red[4];
+  __shared__ int s_idx[4];
+  float mx = -INFINITY;
+  int amx = 0;
+  for (int c = tid; c < kMlpOut; c += 256) {
+    if (z[c] > mx) {
+      mx = z[c];
+      amx = c;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const float om = __shfl_xor(mx, o, 64);
+    const int oi = __shfl_xor(amx, o, 64);
+    if (om > mx || (om == mx && oi < amx)) {
+      mx = om;
+      amx = oi;
+    }
+  }
+  if ((tid & 63) == 0) {
+    s_red[tid >> 6] = mx;
+    s_idx[tid >> 6] = amx;
+  }
+  __syncthreads();
+  mx = s_red[0];
+  amx = s_idx[0];
+  for (int w = 1; w < 4; ++w)
+    if (s_red[w] > mx || (s_red[w] == mx && s_idx[w] < amx)) {
+      mx = s_red[w];
+      amx = s_idx[w];
+    }
+  __syncthreads();
+  float sum = 0.f;
+  for (int c = tid; c < kMlpOut; c += 256) sum += expf(z[c] - mx);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if ((tid & 63) == 0) s_red[tid >> 6] = sum;
+  __syncthreads();
+  sum = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  const float inv = 1.0f / sum;
+  for (int c = tid; c < kMlpOut; c += 256) {
+    float g = 0.f;
+    if (y >= 0) g = (expf(z[c] - mx) * inv - (c == y ? 1.f : 0.f)) * scale;
+    dz[c] = f2bf(g);
+  }
+  if (tid == 0 && y >= 0) {
+    atomicAdd(loss, (logf(sum) + mx - z[y]) * scale);
+    if (amx == y) atomicAdd(correct, 1u);
+  }
+}
+
+// Column sums (bias gradients): out[n] += sum_m A[m*lda + n] (bf16 input). Blocks cover 256
+// columns x 256 rows; partial sums meet in FP32 atomics (out is zeroed beforehand).
+__global__ void __launch_bounds__(256) k_colsum(const uint16_t *__restrict__ A, int M, int N, int lda,
+                                                float *__restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int m0 = blockIdx.y * 256, m1 = min(M, m0 + 256);
+  float s = 0.f;
+  for (int m = m0; m < m1; ++m) s += bf2f(A[(size_t)m * lda + n]);
+  atomicAdd(&out[n], s);
+}
+
+// Adam (bias-corrected) on the fp32 parameters; refreshes the bf16 weight copies.
+__global__ void __launch_bounds__(256) k_adam(float *__restrict__ w, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, uint32_t n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2,
+                                              const uint32_t *__restrict__ bfMap, uint16_t *__restrict__ wbf) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / c1, vh = vi / c2;
+  const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+  w[i] = wi;
+  const uint32_t b = bfMap[i];
+  if (b != 0xFFFFFFFFu) wbf[b] = f2bf(wi);
+}
+
+__global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ w, const uint32_t *__restrict__ bfMap,
+                                                 uint16_t *__restrict__ wbf, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = bfMap[i];
+  if (b != 0xFFFFFFFFu) wbf[b] = f2bf(w[i]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Fused inference: encode -> 3 hidden layers -> 4096 logits with a running argmax, never
+// writing the logits. A workgroup of 4 waves handles 256 rays (64 per wave, 4 row blocks of 16);
+// hidden weights and 64-column chunks of W4t are staged in LDS and shared by the 4 waves, so
+// each chunk read from L2 serves 256 rays.
+// ------------------------------------------------------------------------------------------------
+constexpr int kInferRaysPerWave = 64;
+constexpr int kInferRays = 4 * kInferRaysPerWave;
+constexpr int kChunk = 64;     // W4t rows (output classes) per staged chunk
+constexpr int kActStride = 136;  // bf16 per activation row in LDS (128 + 8 pad)
+constexpr int kWStride = 136;
+
+__global__ void __launch_bounds__(256) k_infer_top1(const GPRTRay *__restrict__ rays, uint32_t count,
+                                                    const uint16_t *__restrict__ W1t,
+                                                    const uint16_t *__restrict__ W2t,
+                                                    const uint16_t *__restrict__ W3t,
+                                                    const uint16_t *__restrict__ W4t, const float *__restrict__ bias,
+                                                    int32_t *__restrict__ outClass, float *__restrict__ outLogit,
+                                                    uint32_t numClasses) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+  uint16_t *act = smem;                                   // [256 rays][kActStride]
+  uint16_t *wbuf = smem + kInferRays * kActStride;        // [128 rows][kWStride]
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const uint32_t ray0 = blockIdx.x * kInferRays;
+  uint16_t *myAct = act + (size_t)w * kInferRaysPerWave * kActStride;
+
+  // Encode: one ray per lane.
+  {
+    const uint32_t ri = ray0 + w * kInferRaysPerWave + l;
+    uint32_t wds[kMlpInPad / 2];
+    if (ri < count) {
+      encode_ray(rays[ri], wds);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kMlpInPad / 2; ++i) wds[i] = 0u;
+    }
+    store_features(wds, &myAct[l * kActStride]);
+  }
+
+  // Hidden layers.
+  const uint16_t *Ws[3] = {W1t, W2t, W3t};
+  for (int layer = 0; layer < 3; ++layer) {
+    __syncthreads();  // previous users of wbuf are done
+    // Stage Wt[128][128] -> wbuf (each thread copies 64 bf16).
+    for (int i = tid; i < 128 * 16; i += 256) {
+      const int row = i >> 4, q = i & 15;
+      *(uint4 *)&wbuf[row * kWStride + q * 8] = *(const uint4 *)&Ws[layer][row * 128 + q * 8];
+    }
+    __syncthreads();
+    const float *b = bias + layer * kMlpHid;
+    f32x4_t acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      Frag a[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[i].u = *(const uint4 *)&myAct[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        Frag bf;
+        bf.u = *(const uint4 *)&wbuf[(j * 16 + (l & 15)) * kWStride + ks * 32 + 8 * (l >> 4)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i][j] = mfma16(a[i], bf, acc[i][j]);
+      }
+    }
+    // All lanes of this wave have read their activations; overwrite them with the outputs.
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = i * 16 + 4 * (l >> 4) + q, col = j * 16 + (l & 15);
+          myAct[row * kActStride + col] = f2bf(fmaxf(acc[i][j][q] + b[col], 0.f));
+        }
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  // Output layer with running argmax. A fragments (h3) stay in registers.
+  Frag a[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      a[i][ks].u = *(const uint4 *)&myAct[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
+  float best[4][4];
+  int bestIdx[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      best[i][q] = -INFINITY;
+      bestIdx[i][q] = 0x7FFFFFFF;
+    }
+  const float *b4 = bias + 3 * kMlpHid;
+  for (int c0 = 0; c0 < kMlpOut; c0 += kChunk) {
+    __syncthreads();
+    for (int i = tid; i < kChunk * 16; i += 256) {
+      const int row = i >> 4, q = i & 15;
+      *(uint4 *)&wbuf[row * kWStride + q * 8] = *(const uint4 *)&W4t[(size_t)(c0 + row) * 128 + q * 8];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kChunk / 16; ++j) {
+      f32x4_t acc[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        Frag bf;
+        bf.u = *(const uint4 *)&wbuf[(j * 16 + (l & 15)) * kWStride + ks * 32 + 8 * (l >> 4)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = mfma16(a[i][ks], bf, acc[i]);
+      }
+      const int col = c0 + j * 16 + (l & 15);
+      const float bcol = b4[col];
+      const bool valid = (uint32_t)col < numClasses;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float v = acc[i][q] + bcol;
+          if (valid && v > best[i][q]) {
+            best[i][q] = v;
+            bestIdx[i][q] = col;
+          }
+        }
+    }
+  }
+  // Reduce over the 16 lanes that share rows (lane bits 0..3 index the column).
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float v = best[i][q];
+      int c = bestIdx[i][q];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        const float ov = __shfl_xor(v, o, 64);
+        const int oc = __shfl_xor(c, o, 64);
+        if (ov > v || (ov == v && oc < c)) {
+          v = ov;
+          c = oc;
+        }
+      }
+      const uint32_t ri = ray0 + w * kInferRaysPerWave + i * 16 + 4 * (l >> 4) + q;
+      if ((l & 15) == 0 && ri < count) {
+        outClass[ri] = c;
+        if (outLogit) outLogit[ri] = v;
+      }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Treelets
+// ------------------------------------------------------------------------------------------------
+// One workgroup: breadth-first over the top `depth` levels. classRoot[c] = node code (inner
+// index >= 0, or ~leaf); innerClass[node] = c for inner roots; leafClass[leaf] = c for leaves
+// that are themselves roots (shallower than `depth`, or at it).
+__global__ void __launch_bounds__(1024) k_treelet_cut(const BVHNode *__restrict__ nodes, uint32_t nprims, int depth,
+                                                      int32_t *__restrict__ classRoot, int32_t *__restrict__ innerClass,
+                                                      int32_t *__restrict__ leafClass, uint32_t *__restrict__ numClasses) {
+  __shared__ int32_t s_front[2][4096];
+  __shared__ uint32_t s_n[2];
+  __shared__ uint32_t s_cls;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_n[0] = 1;
+    s_cls = 0;
+    s_front[0][0] = nprims == 1 ? ~0 : 0;
+  }
+  __syncthreads();
+  for (int d = 0; d <= depth; ++d) {
+    const int cur = d & 1, nxt = cur ^ 1;
+    if (tid == 0) s_n[nxt] = 0;
+    __syncthreads();
+    const uint32_t n = s_n[cur];
+    for (uint32_t i = tid; i < n; i += blockDim.x) {
+      const int32_t code = s_front[cur][i];
+      if (code < 0 || d == depth) {
+        const uint32_t c = atomicAdd(&s_cls, 1u);
+        classRoot[c] = code;
+        if (code < 0)
+          leafClass[~code] = (int32_t)c;
+        else
+          innerClass[code] = (int32_t)c;
+      } else {
+        const int4 dd = nodes[code].d;
+        const uint32_t o = atomicAdd(&s_n[nxt], 2u);
+        s_front[nxt][o] = dd.x;
+        s_front[nxt][o + 1] = dd.y;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *numClasses = s_cls;
+}
+
+// Every leaf below the cut inherits the class of its ancestor at the cut depth.
+__global__ void __launch_bounds__(256) k_leaf_class(const uint32_t *__restrict__ parentLeaf,
+                                                    const uint32_t *__restrict__ parentInner,
+                                                    const int32_t *__restrict__ innerClass,
+                                                    int32_t *__restrict__ leafClass, const uint32_t *__restrict__ ids,
+                                                    int32_t *__restrict__ primClass, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int32_t c = leafClass[k];
+  if (c < 0 && n > 1) {
+    uint32_t link = parentLeaf[k];
+    for (int s = 0; s < 72 && c < 0; ++s) {
+      const uint32_t node = link >> 1;
+      if (node >= n - 1) break;
+      c = innerClass[node];
+      link = parentInner[node];
+      if (link == 0xFFFFFFFFu) break;
+    }
+    leafClass[k] = c;
+  }
+  const uint32_t prim = ids[k];
+  if (prim < n) primClass[prim] = c;
+}
+
+__global__ void __launch_bounds__(256) k_labels(const GPRTHit *__restrict__ hits, uint32_t count,
+                                                const int32_t *__restrict__ primClass, uint32_t nprims,
+                                                int32_t *__restrict__ labels) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int32_t p = hits[i].primID;
+  labels[i] = (p >= 0 && (uint32_t)p < nprims) ? primClass[p] : -1;
+}
+
+unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+void gemm(Context *ctx, hipStream_t s, const GemmArgs &g, const char *name) {
+  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
+  launch(ctx, s, name, k_gemm, grid, dim3(256), 0, g);
+}
+
+GemmArgs gemm_args(const uint16_t *A, long long sam, long long sak, const uint16_t *B, long long sbk, long long sbn,
+                   int M, int N, int K) {
+  GemmArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.A = A;
+  g.sam = sam;
+  g.sak = sak;
+  g.B = B;
+  g.sbk = sbk;
+  g.sbn = sbn;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = 1.f;
+  return g;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------------
+size_t mlp_param_count() { return kMlpParams; }
+
+bool mlp_alloc(Context *ctx, Proposal *p, uint32_t maxBatch) {
+  (void)ctx;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t P = kMlpParams;
+  size_t off = 0;
+  const size_t oW = off;
+  off += al(P * 4);
+  const size_t oG = off;
+  off += al(P * 4);
+  const size_t oM = off;
+  off += al(P * 4);
+  const size_t oV = off;
+  off += al(P * 4);
+  const size_t oMap = off;
+  off += al(P * 4);
+  const size_t oBf = off;
+  off += al(kMlpWeights * 2);
+  const size_t oBias = off;
+  off += al(kMlpBiases * 4);
+  const size_t oStat = off;
+  off += al(64);
+  const size_t B = maxBatch;
+  const size_t oX = off;
+  off += al(B * kMlpInPad * 2);
+  const size_t oH = off;
+  off += al(3 * B * kMlpHid * 2);
+  const size_t oD = off;
+  off += al(3 * B * kMlpHid * 2);
+  const size_t oZ = off;
+  off += al(B * kMlpOut * 4);
+  const size_t oDZ = off;
+  off += al(B * kMlpOut * 2);
+  const size_t oL = off;
+  off += al(B * 4);
+  if (!hip_ok(hipMalloc(&p->block, off), "hipMalloc(proposal)")) return false;
+  p->blockBytes = off;
+  char *b = (char *)p->block;
+  p->w = (float *)(b + oW);
+  p->g = (float *)(b + oG);
+  p->m = (float *)(b + oM);
+  p->v = (float *)(b + oV);
+  p->bfMap = (uint32_t *)(b + oMap);
+  p->wbf = (uint16_t *)(b + oBf);
+  p->bias = (float *)(b + oBias);
+  p->stats = (float *)(b + oStat);
+  p->X = (uint16_t *)(b + oX);
+  p->H = (uint16_t *)(b + oH);
+  p->dA = (uint16_t *)(b + oD);
+  p->Z = (float *)(b + oZ);
+  p->dZ = (uint16_t *)(b + oDZ);
+  p->labels = (int32_t *)(b + oL);
+  p->maxBatch = maxBatch;
+  // Map from flat fp32 parameter index to its bf16 copy (weights only).
+  std::vector<uint32_t> map(P, 0xFFFFFFFFu);
+  for (int layer = 0; layer < 4; ++layer) {
+    const size_t wOff = mlp_weight_offset(layer), n = mlp_weight_size(layer), bfOff = mlp_bf16_offset(layer);
+    for (size_t i = 0; i < n; ++i) map[wOff + i] = (uint32_t)(bfOff + i);
+  }
+  if (!hip_ok(hipMemcpy(p->bfMap, map.data(), P * 4, hipMemcpyHostToDevice), "hipMemcpy(bfMap)")) return false;
+  return true;
+}
+
+void mlp_release(Proposal *p) {
+  if (p->block) hipFree(p->block);
+  if (p->treeBlock) hipFree(p->treeBlock);
+  p->block = p->treeBlock = nullptr;
+}
+
+// Copies fp32 parameters to the device and refreshes the bf16 weights and the bias vector.
+bool mlp_set_params(Context *ctx, hipStream_t s, Proposal *p, const float *host) {
+  if (!hip_ok(hipMemcpyAsync(p->w, host, kMlpParams * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(params)"))
+    return false;
+  launch(ctx, s, "mlp_to_bf16", k_to_bf16, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, (const float *)p->w,
+         (const uint32_t *)p->bfMap, p->wbf, (uint32_t)kMlpParams);
+  fill_u32(ctx, s, p->m, 0u, kMlpParams);
+  fill_u32(ctx, s, p->v, 0u, kMlpParams);
+  p->step = 0;
+  return mlp_refresh_bias(ctx, s, p);
+}
+
+bool mlp_refresh_bias(Context *ctx, hipStream_t s, Proposal *p) {
+  (void)ctx;
+  for (int layer = 0; layer < 4; ++layer) {
+    const size_t n = layer < 3 ? kMlpHid : kMlpOut;
+    if (!hip_ok(hipMemcpyAsync(p->bias + layer * kMlpHid, p->w + mlp_bias_offset(layer), n * 4,
+                               hipMemcpyDeviceToDevice, s),
+                "hipMemcpyAsync(bias)"))
+      return false;
+  }
+  return true;
+}
+
+const uint16_t *mlp_wbf(const Proposal *p, int layer) { return p->wbf + mlp_bf16_offset(layer); }
+
+// Forward of a batch with stored activations (training / logits). Z gets the fp32 logits.
+void mlp_forward_batch(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n) {
+  launch(ctx, s, "mlp_encode", k_encode, dim3(blocks_for(n, 256)), dim3(256), 0, (const GPRTRay *)rays, n, p->X);
+  const uint16_t *in = p->X;
+  int inStride = kMlpInPad;
+  for (int layer = 0; layer < 3; ++layer) {
+    uint16_t *out = p->H + (size_t)layer * p->maxBatch * kMlpHid;
+    GemmArgs g = gemm_args(in, inStride, 1, mlp_wbf(p, layer), 1, kMlpInPad, (int)n, kMlpHid, kMlpInPad);
+    g.Cb = out;
+    g.scm = kMlpHid;
+    g.scn = 1;
+    g.bias = p->w + mlp_bias_offset(layer);
+    g.relu = 1;
+    gemm(ctx, s, g, "mlp_gemm_fwd");
+    in = out;
+    inStride = kMlpHid;
+  }
+  GemmArgs g = gemm_args(in, kMlpHid, 1, mlp_wbf(p, 3), 1, kMlpHid, (int)n, kMlpOut, kMlpHid);
+  g.Cf = p->Z;
+  g.scm = kMlpOut;
+  g.scn = 1;
+  g.bias = p->w + mlp_bias_offset(3);
+  gemm(ctx, s, g, "mlp_gemm_fwd");
+}
+
+// Gradients of the mean cross-entropy over the labelled rays of one batch (into p->g).
+void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, uint32_t labelled) {
+  const float scale = labelled ? 1.0f / (float)labelled : 0.f;
+  fill_u32(ctx, s, p->g, 0u, kMlpParams);  // bias gradients accumulate; weight gradients overwrite
+  launch(ctx, s, "mlp_softmax_ce", k_softmax_ce, dim3(n), dim3(256), 0, (const float *)p->Z,
+         (const int32_t *)p->labels, p->dZ, scale, p->stats, (uint32_t *)(p->stats + 1));
+  const size_t B = p->maxBatch;
+  const uint16_t *H[3] = {p->H, p->H + B * kMlpHid, p->H + 2 * B * kMlpHid};
+  // Output layer: dW4t = dZ^T H3 ; db4 = colsum(dZ) ; dA3 = (dZ W4t) * (H3 > 0)
+  {
+    GemmArgs g = gemm_args(p->dZ, 1, kMlpOut, H[2], kMlpHid, 1, kMlpOut, kMlpHid, (int)n);
+    g.Cf = p->g + mlp_weight_offset(3);
+    g.scm = kMlpHid;
+    g.scn = 1;
+    gemm(ctx, s, g, "mlp_gemm_bwd");
+    launch(ctx, s, "mlp_colsum", k_colsum, dim3(blocks_for(kMlpOut, 256), blocks_for(n, 256)), dim3(256), 0,
+           (const uint16_t *)p->dZ, (int)n, kMlpOut, kMlpOut, p->g + mlp_bias_offset(3));
+    GemmArgs h = gemm_args(p->dZ, kMlpOut, 1, mlp_wbf(p, 3), kMlpHid, 1, (int)n, kMlpHid, kMlpOut);
+    h.Cb = p->dA + 2 * B * kMlpHid;
+    h.scm = kMlpHid;
+    h.scn = 1;
+    h.mask = H[2];
+    h.smm = kMlpHid;
+    h.smn = 1;
+    gemm(ctx, s, h, "mlp_gemm_bwd");
+  }
+  for (int layer = 2; layer >= 0; --layer) {
+    const uint16_t *dA = p->dA + (size_t)layer * B * kMlpHid;
+    const uint16_t *in = layer == 0 ? p->X : H[layer - 1];
+    const int K = kMlpInPad;  // all hidden-layer inputs are 128 wide
+    GemmArgs g = gemm_args(dA, 1, kMlpHid, in, K, 1, kMlpHid, K, (int)n);
+    g.Cf = p->g + mlp_weight_offset(layer);
+    g.scm = K;
+    g.scn = 1;
+    gemm(ctx, s, g, "mlp_gemm_bwd");
+    launch(ctx, s, "mlp_colsum", k_colsum, dim3(blocks_for(kMlpHid, 256), blocks_for(n, 256)), dim3(256), 0, dA,
+           (int)n, kMlpHid, kMlpHid, p->g + mlp_bias_offset(layer));
+    if (layer > 0) {
+      GemmArgs h = gemm_args(dA, kMlpHid, 1, mlp_wbf(p, layer), kMlpInPad, 1, (int)n, kMlpHid, kMlpHid);
+      h.Cb = p->dA + (size_t)(layer - 1) * B * kMlpHid;
+      h.scm = kMlpHid;
+      h.scn = 1;
+      h.mask = H[layer - 1];
+      h.smm = kMlpHid;
+      h.smn = 1;
+      gemm(ctx, s, h, "mlp_gemm_bwd");
+    }
+  }
+}
+
+void mlp_adam(Context *ctx, hipStream_t s, Proposal *p, float lr) {
+  p->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - std::pow(b1, (float)p->step), c2 = 1.f - std::pow(b2, (float)p->step);
+  launch(ctx, s, "mlp_adam", k_adam, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, p->w, (const float *)p->g, p->m,
+         p->v, (uint32_t)kMlpParams, lr, b1, b2, eps, c1, c2, (const uint32_t *)p->bfMap, p->wbf);
+  mlp_refresh_bias(ctx, s, p);
+}
+
+void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint32_t n, int32_t *labels) {
+  launch(ctx, s, "mlp_labels", k_labels, dim3(blocks_for(n, 256)), dim3(256), 0, (const GPRTHit *)hits, n,
+         (const int32_t *)p->primClass, p->treePrims, labels);
+}
+
+void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit) {
+  const size_t lds = ((size_t)kInferRays * kActStride + (size_t)128 * kWStride) * 2;
+  launch(ctx, s, "mlp_infer_top1", k_infer_top1, dim3(blocks_for(n, kInferRays)), dim3(256), lds,
+         (const GPRTRay *)rays, n, mlp_wbf(p, 0), mlp_wbf(p, 1), mlp_wbf(p, 2), mlp_wbf(p, 3),
+         (const float *)p->bias, cls, logit, p->numClasses);
+}
+
+bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth) {
+  const uint32_t n = a->n;
+  if (p->treeBlock && p->treeCap < n) {
+    hipStreamSynchronize(s);
+    hipFree(p->treeBlock);
+    p->treeBlock = nullptr;
+  }
+  if (!p->treeBlock) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t bytes = al(4096 * 4) + al((size_t)n * 4) * 3 + al(64);
+    if (!hip_ok(hipMalloc(&p->treeBlock, bytes), "hipMalloc(treelets)")) return false;
+    char *b = (char *)p->treeBlock;
+    p->classRoot = (int32_t *)b;
+    p->innerClass = (int32_t *)(b + al(4096 * 4));
+    p->leafClass = (int32_t *)(b + al(4096 * 4) + al((size_t)n * 4));
+    p->primClass = (int32_t *)(b + al(4096 * 4) + 2 * al((size_t)n * 4));
+    p->numClassesDev = (uint32_t *)(b + al(4096 * 4) + 3 * al((size_t)n * 4));
+    p->treeCap = n;
+  }
+  BVHStore *st = a->store;
+  const MeshDesc &d = st->desc[a->mesh];
+  fill_u32(ctx, s, p->innerClass, 0xFFFFFFFFu, n);
+  fill_u32(ctx, s, p->leafClass, 0xFFFFFFFFu, n);
+  launch(ctx, s, "mlp_treelet_cut", k_treelet_cut, dim3(1), dim3(1024), 0, (const BVHNode *)a->nodes, n, depth,
+         p->classRoot, p->innerClass, p->leafClass, p->numClassesDev);
+  launch(ctx, s, "mlp_leaf_class", k_leaf_class, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (const uint32_t *)(st->parentLeaf + d.primOffset), (const uint32_t *)(st->parentInner + d.innerOffset),
+         (const int32_t *)p->innerClass, p->leafClass, (const uint32_t *)(st->ids + d.primOffset), p->primClass, n);
+  uint32_t nc = 0;
+  if (!hip_ok(hipMemcpyAsync(&nc, p->numClassesDev, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(classes)"))
+    return false;
+  if (!hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize(treelets)")) return false;
+  p->numClasses = nc;
+  p->treePrims = n;
+  p->accel = a;
+  return true;
+}
+
+}  // namespace gprt

@@ -1,0 +1,336 @@
+// C-ABI of the learned treelet proposal: create / bind to an accel (treelet cut) / train on
+// plain-traversal hits / infer / proposal-first trace, plus parameter and gradient access for
+// tests. None of these has a reference counterpart (see the extension block of gprt_hip.h).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bvh.hpp"
+#include "internal.hpp"
+#include "mlp.hpp"
+
+using namespace gprt;
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_label_flags(const int32_t *__restrict__ labels, uint32_t n,
+                                                     uint32_t *__restrict__ flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = labels[i] >= 0 ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_compact(const GPRTRay *__restrict__ rays, const int32_t *__restrict__ labels,
+                                                 const uint32_t *__restrict__ offs, uint32_t n,
+                                                 GPRTRay *__restrict__ outRays, int32_t *__restrict__ outLabels) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || labels[i] < 0) return;
+  const uint32_t o = offs[i];
+  outRays[o] = rays[i];
+  outLabels[o] = labels[i];
+}
+
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// He-uniform initialisation: W ~ U(-sqrt(6/fan_in), sqrt(6/fan_in)), biases 0; padded input
+// columns of the first layer stay 0. Deterministic in `seed`.
+std::vector<float> init_params(uint64_t seed) {
+  std::vector<float> w(kMlpParams, 0.f);
+  uint64_t ctr = mix64(seed * 0x9E3779B97F4A7C15ull + 0x1234567ull);
+  for (int layer = 0; layer < 4; ++layer) {
+    const int fanIn = layer == 0 ? kMlpIn : kMlpHid;
+    const int rows = layer < 3 ? kMlpHid : kMlpOut;
+    const float lim = std::sqrt(6.0f / (float)fanIn);
+    float *W = w.data() + mlp_weight_offset(layer);
+    for (int r = 0; r < rows; ++r)
+      for (int c = 0; c < fanIn; ++c) {
+        const uint64_t u = mix64(ctr += 0x9E3779B97F4A7C15ull);
+        const float x = (float)((u >> 40) * (1.0 / 16777216.0));
+        W[(size_t)r * (layer == 0 ? kMlpInPad : kMlpHid) + c] = (2.0f * x - 1.0f) * lim;
+      }
+  }
+  return w;
+}
+
+bool ensure_ray_scratch(Proposal *p, size_t bytes, hipStream_t s) {
+  if (bytes <= p->rayScratchBytes) return true;
+  if (p->rayScratch) {
+    hipStreamSynchronize(s);
+    hipFree(p->rayScratch);
+    p->rayScratch = nullptr;
+    p->rayScratchBytes = 0;
+  }
+  if (!hip_ok(hipMalloc(&p->rayScratch, bytes), "hipMalloc(proposal rays)")) return false;
+  p->rayScratchBytes = bytes;
+  return true;
+}
+
+}  // namespace
+
+GPRT_API GPRTProposal gprtProposalCreate(GPRTContext _ctx, uint32_t maxBatch, uint64_t seed) {
+  Context *ctx = (Context *)_ctx;
+  if (!ctx) {
+    fail("gprtProposalCreate: null context");
+    return nullptr;
+  }
+  if (maxBatch < 64 || maxBatch > (1u << 16)) {
+    fail("gprtProposalCreate: maxBatch %u outside [64, 65536]", maxBatch);
+    return nullptr;
+  }
+  Proposal *p = new Proposal();
+  if (!mlp_alloc(ctx, p, maxBatch)) {
+    delete p;
+    return nullptr;
+  }
+  std::vector<float> w = init_params(seed);
+  if (!mlp_set_params(ctx, ctx->stream, p, w.data())) {
+    mlp_release(p);
+    delete p;
+    return nullptr;
+  }
+  hipStreamSynchronize(ctx->stream);
+  return (GPRTProposal)p;
+}
+
+GPRT_API void gprtProposalDestroy(GPRTContext _ctx, GPRTProposal _p) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  if (!p) return;
+  if (ctx) hipStreamSynchronize(ctx->stream);
+  if (p->rayScratch) hipFree(p->rayScratch);
+  mlp_release(p);
+  delete p;
+}
+
+GPRT_API uint32_t gprtProposalBindAccel(GPRTContext _ctx, GPRTProposal _p, GPRTAccel _a, uint32_t cutDepth) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Accel *a = (Accel *)_a;
+  if (!ctx || !p || !a) {
+    fail("gprtProposalBindAccel: null argument");
+    return 0;
+  }
+  if (!a->built) {
+    fail("gprtProposalBindAccel: accel has not been built");
+    return 0;
+  }
+  if (cutDepth > (uint32_t)kMlpCutDepth) {
+    fail("gprtProposalBindAccel: cutDepth %u exceeds %d (at most 4096 treelets)", cutDepth, kMlpCutDepth);
+    return 0;
+  }
+  if (!mlp_treelets(ctx, ctx->stream, p, a, (int)cutDepth)) return 0;
+  return p->numClasses;
+}
+
+GPRT_API void gprtProposalSetParams(GPRTContext _ctx, GPRTProposal _p, const float *host, size_t count) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  if (!ctx || !p || !host) {
+    fail("gprtProposalSetParams: null argument");
+    return;
+  }
+  if (count != kMlpParams) {
+    fail("gprtProposalSetParams: expected %zu parameters, got %zu", (size_t)kMlpParams, count);
+    return;
+  }
+  mlp_set_params(ctx, ctx->stream, p, host);
+  hipStreamSynchronize(ctx->stream);
+}
+
+GPRT_API void gprtProposalGetParams(GPRTContext _ctx, GPRTProposal _p, float *host, size_t count) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  if (!ctx || !p || !host) {
+    fail("gprtProposalGetParams: null argument");
+    return;
+  }
+  if (count != kMlpParams) {
+    fail("gprtProposalGetParams: expected %zu parameters, got %zu", (size_t)kMlpParams, count);
+    return;
+  }
+  hip_ok(hipMemcpyAsync(host, p->w, kMlpParams * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync(params)");
+  hipStreamSynchronize(ctx->stream);
+}
+
+GPRT_API size_t gprtProposalParamCount(void) { return kMlpParams; }
+
+GPRT_API void gprtProposalLogits(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, uint32_t count,
+                                 GPRTBuffer _logits) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *logits = (Buffer *)_logits;
+  if (!ctx || !p || !rays || !logits) {
+    fail("gprtProposalLogits: null argument");
+    return;
+  }
+  if (count > p->maxBatch || (size_t)count * sizeof(GPRTRay) > rays->size ||
+      (size_t)count * kMlpOut * 4 > logits->size) {
+    fail("gprtProposalLogits: count %u exceeds the batch (%u) or the buffers", count, p->maxBatch);
+    return;
+  }
+  mlp_forward_batch(ctx, ctx->stream, p, rays->d, count);
+  hip_ok(hipMemcpyAsync(logits->d, p->Z, (size_t)count * kMlpOut * 4, hipMemcpyDeviceToDevice, ctx->stream),
+         "hipMemcpyAsync(logits)");
+}
+
+GPRT_API void gprtProposalGradients(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _labels,
+                                    uint32_t count, float *hostGrads, size_t gradCount) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *labels = (Buffer *)_labels;
+  if (!ctx || !p || !rays || !labels || !hostGrads) {
+    fail("gprtProposalGradients: null argument");
+    return;
+  }
+  if (gradCount != kMlpParams || count == 0 || count > p->maxBatch ||
+      (size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * 4 > labels->size) {
+    fail("gprtProposalGradients: bad sizes (count %u, batch %u, grads %zu)", count, p->maxBatch, gradCount);
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  hip_ok(hipMemcpyAsync(p->labels, labels->d, (size_t)count * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+  fill_u32(ctx, s, p->stats, 0u, 2);
+  mlp_forward_batch(ctx, s, p, rays->d, count);
+  mlp_backward_batch(ctx, s, p, count, count);
+  hip_ok(hipMemcpyAsync(hostGrads, p->g, kMlpParams * 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(grads)");
+  hipStreamSynchronize(s);
+}
+
+GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _hits,
+                                 uint32_t count, uint32_t epochs, float lr, float *accuracy) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits;
+  if (!ctx || !p || !rays || !hits) {
+    fail("gprtProposalTrain: null argument");
+    return NAN;
+  }
+  if (!p->accel || !p->primClass) {
+    fail("gprtProposalTrain: bind the proposal to an accel first (gprtProposalBindAccel)");
+    return NAN;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * sizeof(GPRTHit) > hits->size) {
+    fail("gprtProposalTrain: %u rays exceed the ray or hit buffer", count);
+    return NAN;
+  }
+  if (count == 0) return 0.f;
+  hipStream_t s = ctx->stream;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  // Scratch: labels | flags | offsets | compacted rays | compacted labels | scan state
+  const size_t need = al((size_t)count * 4) * 4 + al((size_t)count * sizeof(GPRTRay)) + al(scan_scratch_bytes(count));
+  if (!ensure_ray_scratch(p, need, s)) return NAN;
+  char *b = (char *)p->rayScratch;
+  int32_t *labels = (int32_t *)b;
+  uint32_t *flags = (uint32_t *)(b + al((size_t)count * 4));
+  uint32_t *offs = (uint32_t *)(b + 2 * al((size_t)count * 4));
+  int32_t *cLabels = (int32_t *)(b + 3 * al((size_t)count * 4));
+  GPRTRay *cRays = (GPRTRay *)(b + 4 * al((size_t)count * 4));
+  void *scanState = b + 4 * al((size_t)count * 4) + al((size_t)count * sizeof(GPRTRay));
+  const unsigned g = (count + 255) / 256;
+  mlp_labels(ctx, s, p, hits->d, count, labels);
+  launch(ctx, s, "mlp_label_flags", k_label_flags, dim3(g), dim3(256), 0, (const int32_t *)labels, count, flags);
+  scan_run(ctx, s, flags, offs, count, 0, false, scanState, nullptr);
+  launch(ctx, s, "mlp_compact", k_compact, dim3(g), dim3(256), 0, (const GPRTRay *)rays->d, (const int32_t *)labels,
+         (const uint32_t *)offs, count, cRays, cLabels);
+  uint32_t labelled = 0;
+  hip_ok(hipMemcpyAsync(&labelled, scanState, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(labelled)");
+  hipStreamSynchronize(s);
+  float lastLoss = 0.f, acc = 0.f;
+  for (uint32_t e = 0; e < epochs; ++e) {
+    fill_u32(ctx, s, p->stats, 0u, 2);
+    uint32_t batches = 0;
+    for (uint32_t b0 = 0; b0 < labelled; b0 += p->maxBatch) {
+      const uint32_t n = std::min<uint32_t>(p->maxBatch, labelled - b0);
+      hip_ok(hipMemcpyAsync(p->labels, cLabels + b0, (size_t)n * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+      mlp_forward_batch(ctx, s, p, cRays + b0, n);
+      mlp_backward_batch(ctx, s, p, n, n);
+      mlp_adam(ctx, s, p, lr);
+      ++batches;
+    }
+    float st[2];
+    hip_ok(hipMemcpyAsync(st, p->stats, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(stats)");
+    hipStreamSynchronize(s);
+    uint32_t correct;
+    std::memcpy(&correct, &st[1], 4);
+    lastLoss = batches ? st[0] / (float)batches : 0.f;
+    acc = labelled ? (float)correct / (float)labelled : 0.f;
+  }
+  if (accuracy) *accuracy = acc;
+  return lastLoss;
+}
+
+GPRT_API void gprtProposalInfer(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, uint32_t count,
+                                GPRTBuffer _classes) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *cls = (Buffer *)_classes;
+  if (!ctx || !p || !rays || !cls) {
+    fail("gprtProposalInfer: null argument");
+    return;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * 4 > cls->size) {
+    fail("gprtProposalInfer: %u rays exceed the ray or class buffer", count);
+    return;
+  }
+  if (count == 0) return;
+  mlp_infer(ctx, ctx->stream, p, rays->d, count, (int32_t *)cls->d, nullptr);
+}
+
+GPRT_API void gprtProposalExportTreelets(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _classRoot,
+                                         GPRTBuffer _primClass) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *cr = (Buffer *)_classRoot, *pc = (Buffer *)_primClass;
+  if (!ctx || !p || !cr || !pc || !p->accel) {
+    fail("gprtProposalExportTreelets: null argument or unbound proposal");
+    return;
+  }
+  if (cr->size < (size_t)p->numClasses * 4 && !buffer_realloc(cr, (size_t)p->numClasses * 4, false)) return;
+  if (pc->size < (size_t)p->treePrims * 4 && !buffer_realloc(pc, (size_t)p->treePrims * 4, false)) return;
+  hip_ok(hipMemcpyAsync(cr->d, p->classRoot, (size_t)p->numClasses * 4, hipMemcpyDeviceToDevice, ctx->stream),
+         "hipMemcpyAsync");
+  hip_ok(hipMemcpyAsync(pc->d, p->primClass, (size_t)p->treePrims * 4, hipMemcpyDeviceToDevice, ctx->stream),
+         "hipMemcpyAsync");
+}
+
+GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
+                                         GPRTBuffer _hits, uint32_t count, GPRTBuffer _visits) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits, *visits = (Buffer *)_visits;
+  if (!ctx || !a || !rays || !hits) {
+    fail("gprtAccelTraceClosestHitEx: null argument");
+    return;
+  }
+  if (!a->built) {
+    fail("gprtAccelTraceClosestHitEx: accel has not been built");
+    return;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * sizeof(GPRTHit) > hits->size ||
+      (visits && (size_t)count * 4 > visits->size)) {
+    fail("gprtAccelTraceClosestHitEx: %u rays exceed a buffer", count);
+    return;
+  }
+  if (p && p->accel != a) {
+    fail("gprtAccelTraceClosestHitEx: the proposal is bound to another accel");
+    return;
+  }
+  if (p && p->treePrims != a->n) {
+    fail("gprtAccelTraceClosestHitEx: the accel was rebuilt with a different size; bind the proposal again");
+    return;
+  }
+  if (count == 0) return;
+  hipStream_t s = ctx->stream;
+  int32_t *cls = nullptr;
+  if (p) {
+    if (!ensure_ray_scratch(p, (size_t)count * 4, s)) return;
+    cls = (int32_t *)p->rayScratch;
+    mlp_infer(ctx, s, p, rays->d, count, cls, nullptr);
+  }
+  trace_closest_ex(ctx, s, a, rays->d, hits->d, count, cls, p ? p->classRoot : nullptr, p ? p->numClasses : 0,
+                   visits ? (uint32_t *)visits->d : nullptr);
+}

@@ -12,6 +12,10 @@
 // inner child is descended and the far one pushed on a per-lane stack whose first kLdsStack
 // entries live in LDS and the rest in a per-lane global spill area. A binary radix tree over
 // distinct 64-bit keys is at most 64 levels deep, so 64 entries always suffice.
+//
+// With a proposal, each ray first traverses the subtree of its proposed treelet (a real hit found
+// there becomes the starting (t, primID)), then the ordinary traversal from the root runs with
+// that tighter bound; the lexicographic (t, primID) rule makes the result identical.
 #include "bvh.hpp"
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -21,29 +25,90 @@ namespace gprt {
 namespace {
 
 constexpr int kTraceBlock = 128;
-constexpr int kLdsStack = 24;
+constexpr int kLdsStack = 20;
+constexpr int kBlocksPerCU = 14;  // 28 waves per CU: the register-limited occupancy at ~60 VGPRs
 constexpr int kMaxStack = 64;
 
+struct HitState {
+  float t, u, v;
+  int prim;
+};
+
 __device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord *__restrict__ tris, int leaf,
-                                               uint32_t nprims, float &tbest, int &best, float &bu, float &bv) {
+                                               uint32_t nprims, HitState &h) {
   if ((uint32_t)leaf >= nprims) return;
   const TriRecord tr = tris[leaf];
   float t, u, v;
   if (tri_hit(r, tr.v0, tr.v1, tr.v2, t, u, v)) {
     const int prim = (int)__float_as_uint(tr.v0.w);
-    if (t < tbest || (t == tbest && (unsigned)prim < (unsigned)best)) {
-      tbest = t;
-      best = prim;
-      bu = u;
-      bv = v;
+    if (t < h.t || (t == h.t && (unsigned)prim < (unsigned)h.prim)) {
+      h.t = t;
+      h.prim = prim;
+      h.u = u;
+      h.v = v;
     }
   }
+}
+
+// Depth-first traversal of the subtree rooted at `start` (an inner node index, or ~leaf).
+__device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__restrict__ nodes,
+                                             const TriRecord *__restrict__ tris, uint32_t nprims, int start,
+                                             HitState &h, uint32_t (*s_stack)[kTraceBlock], int tid,
+                                             uint32_t *__restrict__ mySpill) {
+  if (start < 0) {
+    intersect_leaf(r, tris, ~start, nprims, h);
+    return 0;
+  }
+  int node = start;
+  int sp = 0;
+  uint32_t visits = 0;
+  // Each inner node is fetched at most once per traversal; the step bound and the index check
+  // only guard against a malformed tree.
+  for (uint32_t step = 0; step < nprims; ++step) {
+    if ((uint32_t)node >= nprims - 1) break;
+    const BVHNode nd = nodes[node];
+    ++visits;
+    float tn0, tn1;
+    bool h0 = box_hit(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, h.t, tn0);
+    bool h1 = box_hit(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, h.t, tn1);
+    const int c0 = nd.d.x, c1 = nd.d.y;
+    if (h0 && c0 < 0) {
+      intersect_leaf(r, tris, ~c0, nprims, h);
+      h0 = false;
+    }
+    if (h1 && c1 < 0) {
+      intersect_leaf(r, tris, ~c1, nprims, h);
+      h1 = false;
+    }
+    if (h0 && h1) {
+      const bool firstNear = tn0 <= tn1;
+      const int nearC = firstNear ? c0 : c1;
+      const int farC = firstNear ? c1 : c0;
+      if (sp < kLdsStack)
+        s_stack[sp][tid] = (uint32_t)farC;
+      else if (sp < kMaxStack)
+        mySpill[sp - kLdsStack] = (uint32_t)farC;
+      ++sp;
+      node = nearC;
+    } else if (h0) {
+      node = c0;
+    } else if (h1) {
+      node = c1;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      if (sp >= kMaxStack) break;  // unreachable for trees of depth <= 64
+      node = (int)(sp < kLdsStack ? s_stack[sp][tid] : mySpill[sp - kLdsStack]);
+    }
+  }
+  return visits;
 }
 
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const BVHNode *__restrict__ nodes, const TriRecord *__restrict__ tris, uint32_t nprims,
             const GPRTRay *__restrict__ rays, GPRTHit *__restrict__ hits, uint32_t count,
-            uint32_t *__restrict__ work, uint32_t *__restrict__ spill) {
+            uint32_t *__restrict__ work, uint32_t *__restrict__ spill, const int32_t *__restrict__ proposal,
+            const int32_t *__restrict__ classRoot, uint32_t numClasses, uint32_t *__restrict__ visitsOut) {
   __shared__ uint32_t s_stack[kLdsStack][kTraceBlock];
   const int tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * kTraceBlock + tid;
@@ -57,70 +122,39 @@ __global__ void __launch_bounds__(kTraceBlock)
     if (base >= count) break;
     const uint32_t ri = base + (uint32_t)l;
     if (ri < count) {
-    const GPRTRay ray = rays[ri];
-    const RayPre r = ray_setup(ray.ox, ray.oy, ray.oz, ray.tmin, ray.dx, ray.dy, ray.dz);
-    float tbest = ray.tmax, bu = 0.f, bv = 0.f;
-    int best = -1;
-
-    if (nprims == 1) {
-      intersect_leaf(r, tris, 0, nprims, tbest, best, bu, bv);
-    } else {
-      int node = 0;
-      int sp = 0;
-      // Each inner node is fetched at most once per ray; the step bound and the index check
-      // only guard against a malformed tree.
-      for (uint32_t step = 0; step < nprims; ++step) {
-        if ((uint32_t)node >= nprims - 1) break;
-        const BVHNode nd = nodes[node];
-        float tn0, tn1;
-        bool h0 = box_hit(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tbest, tn0);
-        bool h1 = box_hit(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tbest, tn1);
-        const int c0 = nd.d.x, c1 = nd.d.y;
-        if (h0 && c0 < 0) {
-          intersect_leaf(r, tris, ~c0, nprims, tbest, best, bu, bv);
-          h0 = false;
-        }
-        if (h1 && c1 < 0) {
-          intersect_leaf(r, tris, ~c1, nprims, tbest, best, bu, bv);
-          h1 = false;
-        }
-        if (h0 && h1) {
-          const bool firstNear = tn0 <= tn1;
-          const int nearC = firstNear ? c0 : c1;
-          const int farC = firstNear ? c1 : c0;
-          if (sp < kLdsStack)
-            s_stack[sp][tid] = (uint32_t)farC;
-          else if (sp < kMaxStack)
-            mySpill[sp - kLdsStack] = (uint32_t)farC;
-          ++sp;
-          node = nearC;
-        } else if (h0) {
-          node = c0;
-        } else if (h1) {
-          node = c1;
-        } else {
-          if (sp == 0) break;
-          --sp;
-          if (sp >= kMaxStack) break;  // unreachable for trees of depth <= 64
-          node = (int)(sp < kLdsStack ? s_stack[sp][tid] : mySpill[sp - kLdsStack]);
+      const GPRTRay ray = rays[ri];
+      const RayPre r = ray_setup(ray.ox, ray.oy, ray.oz, ray.tmin, ray.dx, ray.dy, ray.dz);
+      HitState h;
+      h.t = ray.tmax;
+      h.u = h.v = 0.f;
+      h.prim = -1;
+      uint32_t visits = 0;
+      if (proposal) {
+        const int32_t c = proposal[ri];
+        if (c >= 0 && (uint32_t)c < numClasses) {
+          const int32_t root = classRoot[c];
+          // A treelet rooted at the tree's root (cut depth 0) would only repeat the full pass.
+          if (root != 0) visits += traverse(r, nodes, tris, nprims, root, h, s_stack, tid, mySpill);
         }
       }
-    }
-    GPRTHit h;
-    h.t = best >= 0 ? tbest : ray.tmax;
-    h.primID = best;
-    h.u = bu;
-    h.v = bv;
-    hits[ri] = h;
+      visits += traverse(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, s_stack, tid, mySpill);
+      GPRTHit o;
+      o.t = h.prim >= 0 ? h.t : ray.tmax;
+      o.primID = h.prim;
+      o.u = h.u;
+      o.v = h.v;
+      hits[ri] = o;
+      if (visitsOut) visitsOut[ri] = visits;
     }
   }
 }
 
 }  // namespace
 
-bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count) {
+bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
+                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits) {
   if (count == 0) return true;
-  const unsigned blocks = (unsigned)ctx->cuCount * 8;
+  const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
   const size_t need = (size_t)blocks * kTraceBlock * (kMaxStack - kLdsStack) * 4 + 256;
   if (need > ctx->traceBytes) {
     if (ctx->traceScratch) {
@@ -135,9 +169,14 @@ bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void
   uint32_t *work = (uint32_t *)ctx->traceScratch;
   void *spill = (char *)ctx->traceScratch + 256;
   fill_u32(ctx, s, work, 0u, 1);
-  launch(ctx, s, "trace_closest", k_trace, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
-         (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, (uint32_t *)spill);
+  launch(ctx, s, proposal ? "trace_closest_proposal" : "trace_closest", k_trace, dim3(blocks), dim3(kTraceBlock), 0,
+         (const BVHNode *)a->nodes, (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count,
+         work, (uint32_t *)spill, proposal, classRoot, numClasses, visits);
   return true;
+}
+
+bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count) {
+  return trace_closest_ex(ctx, s, a, rays, hits, count, nullptr, nullptr, 0, nullptr);
 }
 
 }  // namespace gprt

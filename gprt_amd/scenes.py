@@ -143,3 +143,21 @@ def random_rays(n: int, seed: int, lo=-0.5, hi=1.5) -> np.ndarray:
     r["tmin"] = 0.0
     r["tmax"] = TMAX
     return r
+
+
+def jittered_pinhole_rays(n: int, seed: int, width: int = 2048, height: int = 2048, origin=(0.5, 0.5, -1.0),
+                          fov_deg: float = 60.0) -> np.ndarray:
+    """Pinhole rays through uniformly random image-plane points (a seeded variant of pinhole_rays,
+    used for training data drawn from the same distribution as the primary rays)."""
+    tan_half = math.tan(math.radians(fov_deg) / 2.0)
+    aspect = width / height
+    px = (rand01(seed, 40, n) * 2.0 - 1.0) * tan_half * aspect
+    py = (rand01(seed, 41, n) * 2.0 - 1.0) * tan_half
+    d = np.stack([px, py, np.ones(n)], axis=1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros(n, dtype=RAY_DTYPE)
+    r["ox"], r["oy"], r["oz"] = origin
+    r["tmin"] = 0.0
+    r["dx"], r["dy"], r["dz"] = d[:, 0], d[:, 1], d[:, 2]
+    r["tmax"] = TMAX
+    return r

@@ -215,6 +215,45 @@ typedef struct GPRTHit {
 GPRT_API void gprtAccelTraceClosestHit(GPRTContext context, GPRTAccel accel, GPRTBuffer rays, GPRTBuffer hits,
                                        uint32_t count);
 
+/* Learned treelet proposal (no reference counterpart). A 102-128-128-128-4096 MLP in bf16 on
+ * MFMA (FP32 accumulate) maps a ray (origin, direction with 8-band sin/cos encoding) to one of
+ * the treelets obtained by cutting the accel's LBVH at `cutDepth` (<= 12: at most 4096 treelets).
+ * It is trained on the device (hand-written forward / backward / Adam) with labels taken from
+ * the plain traversal's hits. gprtAccelTraceClosestHitEx with a proposal first traverses the
+ * proposed treelet to get a hit that tightens tmax, then runs the exact traversal from the root:
+ * hits are identical to the plain trace (the plain traversal is the fallback and the judge).
+ * Parameters use the flat fp32 layout W1t b1 W2t b2 W3t b3 W4t b4 with Wt[out][in] and the
+ * first layer's input padded to 128 (gprtProposalParamCount() values). */
+typedef struct GPRTProposal_T *GPRTProposal;
+GPRT_API GPRTProposal gprtProposalCreate(GPRTContext context, uint32_t maxBatch, uint64_t seed);
+GPRT_API void gprtProposalDestroy(GPRTContext context, GPRTProposal proposal);
+/* Cuts the accel's tree at cutDepth; returns the number of treelets. Rebind after a rebuild. */
+GPRT_API uint32_t gprtProposalBindAccel(GPRTContext context, GPRTProposal proposal, GPRTAccel accel,
+                                        uint32_t cutDepth);
+GPRT_API size_t gprtProposalParamCount(void);
+GPRT_API void gprtProposalSetParams(GPRTContext context, GPRTProposal proposal, const float *params, size_t count);
+GPRT_API void gprtProposalGetParams(GPRTContext context, GPRTProposal proposal, float *params, size_t count);
+/* Epochs of minibatch Adam over the rays the plain traversal hit (hits: GPRTHit per ray, as
+ * written by gprtAccelTraceClosestHit). Returns the last epoch's mean loss; *accuracy = top-1. */
+GPRT_API float gprtProposalTrain(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, GPRTBuffer hits,
+                                 uint32_t count, uint32_t epochs, float learningRate, float *accuracy);
+/* Top-1 treelet per ray (int32), fused: the 4096 logits are never written to memory. */
+GPRT_API void gprtProposalInfer(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, uint32_t count,
+                                GPRTBuffer classes);
+/* Test access: fp32 logits (count x 4096, count <= maxBatch) and the gradient of the mean
+ * cross-entropy over `count` labelled rays (int32 labels) in the flat parameter layout. */
+GPRT_API void gprtProposalLogits(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, uint32_t count,
+                                 GPRTBuffer logits);
+GPRT_API void gprtProposalGradients(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, GPRTBuffer labels,
+                                    uint32_t count, float *grads, size_t gradCount);
+/* classRoot[c]: treelet root (inner node index, or ~leaf position); primClass[prim]: its treelet. */
+GPRT_API void gprtProposalExportTreelets(GPRTContext context, GPRTProposal proposal, GPRTBuffer classRoot,
+                                         GPRTBuffer primClass);
+/* Closest hit with an optional proposal (may be NULL) and optional per-ray count of inner nodes
+ * fetched (visits, uint32, may be NULL). */
+GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext context, GPRTAccel accel, GPRTProposal proposal,
+                                         GPRTBuffer rays, GPRTBuffer hits, uint32_t count, GPRTBuffer visits);
+
 /* Export of the built LBVH in the reference's development layout (gprt/development/gprt_lbvh.h:28-55):
  * nodes int4[2N-1] {left, right, parent, leaf primID or -1} with inner nodes [0, N-2] and leaf
  * i at N-1+i; aabbs float3[2(2N-1)] {min, max} per node; keys uint64[N] sorted
