@@ -1,20 +1,30 @@
-"""Benchmark: LBVH build Mprims/s + closest-hit Mrays/s on one MI355X.
+"""Benchmark: LBVH build Mprims/s + closest-hit Mrays/s (plain and learned proposal), one MI355X.
 
-Workload (the large single scene): 4,194,304 triangles from 64 Gaussian blobs (FP32), 8,388,608
-rays = 2048x2048 pinhole primary rays + 4,194,304 cosine-weighted bounce rays from the primary
-hits. A step is one full LBVH build (bounds, Morton keys, key sort, Karras, refit) followed by a
-closest-hit trace of all 8,388,608 rays, with every input resident in HBM before timing starts.
+Default workload (the large single scene): 4,194,304 triangles from 64 Gaussian blobs (FP32);
+8,388,608 rays = 2048x2048 pinhole primary rays + 4,194,304 cosine-weighted bounce rays from the
+primary hits. A step is one full LBVH build (bounds, Morton keys, key sort, Karras, refit)
+followed by a closest-hit trace of all rays, once plainly and once with the learned treelet
+proposal (top-1 treelet from the bf16 MFMA MLP, traversed first to tighten tmax, then the exact
+traversal). Every input is resident in HBM before timing starts. Before the timed steps the
+proposal network is trained on the device on rays drawn from the same distribution with other
+seeds (jittered primary + bounce), labelled by the plain traversal's hits; training is reported,
+not timed in the step. Hits of the two traces are compared for every ray in the timed steps.
 
 `value` is the build throughput (primitives / build time). Build and trace times come from HIP
 events the library records around every kernel on its stream during the timed steps; the whole
-timed region is also wall-clocked between two stream synchronisations. The roofline entries use
+timed region is also wall-clocked between two stream synchronisations. Roofline entries use
 algorithmic bytes per launch (fixed by the workload, not by tuning knobs):
-  * sort pass (u32 Morton key + u32 primID payload): read 8 B + write 8 B per key;
+  * LBVH sort pass (u32 Morton key + u32 primID payload): read 8 B + write 8 B per key;
   * trace: 32 B ray + 16 B hit + 64 B per inner node fetched + 48 B per triangle tested, with
-    the node / triangle counts measured by the CPU reference traversal on a ray sample.
+    the node / triangle counts measured by the CPU reference traversal on a ray sample;
+  * proposal inference: 1,140,224 FLOP per ray against the bf16 dense MFMA peak.
 The CPU baseline times the oracle (single thread) on a bounded sample and is reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other workloads (one JSON line each): --workload batched (4,096 meshes, log-uniform 64..262,144
+triangles: per-mesh launch sequence vs one hipGraph-captured batch launch vs the autotuned
+configuration). bench.py --gpus N with N > 1 exits: this build measures one GPU.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload scene|batched]
 """
 from __future__ import annotations
 
@@ -31,35 +41,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_RAY = 2 * (102 * 128 + 2 * 128 * 128 + 128 * 4096)
 N_TRIS = 4_194_304
 IMG = 2048
 N_BOUNCE = 4_194_304
+LBVH_BYTES_PER_PRIM = 368
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_ms(lib, ctx, name):
-    n = C.c_uint32(0)
-    ms = lib.gprtKernelTimerQuery(ctx.h, name.encode(), C.byref(n))
-    return float(ms), int(n.value)
+def kernel_report(lib, ctx):
+    buf = C.create_string_buffer(1 << 16)
+    lib.gprtKernelTimerReport(ctx.h, buf, len(buf))
+    out = {}
+    for line in buf.value.decode().strip().splitlines():
+        name, n, ms = line.split(",")
+        out[name] = {"launches": int(n), "total_ms": float(ms)}
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rays", type=int, default=16384)
-    ap.add_argument("--cpu-build-tris", type=int, default=1 << 20)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
-              f"(the path has no multi-GPU form here). Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def ms_of(per_kernel, prefix):
+    ms = sum(r["total_ms"] for k, r in per_kernel.items() if k.startswith(prefix))
+    n = sum(r["launches"] for k, r in per_kernel.items() if k.startswith(prefix))
+    return ms, n
 
+
+def scene_workload(args):
     from gprt_amd import api, scenes
 
     lib = api.load()
@@ -67,32 +77,57 @@ def main():
     verts, idx = scenes.blob_scene(N_TRIS, seed=2)
     prim = scenes.pinhole_rays(IMG, IMG)
     log(f"scene + primary rays generated in {time.time() - t0:.1f}s")
-
     ctx = api.Context(0)
     acc = api.TriangleAccel(ctx, verts, idx)
     acc.build()
-    pb = ctx.device_buffer(prim)
-    phb = ctx.device_buffer(nbytes=len(prim) * 16)
-    acc.trace(pb, phb, len(prim))
-    ph = phb.read(api.HIT_DTYPE)
-    t0 = time.time()
+
+    def trace_np(rays):
+        rb = ctx.device_buffer(rays)
+        hb = ctx.device_buffer(nbytes=len(rays) * 16)
+        acc.trace(rb, hb, len(rays))
+        h = hb.read(api.HIT_DTYPE)
+        rb.destroy()
+        hb.destroy()
+        return h
+
+    ph = trace_np(prim)
     bounce = scenes.bounce_rays(prim, ph, verts, idx, N_BOUNCE, seed=3)
     rays = np.concatenate([prim, bounce])
-    log(f"bounce rays generated in {time.time() - t0:.1f}s; primary hit rate {(ph['primID'] >= 0).mean():.3f}")
-    pb.destroy()
-    phb.destroy()
     n_rays = len(rays)
+    log(f"rays ready; primary hit rate {(ph['primID'] >= 0).mean():.3f}")
+
+    # Proposal training data: same distribution, other seeds.
+    t0 = time.time()
+    tr = scenes.jittered_pinhole_rays(args.train_rays // 2, seed=6)
+    th = trace_np(tr)
+    tr = np.concatenate([tr, scenes.bounce_rays(tr, th, verts, idx, args.train_rays // 2, seed=16)])
+    th = trace_np(tr)
+    gen_s = time.time() - t0
+    prop = api.Proposal(ctx, max_batch=args.batch, seed=1)
+    classes = prop.bind(acc, 12)
+    trb, thb = ctx.device_buffer(tr), ctx.device_buffer(th)
+    t0 = time.time()
+    loss, train_acc = prop.train(trb, thb, len(tr), epochs=args.epochs, lr=args.lr)
+    train_s = time.time() - t0
+    trb.destroy()
+    thb.destroy()
+    log(f"proposal: {classes} treelets; trained on {len(tr)} rays x {args.epochs} epochs in {train_s:.1f}s "
+        f"(data {gen_s:.1f}s): loss {loss:.3f}, train top-1 {train_acc:.3f}")
+
     rb = ctx.device_buffer(rays)
     hb = ctx.device_buffer(nbytes=n_rays * 16)
+    hb2 = ctx.device_buffer(nbytes=n_rays * 16)
 
     def step():
+        # The rebuild is deterministic (same keys, same tree), so the proposal's treelet cut stays
+        # valid; after a geometry change a stale cut would cost speed, never correctness.
         acc.build()
         acc.trace(rb, hb, n_rays)
+        lib.gprtAccelTraceClosestHitEx(ctx.h, acc.h, prop.h, rb.h, hb2.h, n_rays, None)
 
     for _ in range(args.warmup):
         step()
     ctx.sync()
-
     lib.gprtKernelTimerReset(ctx.h)
     lib.gprtKernelTimerEnable(ctx.h, True)
     ctx.sync()
@@ -102,109 +137,215 @@ def main():
     ctx.sync()
     wall = time.perf_counter() - t0
     lib.gprtKernelTimerEnable(ctx.h, False)
-
     K = args.steps
-    build_names = ["lbvh_"]
-    trace_ms_tot, trace_launches = kernel_ms(lib, ctx, "trace_closest")
-    build_ms_tot = 0.0
-    buf = C.create_string_buffer(1 << 16)
-    lib.gprtKernelTimerReport(ctx.h, buf, len(buf))
-    per_kernel = {}
-    for line in buf.value.decode().strip().splitlines():
-        name, n, ms = line.split(",")
-        per_kernel[name] = {"launches": int(n), "total_ms": float(ms)}
-    for name, rec in per_kernel.items():
-        if any(name.startswith(p) for p in build_names):
-            build_ms_tot += rec["total_ms"]
-    build_ms = build_ms_tot / K
-    trace_ms = trace_ms_tot / K
-    ms_per_step = wall * 1e3 / K
-    build_mprims = N_TRIS / (build_ms * 1e3)
-    trace_mrays = n_rays / (trace_ms * 1e3)
-
+    per_kernel = kernel_report(lib, ctx)
     for name, rec in sorted(per_kernel.items(), key=lambda kv: -kv[1]["total_ms"]):
-        log(f"  {name:24s} {rec['launches']:5d} launches {rec['total_ms'] / K:9.3f} ms/step "
+        log(f"  {name:26s} {rec['launches']:5d} launches {rec['total_ms'] / K:9.3f} ms/step "
             f"{rec['total_ms'] / rec['launches']:9.4f} ms/launch")
-    hits = hb.read(api.HIT_DTYPE)
-    hit_rate = float((hits["primID"] >= 0).mean())
+    build_ms = ms_of(per_kernel, "lbvh_")[0] / K
+    cut_ms = (ms_of(per_kernel, "mlp_treelet")[0] + ms_of(per_kernel, "mlp_leaf_class")[0]) / K
+    trace_ms_tot, trace_n = ms_of(per_kernel, "trace_closest")
+    plain_ms = (trace_ms_tot - ms_of(per_kernel, "trace_closest_proposal")[0]) / K
+    prop_trace_ms = ms_of(per_kernel, "trace_closest_proposal")[0] / K
+    infer_ms_tot, infer_n = ms_of(per_kernel, "mlp_infer_top1")
+    infer_ms = infer_ms_tot / K
+    learned_ms = infer_ms + prop_trace_ms
+    h1, h2 = hb.read(api.HIT_DTYPE), hb2.read(api.HIT_DTYPE)
+    mismatches = int(np.sum((h1["primID"] != h2["primID"]) | (h1["t"].view(np.uint32) != h2["t"].view(np.uint32))))
+    hit_rate = float((h1["primID"] >= 0).mean())
 
-    # Roofline of the dominant build kernel: one onesweep pass over u32 key + u32 payload.
-    sw_ms, sw_n = kernel_ms(lib, ctx, "lbvh_sort_pass")
-    sw_avg_ms = sw_ms / max(sw_n, 1)
-    sw_bytes = N_TRIS * 16
-    roof_sort = {"kernel": "lbvh_sort_pass", "bound": "hbm", "achieved": sw_bytes / (sw_avg_ms * 1e-3) / 1e9,
-                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-                 "avg_launch_ms": sw_avg_ms, "bytes_per_launch": sw_bytes}
+    sw_ms, sw_n = ms_of(per_kernel, "lbvh_sort_pass")
+    sw_avg = sw_ms / max(sw_n, 1)
+    roof_sort = {"kernel": "lbvh_sort_pass", "bound": "hbm", "achieved": N_TRIS * 16 / (sw_avg * 1e-3) / 1e9,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "avg_launch_ms": sw_avg,
+                 "bytes_per_launch": N_TRIS * 16}
     roof_sort["frac"] = roof_sort["achieved"] / HBM_PEAK_GBS
+    roof_build = {"bound": "hbm", "achieved": N_TRIS * LBVH_BYTES_PER_PRIM / (build_ms * 1e-3) / 1e9,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": LBVH_BYTES_PER_PRIM}
+    roof_build["frac"] = roof_build["achieved"] / HBM_PEAK_GBS
+    infer_avg = infer_ms_tot / max(infer_n, 1)
+    roof_mlp = {"kernel": "mlp_infer_top1", "bound": "mfma",
+                "achieved": n_rays * MLP_FLOP_PER_RAY / (infer_avg * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "avg_launch_ms": infer_avg}
+    roof_mlp["frac"] = roof_mlp["achieved"] / BF16_PEAK_TFLOPS
 
     cpu = None
-    trace_bytes = None
+    roof_trace = None
     if not args.no_cpu:
         from oracle import binding as orc
-        # Node / triangle visit counts on a ray sample (for the trace's algorithmic bytes) and the
-        # single-thread CPU traversal rate on the same BVH.
         keys, nodes, aabbs = acc.export()
         sel = np.random.default_rng(0).choice(n_rays, args.cpu_rays, replace=False)
         t0 = time.perf_counter()
         ch, counters = orc.trace_bvh(rays[sel], verts, idx, nodes, aabbs, threads=1)
         cpu_trace_s = time.perf_counter() - t0
-        mism = int((ch["primID"] != hits["primID"][sel]).sum())
+        mism = int((ch["primID"] != h1["primID"][sel]).sum())
         nodes_per_ray = counters[0] / len(sel)
         tris_per_ray = counters[1] / len(sel)
         trace_bytes = 32 + 16 + 64 * nodes_per_ray + 48 * tris_per_ray
-        # Single-thread oracle LBVH build on a bounded prefix of the scene.
         m = args.cpu_build_tris
         t0 = time.perf_counter()
         orc.lbvh_build(verts[: 3 * m], idx[:m])
         cpu_build_s = time.perf_counter() - t0
         cpu = {"value": m / cpu_build_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
-               "sample": f"oracle LBVH build of the first {m} triangles of the scene (single thread); "
-                         f"oracle BVH traversal of {len(sel)} sampled rays: "
-                         f"{len(sel) / cpu_trace_s / 1e6:.4f} Mrays/s single thread",
-               "trace_mrays_s": len(sel) / cpu_trace_s / 1e6,
-               "host_cpus": os.cpu_count(), "trace_mismatches_vs_gpu": mism}
+               "sample": f"oracle LBVH build of the first {m} triangles of the scene, single thread; oracle BVH "
+                         f"traversal of {len(sel)} sampled rays of the workload, single thread "
+                         f"({len(sel) / cpu_trace_s / 1e6:.4f} Mrays/s)",
+               "trace_mrays_s": len(sel) / cpu_trace_s / 1e6, "host_cpus": os.cpu_count(),
+               "trace_mismatches_vs_gpu": mism}
+        trace_avg = plain_ms
+        ach = n_rays * trace_bytes / (trace_avg * 1e-3) / 1e9
+        roof_trace = {"kernel": "trace_closest", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": trace_avg,
+                      "bytes_per_ray": trace_bytes, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray}
         log(f"cpu: build {cpu['value']:.3f} Mprims/s, trace {cpu['trace_mrays_s']:.4f} Mrays/s; "
             f"{nodes_per_ray:.1f} nodes + {tris_per_ray:.1f} tris per ray")
 
-    roof_trace = None
-    if trace_bytes is not None:
-        avg_ms = trace_ms_tot / max(trace_launches, 1)
-        ach = n_rays * trace_bytes / (avg_ms * 1e-3) / 1e9
-        roof_trace = {"kernel": "trace_closest", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
-                      "bytes_per_ray": trace_bytes}
-    # The dominant kernel of a step is the trace (8.4M rays) unless the build outweighs it.
-    dominant = roof_trace if (roof_trace and trace_ms > sw_avg_ms) else roof_sort
-
     out = {
         "metric": "LBVH build Mprims/s + closest-hit Mrays/s, plain vs learned-proposal, 1 GPU",
-        "value": build_mprims,
+        "value": N_TRIS / (build_ms * 1e3),
         "unit": "Mprims/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded blob scene + pinhole/bounce rays)",
-        "config": {"workload": "blob scene 4,194,304 tris, 64 blobs; 8,388,608 rays (2048^2 primary + "
-                               "4,194,304 bounce); step = LBVH build + closest-hit trace",
-                   "triangles": N_TRIS, "rays": n_rays},
-        "components": {"build_ms": build_ms, "trace_ms": trace_ms, "build_mprims_s": build_mprims,
-                       "trace_mrays_s_plain": trace_mrays, "trace_mrays_s_learned": None,
-                       "hit_rate": hit_rate, "wall_ms_per_step": ms_per_step},
-        "roofline": dominant,
-        "roofline_build": roof_sort,
+        "dtype": "f32 (geometry, traversal); bf16 MFMA with f32 accumulate (proposal MLP)",
+        "data": "synthetic (seeded blob scene, pinhole + bounce rays; random-init proposal trained on the device)",
+        "config": {"workload": "blob scene 4,194,304 tris (64 blobs); 8,388,608 rays (2048^2 primary + 4,194,304 "
+                               "bounce); step = LBVH build + plain trace + learned-proposal trace",
+                   "triangles": N_TRIS, "rays": n_rays, "treelets": classes, "train_rays": len(tr),
+                   "train_epochs": args.epochs},
+        "components": {"build_ms": build_ms, "build_mprims_s": N_TRIS / (build_ms * 1e3),
+                       "trace_ms_plain": plain_ms, "trace_mrays_s_plain": n_rays / (plain_ms * 1e3),
+                       "trace_ms_learned": learned_ms, "trace_mrays_s_learned": n_rays / (learned_ms * 1e3),
+                       "learned_infer_ms": infer_ms, "learned_traverse_ms": prop_trace_ms,
+                       "learned_vs_plain": plain_ms / learned_ms, "treelet_cut_ms": cut_ms,
+                       "hit_mismatches_learned_vs_plain": mismatches, "hit_rate": hit_rate,
+                       "train_s": train_s, "train_loss": loss, "train_top1": train_acc},
+        "roofline": roof_trace if roof_trace else roof_sort,
+        "roofline_build": roof_build,
+        "roofline_sort_pass": roof_sort,
+        "roofline_mlp": roof_mlp,
         "cpu_baseline": cpu,
         "kernels": per_kernel,
     }
     print(json.dumps(out))
-    rb.destroy()
-    hb.destroy()
+    for b in (rb, hb, hb2):
+        b.destroy()
+    prop.destroy()
     acc.destroy()
     ctx.close()
+
+
+def batched_workload(args):
+    from gprt_amd import api, scenes
+
+    lib = api.load()
+    ctx = api.Context(0)
+    sizes = scenes.batch_sizes(args.meshes, seed=3)
+    total = int(sizes.sum())
+    t0 = time.time()
+    accs = []
+    for i, n in enumerate(sizes):
+        vb, ib = ctx.device_buffer(nbytes=36 * int(n)), ctx.device_buffer(nbytes=12 * int(n))
+        one = np.array([n], dtype=np.int64)
+        lib.gprtSynthBlobMeshes(ctx.h, one.ctypes.data_as(C.c_void_p), 1, 1000 + i, 0.002, vb.h, ib.h)
+        accs.append(api.TriangleAccel.from_buffers(ctx, vb, ib, 3 * int(n), int(n)))
+    ctx.sync()
+    log(f"{len(accs)} meshes, {total} triangles generated on the device in {time.time() - t0:.1f}s")
+    ev = (C.c_void_p * len(accs))(*[a.h for a in accs])
+    params = api.GPRTBuildParams()
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)) * 1e3
+
+    def per_mesh():
+        for a in accs:
+            lib.gprtAccelBuild(ctx.h, a.h, params)
+
+    def batch():
+        lib.gprtAccelBuildBatch(ctx.h, ev, len(accs), params)
+
+    per_mesh_ms = timed(per_mesh, args.steps)
+    lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
+    batch_ms = timed(batch, args.steps)
+    rep = C.create_string_buffer(1 << 14)
+    best = lib.gprtAccelBuildBatchAutotune(ctx.h, ev, len(accs), 3, rep, len(rep))
+    tune = json.loads(rep.value.decode())
+    tuned_ms = timed(batch, args.steps)
+    # device-event time of one graph replay (the batch build alone)
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    for _ in range(args.steps):
+        batch()
+    ctx.sync()
+    g_ms, g_n = ms_of(kernel_report(lib, ctx), "lbvh_batch_graph")
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    graph_ms = g_ms / max(g_n, 1)
+    ach = total * LBVH_BYTES_PER_PRIM / (graph_ms * 1e-3) / 1e9
+    out = {
+        "metric": "batched LBVH build Mprims/s (4,096 uneven meshes, one hipGraph launch), 1 GPU",
+        "value": total / (graph_ms * 1e3),
+        "unit": "Mprims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": tuned_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated blob meshes, sizes log-uniform 64..262,144)",
+        "config": {"workload": f"{len(accs)} meshes, {total} triangles", "meshes": len(accs), "triangles": total},
+        "components": {"per_mesh_launches_ms": per_mesh_ms, "batch_graph_default_ms": batch_ms,
+                       "batch_graph_tuned_ms": tuned_ms, "batch_graph_device_ms": graph_ms,
+                       "speedup_vs_per_mesh": per_mesh_ms / tuned_ms, "autotune_best": tune["best"],
+                       "autotune_best_index": int(best)},
+        "roofline": {"kernel": "lbvh batch graph (all build kernels)", "bound": "hbm", "achieved": ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_prim": LBVH_BYTES_PER_PRIM},
+        "autotune": tune,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    for a in accs:
+        a.destroy()
+    ctx.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="scene", choices=["scene", "batched"])
+    ap.add_argument("--cpu-rays", type=int, default=16384)
+    ap.add_argument("--cpu-build-tris", type=int, default=1 << 20)
+    ap.add_argument("--train-rays", type=int, default=4 << 20)
+    ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--meshes", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
+              f"(the path has no multi-GPU form here). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "batched":
+        batched_workload(args)
+    else:
+        scene_workload(args)
 
 
 if __name__ == "__main__":

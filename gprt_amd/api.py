@@ -65,7 +65,10 @@ _SIGS = {
     "gprtBufferSort": (None, [_P, _P, _P]),
     "gprtBufferSortPayload": (None, [_P, _P, _P, _P]),
     "gprtBufferSortKV": (None, [_P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
-    "gprtSetSortConfig": (None, [_P, C.c_uint32, C.c_uint32]),
+    "gprtSetSortConfig": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gprtGetSortConfig": (None, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "gprtAccelBuildBatchAutotune": (C.c_uint32, [_P, C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_char_p,
+                                                 C.c_size_t]),
     "gprtGeomTypeCreate": (_P, [_P, C.c_int, C.c_size_t]),
     "gprtGeomTypeDestroy": (None, [_P]),
     "gprtGeomCreate": (_P, [_P, _P]),
@@ -94,6 +97,8 @@ _SIGS = {
     "gprtProposalGradients": (None, [_P, _P, _P, _P, C.c_uint32, _P, C.c_size_t]),
     "gprtProposalExportTreelets": (None, [_P, _P, _P, _P]),
     "gprtAccelTraceClosestHitEx": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P]),
+    "gprtSynthBlobMeshes": (None, [_P, _P, C.c_uint32, C.c_uint64, C.c_float, _P, _P]),
+    "gprtSynthPerturb": (None, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_float]),
     "gprtComputeSynchronize": (C.c_uint64, [_P]),
     "gprtDeviceSynchronize": (C.c_uint64, [_P]),
     "gprtBeginProfile": (None, [_P]),
@@ -212,18 +217,26 @@ def last_error() -> str | None:
 class TriangleAccel:
     """gprtGeomTypeCreate + gprtGeomCreate + gprtTriangles* + gprtTriangleAccelCreate."""
 
-    def __init__(self, ctx: Context, verts: np.ndarray, idx: np.ndarray):
+    def __init__(self, ctx: Context, verts: np.ndarray = None, idx: np.ndarray = None, *, buffers=None):
         self.ctx, self.lib = ctx, ctx.lib
-        verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
-        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
-        self.nv, self.nt = len(verts), len(idx)
-        self.vbuf = ctx.device_buffer(verts)
-        self.ibuf = ctx.device_buffer(idx)
+        if buffers is None:
+            verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+            idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+            self.nv, self.nt = len(verts), len(idx)
+            self.vbuf = ctx.device_buffer(verts)
+            self.ibuf = ctx.device_buffer(idx)
+        else:
+            self.vbuf, self.ibuf, self.nv, self.nt = buffers
         self.gtype = self.lib.gprtGeomTypeCreate(ctx.h, GPRT_TRIANGLES, 0)
         self.geom = self.lib.gprtGeomCreate(ctx.h, self.gtype)
         self.lib.gprtTrianglesSetVertices(self.geom, self.vbuf.h, self.nv, 12, 0)
         self.lib.gprtTrianglesSetIndices(self.geom, self.ibuf.h, self.nt, 12, 0)
         self.h = self.lib.gprtTriangleAccelCreate(ctx.h, self.geom, 0)
+
+    @classmethod
+    def from_buffers(cls, ctx: Context, vbuf: Buffer, ibuf: Buffer, nverts: int, ntris: int) -> "TriangleAccel":
+        """An accel over geometry already in device buffers (float3 vertices, uint3 indices)."""
+        return cls(ctx, buffers=(vbuf, ibuf, nverts, ntris))
 
     def build(self, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE):
         self.lib.gprtAccelBuild(self.ctx.h, self.h, GPRTBuildParams(mode))

@@ -2,6 +2,9 @@
 // gprtTrianglesSetVertices/Indices, gprtTriangleAccelCreate, gprtAccelBuild/Update/Destroy
 // (gprt/gprt_host.h:406-498, :1308, :1421-1557; gprt/gprt.cpp:3149-3575, :9152-9262) and the
 // ray-batch trace and LBVH export extensions.
+#include <algorithm>
+#include <cstring>
+#include <string>
 #include <unordered_set>
 #include <vector>
 
@@ -255,4 +258,80 @@ GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _no
   if (aabbs->size < total * 24 && !buffer_realloc(aabbs, total * 24, false)) return;
   if (keys->size < n * 8 && !buffer_realloc(keys, n * 8, false)) return;
   lbvh_export(ctx, ctx->stream, a, (int32_t *)nodes->d, (float *)aabbs->d, (uint64_t *)keys->d);
+}
+
+// Autotuner for batched builds: times the graph-captured batch build of `accels` for every
+// digit width {4, 6, 8} x tile {2048, 4096, 8192} keys x sort-pass occupancy {none, 8, 16} waves
+// per CU (median of `reps` graph launches, HIP events on the context's stream), keeps the
+// fastest configuration in the context and leaves the accels built with it. Results do not depend
+// on the configuration. Writes a JSON report into `report` (up to `cap` bytes); returns the index
+// of the best configuration.
+GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_accels, uint32_t count, uint32_t reps,
+                                              char *report, size_t cap) {
+  Context *ctx = (Context *)_ctx;
+  if (!ctx || !_accels || count == 0) {
+    fail("gprtAccelBuildBatchAutotune: null argument or empty batch");
+    return 0;
+  }
+  if (reps == 0) reps = 3;
+  std::vector<Accel *> accels(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    accels[i] = (Accel *)_accels[i];
+    if (!accels[i] || !validate_geom(accels[i]->geom, "gprtAccelBuildBatchAutotune")) return 0;
+  }
+  const int bits[3] = {4, 6, 8}, tiles[3] = {2048, 4096, 8192}, waves[3] = {0, 8, 16};
+  const bool timer = ctx->timer.enabled;
+  ctx->timer.enabled = false;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  std::string js = "{\"configs\": [";
+  float bestMs = 1e30f;
+  uint32_t best = 0, idx = 0;
+  SortConfig bestCfg = ctx->sortCfg;
+  for (int b : bits)
+    for (int t : tiles)
+      for (int w : waves) {
+        SortConfig cfg = ctx->sortCfg;
+        cfg.digitBits = b;
+        cfg.tileKeys = t;
+        cfg.wavesPerCU = w;
+        ctx->sortCfg = cfg;
+        if (!lbvh_build_batch(ctx, ctx->stream, accels.data(), count, true)) break;  // capture + warm-up
+        std::vector<float> ms(reps);
+        for (uint32_t r = 0; r < reps; ++r) {
+          hipEventRecord(e0, ctx->stream);
+          lbvh_build_batch(ctx, ctx->stream, accels.data(), count, true);
+          hipEventRecord(e1, ctx->stream);
+          hipEventSynchronize(e1);
+          hipEventElapsedTime(&ms[r], e0, e1);
+        }
+        std::sort(ms.begin(), ms.end());
+        const float med = ms[reps / 2];
+        char line[160];
+        snprintf(line, sizeof(line), "%s{\"digitBits\": %d, \"tileKeys\": %d, \"wavesPerCU\": %d, \"ms\": %.4f}",
+                 idx ? ", " : "", b, t, w, med);
+        js += line;
+        if (med < bestMs) {
+          bestMs = med;
+          best = idx;
+          bestCfg = cfg;
+        }
+        ++idx;
+      }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  ctx->sortCfg = bestCfg;
+  lbvh_build_batch(ctx, ctx->stream, accels.data(), count, true);
+  ctx->timer.enabled = timer;
+  char tail[200];
+  snprintf(tail, sizeof(tail), "], \"best\": {\"digitBits\": %d, \"tileKeys\": %d, \"wavesPerCU\": %d, \"ms\": %.4f}}",
+           bestCfg.digitBits, bestCfg.tileKeys, bestCfg.wavesPerCU, bestMs);
+  js += tail;
+  if (report && cap) {
+    const size_t n = std::min(js.size(), cap - 1);
+    std::memcpy(report, js.data(), n);
+    report[n] = 0;
+  }
+  return best;
 }

@@ -43,6 +43,7 @@ struct SortConfig {
   int digitBits = 8;     // 4, 6 or 8
   int tileKeys = 4096;   // keys per tile: 256 threads x {8, 16} (x2 for 512 threads)
   int threads = 256;     // 256 or 512
+  int wavesPerCU = 0;    // LBVH sort passes: occupancy cap through LDS padding (0 = none)
 };
 
 struct Context {

@@ -700,15 +700,31 @@ void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a,
          dim3(kThreads), 0, a);
 }
 
+// Extra dynamic LDS that caps a sort pass at `wavesPerCU` resident waves per CU (160 KB of LDS
+// per CU shared by the workgroups); 0 when no cap is asked or the cap is above the natural one.
+template <int BITS, int THREADS, int ITEMS>
+size_t sort_lds_pad(int wavesPerCU) {
+  if (wavesPerCU <= 0) return 0;
+  constexpr int R = 1 << BITS, W = THREADS / 64;
+  const size_t natural = (size_t)(THREADS * ITEMS + W * R + 2 * R + W + 1) * 4;
+  const int wgs = std::max(1, wavesPerCU / W);
+  const size_t target = std::min<size_t>((160u * 1024u) / wgs, 64u * 1024u);
+  return target > natural ? target - natural : 0;
+}
+
 template <int BITS>
 void launch_sort_passes(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a) {
+  const int wpc = st->cfg.wavesPerCU;
   for (int p = 0; p < st->passes; ++p) {
     if (st->cfg.tileKeys == 2048)
-      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 8>, dim3(st->tiles), dim3(256), 0, a, p);
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 8>, dim3(st->tiles), dim3(256),
+             sort_lds_pad<BITS, 256, 8>(wpc), a, p);
     else if (st->cfg.tileKeys == 4096)
-      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 16>, dim3(st->tiles), dim3(256), 0, a, p);
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 16>, dim3(st->tiles), dim3(256),
+             sort_lds_pad<BITS, 256, 16>(wpc), a, p);
     else
-      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 512, 16>, dim3(st->tiles), dim3(512), 0, a, p);
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 512, 16>, dim3(st->tiles), dim3(512),
+             sort_lds_pad<BITS, 512, 16>(wpc), a, p);
   }
 }
 
@@ -745,7 +761,8 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
 }
 
 bool same_plan(const BVHStore *st, const std::vector<MeshDesc> &desc, const SortConfig &cfg) {
-  if (st->cfg.digitBits != cfg.digitBits || st->cfg.tileKeys != cfg.tileKeys) return false;
+  if (st->cfg.digitBits != cfg.digitBits || st->cfg.tileKeys != cfg.tileKeys || st->cfg.wavesPerCU != cfg.wavesPerCU)
+    return false;
   if (st->desc.size() != desc.size()) return false;
   return std::memcmp(st->desc.data(), desc.data(), desc.size() * sizeof(MeshDesc)) == 0;
 }

@@ -502,7 +502,7 @@ GPRT_API void gprtBufferSortKV(GPRTContext _ctx, GPRTBuffer _keys, GPRTBuffer _v
               "gprtBufferSortKV");
 }
 
-GPRT_API void gprtSetSortConfig(GPRTContext _ctx, uint32_t digitBits, uint32_t tileKeys) {
+GPRT_API void gprtSetSortConfig(GPRTContext _ctx, uint32_t digitBits, uint32_t tileKeys, uint32_t wavesPerCU) {
   Context *ctx = (Context *)_ctx;
   if (!ctx) {
     fail("gprtSetSortConfig: null context");
@@ -516,8 +516,24 @@ GPRT_API void gprtSetSortConfig(GPRTContext _ctx, uint32_t digitBits, uint32_t t
     fail("gprtSetSortConfig: tileKeys %u is not one of 2048, 4096, 8192", tileKeys);
     return;
   }
+  if (wavesPerCU > 64) {
+    fail("gprtSetSortConfig: wavesPerCU %u exceeds 64", wavesPerCU);
+    return;
+  }
   ctx->sortCfg.digitBits = (int)digitBits;
   ctx->sortCfg.tileKeys = (int)tileKeys;
+  ctx->sortCfg.wavesPerCU = (int)wavesPerCU;
+}
+
+GPRT_API void gprtGetSortConfig(GPRTContext _ctx, uint32_t *digitBits, uint32_t *tileKeys, uint32_t *wavesPerCU) {
+  Context *ctx = (Context *)_ctx;
+  if (!ctx) {
+    fail("gprtGetSortConfig: null context");
+    return;
+  }
+  if (digitBits) *digitBits = (uint32_t)ctx->sortCfg.digitBits;
+  if (tileKeys) *tileKeys = (uint32_t)ctx->sortCfg.tileKeys;
+  if (wavesPerCU) *wavesPerCU = (uint32_t)ctx->sortCfg.wavesPerCU;
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -161,3 +161,93 @@ def jittered_pinhole_rays(n: int, seed: int, width: int = 2048, height: int = 20
     r["dx"], r["dy"], r["dz"] = d[:, 0], d[:, 1], d[:, 2]
     r["tmax"] = TMAX
     return r
+
+
+def _rand01_keyed(keys: np.ndarray, ctr: np.ndarray) -> np.ndarray:
+    """rand01 with a per-element stream key (as rand_u64's `key`) and 0-based counter."""
+    with np.errstate(over="ignore"):
+        return (_mix(keys + (ctr.astype(np.uint64) + np.uint64(1)) * _GOLD) >> np.uint64(40)).astype(np.float64) \
+            * (2.0 ** -24)
+
+
+def _stream_key(seeds: np.ndarray, stream: int) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        k = seeds.astype(np.uint64) * _GOLD + np.uint64((stream * 0xD1B54A32D192ED03 + 1) & (2**64 - 1))
+        return _mix(k)
+
+
+def batch_sizes(count: int = 4096, seed: int = 3, lo: int = 64, hi: int = 262144) -> np.ndarray:
+    """Triangle counts log-uniform in [lo, hi]: round(lo * (hi/lo)^u)."""
+    u = rand01(seed, 50, count)
+    return np.rint(lo * (hi / lo) ** u).astype(np.int64)
+
+
+def _blob_params(seeds: np.ndarray, blobs: int):
+    M = len(seeds)
+    centers = _rand01_keyed(np.repeat(_stream_key(seeds, 10), 3 * blobs), np.tile(np.arange(3 * blobs), M))
+    sig = np.exp(math.log(0.02) + _rand01_keyed(np.repeat(_stream_key(seeds, 11), blobs),
+                                                np.tile(np.arange(blobs), M)) * (math.log(0.1) - math.log(0.02)))
+    return centers.reshape(M, blobs, 3), sig.reshape(M, blobs)
+
+
+def _blob_range(sizes, offs, seeds, centers, sig, blobs, offset, t0, t1, out):
+    """Vertices of global triangles [t0, t1) into out[3*t0 : 3*t1]."""
+    m = np.searchsorted(offs, np.arange(t0, t1), side="right") - 1
+    loc = np.arange(t0, t1) - offs[m]
+    n = sizes[m]
+    b = (loc * blobs) // n
+    c3 = (3 * loc)[:, None] + np.arange(3)
+    u1 = 1.0 - _rand01_keyed(_stream_key(seeds, 12)[m][:, None], c3)
+    u2 = _rand01_keyed(_stream_key(seeds, 13)[m][:, None], c3)
+    g = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+    cent = centers[m, b] + sig[m, b][:, None] * g
+    c9 = (9 * loc)[:, None] + np.arange(9)
+    off = _rand01_keyed(_stream_key(seeds, 14)[m][:, None], c9) * (2.0 * offset) - offset
+    out[3 * t0:3 * t1] = (cent[:, None, :] + off.reshape(-1, 3, 3)).astype(np.float32).reshape(-1, 3)
+
+
+_SHARED = {}
+
+
+def _blob_worker(args):
+    t0, t1 = args
+    g = _SHARED
+    _blob_range(g["sizes"], g["offs"], g["seeds"], g["centers"], g["sig"], g["blobs"], g["offset"], t0, t1, g["out"])
+    return t1 - t0
+
+
+def blob_meshes(sizes, seed0: int = 1000, blobs: int = 64, offset: float = 0.002, workers: int = 1):
+    """Concatenated blob meshes: mesh i equals blob_scene(sizes[i], seed0 + i) exactly.
+    Returns (verts float32 [3T, 3], idx uint32 [T, 3] with per-mesh local indices, prim offsets).
+    `workers` > 1 splits the triangle range over forked processes writing into shared memory."""
+    from multiprocessing import get_context, shared_memory
+
+    sizes = np.asarray(sizes, dtype=np.int64)
+    M = len(sizes)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    T = int(offs[-1])
+    seeds = seed0 + np.arange(M, dtype=np.int64)
+    centers, sig = _blob_params(seeds, blobs)
+    chunk = 1 << 20
+    ranges = [(t0, min(T, t0 + chunk)) for t0 in range(0, T, chunk)]
+    if workers <= 1 or len(ranges) == 1:
+        verts = np.empty((3 * T, 3), dtype=np.float32)
+        for t0, t1 in ranges:
+            _blob_range(sizes, offs, seeds, centers, sig, blobs, offset, t0, t1, verts)
+    else:
+        shm = shared_memory.SharedMemory(create=True, size=max(1, 3 * T * 3 * 4))
+        try:
+            out = np.ndarray((3 * T, 3), dtype=np.float32, buffer=shm.buf)
+            _SHARED.update(sizes=sizes, offs=offs, seeds=seeds, centers=centers, sig=sig, blobs=blobs,
+                           offset=offset, out=out)
+            with get_context("fork").Pool(workers) as pool:
+                for _ in pool.imap_unordered(_blob_worker, ranges):
+                    pass
+            verts = out.copy()
+            del out
+        finally:
+            _SHARED.clear()
+            shm.close()
+            shm.unlink()
+    idx = (np.arange(3 * T, dtype=np.int64) - 3 * np.repeat(offs[:-1], 3 * sizes)).astype(np.uint32).reshape(T, 3)
+    return verts, idx, offs

@@ -182,9 +182,11 @@ GPRT_API void gprtBufferSortKV(GPRTContext context, GPRTBuffer keys, GPRTBuffer 
                                uint32_t keyBits, uint32_t valueBits, GPRTBuffer scratch);
 
 /* Radix sort knobs used by every sort and accel build on this context (the autotuner's search
- * space): digitBits in {4, 6, 8}, tileKeys (keys per workgroup tile) in {2048, 4096, 8192}.
- * Results do not depend on them. */
-GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_t tileKeys);
+ * space): digitBits in {4, 6, 8}, tileKeys (keys per workgroup tile) in {2048, 4096, 8192},
+ * wavesPerCU (0 = no cap) caps the resident waves of the accel builds' sort passes through LDS
+ * padding. Results do not depend on them. */
+GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_t tileKeys, uint32_t wavesPerCU);
+GPRT_API void gprtGetSortConfig(GPRTContext context, uint32_t *digitBits, uint32_t *tileKeys, uint32_t *wavesPerCU);
 
 /* Batched build: every accel in `accels` (each over its own triangle geometry) is built by one
  * launch sequence over a shared tile table — bounds, keys, segmented radix sort, Karras and refit
@@ -195,6 +197,13 @@ GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_
  * device synchronisation per build (gprt/gprt.cpp:3216, :3271, :3285). */
 GPRT_API void gprtAccelBuildBatch(GPRTContext context, GPRTAccel *accels, uint32_t count,
                                   GPRTBuildParams buildParams GPRT_IF_CPP(= GPRTBuildParams()));
+
+/* Project-written autotuner for batched builds: times the graph-captured batch build for every
+ * digitBits {4,6,8} x tileKeys {2048,4096,8192} x wavesPerCU {0,8,16} (median of `reps` launches),
+ * keeps the fastest configuration in the context (gprtGetSortConfig reads it back), leaves the
+ * accels built with it and writes a JSON report. Returns the index of the best configuration. */
+GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext context, GPRTAccel *accels, uint32_t count, uint32_t reps,
+                                              char *report, size_t cap);
 
 /* Ray batch closest hit against a built triangle accel. rays: `count` records of
  * {float ox, oy, oz, tmin; float dx, dy, dz, tmax} (32 B); hits: `count` records of
@@ -270,6 +279,16 @@ GPRT_API void gprtKernelTimerReset(GPRTContext context);
 GPRT_API double gprtKernelTimerQuery(GPRTContext context, const char *name, uint32_t *launches);
 /* Writes up to `cap` bytes of "name,launches,total_ms\n" lines; returns the bytes needed. */
 GPRT_API size_t gprtKernelTimerReport(GPRTContext context, char *out, size_t cap);
+
+/* Synthetic workloads on the device (benchmarks, tests): `meshes` blob meshes, mesh i with
+ * sizes[i] triangles from seed seed0 + i, written concatenated (float3 vertices, uint3 indices
+ * local to each mesh) into verts / idx (resized as needed); and dst = src + U(-amplitude,
+ * amplitude) per float (the dynamic scene's per-frame perturbation). Same streams as
+ * gprt_amd/scenes.py. */
+GPRT_API void gprtSynthBlobMeshes(GPRTContext context, const int64_t *sizes, uint32_t meshes, uint64_t seed0,
+                                  float offset, GPRTBuffer verts, GPRTBuffer idx);
+GPRT_API void gprtSynthPerturb(GPRTContext context, GPRTBuffer src, GPRTBuffer dst, uint64_t floats, uint64_t seed,
+                               float amplitude);
 
 /* Library version string, for logs. */
 GPRT_API const char *gprtVersion(void);

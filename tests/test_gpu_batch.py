@@ -29,7 +29,7 @@ def _check(acc, orc, v, idx):
 
 @pytest.mark.parametrize("bits,tile", [(8, 4096), (6, 2048), (4, 8192), (8, 8192)])
 def test_batch_equals_oracle_per_mesh(ctx, orc, bits, tile):
-    ctx.lib.gprtSetSortConfig(ctx.h, bits, tile)
+    ctx.lib.gprtSetSortConfig(ctx.h, bits, tile, 0)
     try:
         meshes = _meshes(1000)
         accs = [api.TriangleAccel(ctx, v, idx) for v, idx in meshes]
@@ -40,7 +40,7 @@ def test_batch_equals_oracle_per_mesh(ctx, orc, bits, tile):
         for a in accs:
             a.destroy()
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 4096)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
 
 
 def test_batch_graph_replays_with_moving_vertices(ctx, orc):
