@@ -99,6 +99,9 @@ _SIGS = {
     "gprtAccelTraceClosestHitEx": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P]),
     "gprtSynthBlobMeshes": (None, [_P, _P, C.c_uint32, C.c_uint64, C.c_float, _P, _P]),
     "gprtSynthPerturb": (None, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_float]),
+    "gprtAccelTraceClosestHitMulti": (None, [_P, C.POINTER(C.c_void_p), C.c_uint32, _P, _P, C.c_uint32, _P]),
+    "gprtSynthBounceRays": (None, [_P, C.POINTER(C.c_void_p), C.c_uint32, _P, _P, _P, C.c_uint32, _P, C.c_uint32,
+                                   C.c_uint64, C.c_float]),
     "gprtComputeSynchronize": (C.c_uint64, [_P]),
     "gprtDeviceSynchronize": (C.c_uint64, [_P]),
     "gprtBeginProfile": (None, [_P]),
@@ -275,6 +278,23 @@ def build_batch(ctx: Context, accels, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE)
     """gprtAccelBuildBatch over a list of TriangleAccel objects."""
     arr = (C.c_void_p * len(accels))(*[a.h for a in accels])
     ctx.lib.gprtAccelBuildBatch(ctx.h, arr, len(accels), GPRTBuildParams(mode))
+
+
+def _accel_array(accels):
+    return (C.c_void_p * len(accels))(*[a.h for a in accels])
+
+
+def trace_multi(ctx: Context, accels, rays_buf: Buffer, hits_buf: Buffer, count: int, ids_buf: Buffer | None = None):
+    """gprtAccelTraceClosestHitMulti: closest hit over several accels, order (t, instance, primID)."""
+    ctx.lib.gprtAccelTraceClosestHitMulti(ctx.h, _accel_array(accels), len(accels), rays_buf.h, hits_buf.h, count,
+                                          ids_buf.h if ids_buf is not None else None)
+
+
+def bounce_rays(ctx: Context, accels, rays_buf: Buffer, hits_buf: Buffer, ids_buf: Buffer | None, count: int,
+                out_buf: Buffer, out_count: int, seed: int, tmin: float = 1e-4):
+    """gprtSynthBounceRays: cosine-weighted bounce rays from the hits of `count` traced rays."""
+    ctx.lib.gprtSynthBounceRays(ctx.h, _accel_array(accels), len(accels), rays_buf.h, hits_buf.h,
+                                ids_buf.h if ids_buf is not None else None, count, out_buf.h, out_count, seed, tmin)
 
 
 class Proposal:

@@ -130,6 +130,19 @@ def bounce_rays(prim_rays: np.ndarray, hits: np.ndarray, verts: np.ndarray, idx:
     return out
 
 
+def bounce_rays_multi(prim_rays: np.ndarray, hits: np.ndarray, inst: np.ndarray, meshes, count: int,
+                      seed: int = 3, tmin: float = 1e-4) -> np.ndarray:
+    """bounce_rays over several meshes: hit i lies on triangle hits[i].primID of meshes[inst[i]]."""
+    offs_v = np.cumsum([0] + [len(v) for v, _ in meshes])
+    offs_t = np.cumsum([0] + [len(i) for _, i in meshes])
+    verts = np.concatenate([v for v, _ in meshes])
+    idx = np.concatenate([i.astype(np.int64) + offs_v[k] for k, (_, i) in enumerate(meshes)])
+    h = hits.copy()
+    hit = h["primID"] >= 0
+    h["primID"][hit] = h["primID"][hit] + offs_t[inst[hit]]
+    return bounce_rays(prim_rays, h, verts, idx, count, seed, tmin)
+
+
 def random_rays(n: int, seed: int, lo=-0.5, hi=1.5) -> np.ndarray:
     """Rays with origins uniform in [lo, hi]^3 and directions uniform on the sphere."""
     o = rand01(seed, 30, 3 * n).reshape(n, 3) * (hi - lo) + lo

@@ -290,6 +290,20 @@ GPRT_API void gprtSynthBlobMeshes(GPRTContext context, const int64_t *sizes, uin
 GPRT_API void gprtSynthPerturb(GPRTContext context, GPRTBuffer src, GPRTBuffer dst, uint64_t floats, uint64_t seed,
                                float amplitude);
 
+/* Scenes of several triangle accels (identity transforms; no reference counterpart at buffer
+ * level: the reference traces instances through gprtInstanceAccelCreate, gprt/gprt_host.h:1418,
+ * and the driver's TLAS). gprtAccelTraceClosestHitMulti returns, per ray, the closest hit over
+ * all `count` accels under the order (t, instance index, primID), and the instance index
+ * (int32, -1 on a miss) in instanceIDs (may be NULL). gprtSynthBounceRays writes `outCount`
+ * cosine-weighted bounce rays (tmin given, tmax 1e30) from the hit points of the first `count`
+ * traced rays: misses are skipped, hits are cycled in ray order, seeded as
+ * gprt_amd/scenes.bounce_rays; with no hit at all every output ray is empty (tmax < tmin). */
+GPRT_API void gprtAccelTraceClosestHitMulti(GPRTContext context, GPRTAccel *accels, uint32_t count, GPRTBuffer rays,
+                                            GPRTBuffer hits, uint32_t rayCount, GPRTBuffer instanceIDs);
+GPRT_API void gprtSynthBounceRays(GPRTContext context, GPRTAccel *accels, uint32_t count, GPRTBuffer rays,
+                                  GPRTBuffer hits, GPRTBuffer instanceIDs, uint32_t rayCount, GPRTBuffer out,
+                                  uint32_t outCount, uint64_t seed, float tmin);
+
 /* Library version string, for logs. */
 GPRT_API const char *gprtVersion(void);
 
