@@ -38,6 +38,32 @@ class GPRTBuildParams(C.Structure):
         super().__init__(buildMode, False, False, False, False, False)
 
 
+class GPRTInstance(C.Structure):
+    """gprt::Instance (gprt/gprt_shared.h:92-101): 3x4 row-major transform [A | b], then
+    instanceCustomIndex:24 | mask:8, geometryIndex:24 | flags:8, accelAddress (64 bytes)."""
+    _fields_ = [("transform", (C.c_float * 4) * 3), ("instanceCustomIndex", C.c_uint32, 24), ("mask", C.c_uint32, 8),
+                ("geometryIndex", C.c_uint32, 24), ("flags", C.c_uint32, 8), ("accelAddress", C.c_uint64)]
+
+
+# The same record as a numpy dtype (little-endian bitfields: low 24 bits, then the top 8).
+INSTANCE_DTYPE = np.dtype([("transform", "<f4", (3, 4)), ("word0", "<u4"), ("word1", "<u4"),
+                           ("accelAddress", "<u8")])
+
+
+def instance_records(accels, transforms=None, masks=None) -> np.ndarray:
+    """gprt::Instance records for `accels` (gprtAccelGetInstance), optionally with 3x4 transforms
+    and 8-bit masks; instanceCustomIndex = position in the list."""
+    lib = load()
+    rec = np.zeros(len(accels), dtype=INSTANCE_DTYPE)
+    for i, a in enumerate(accels):
+        inst = lib.gprtAccelGetInstance(a.h)
+        rec["accelAddress"][i] = inst.accelAddress
+        rec["transform"][i] = np.eye(3, 4, dtype=np.float32) if transforms is None else transforms[i]
+        m = 0xFF if masks is None else int(masks[i]) & 0xFF
+        rec["word0"][i] = (i & 0xFFFFFF) | (m << 24)
+    return rec
+
+
 _P = C.c_void_p
 _SIGS = {
     # name: (restype, argtypes)
@@ -99,9 +125,12 @@ _SIGS = {
     "gprtAccelTraceClosestHitEx": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P]),
     "gprtSynthBlobMeshes": (None, [_P, _P, C.c_uint32, C.c_uint64, C.c_float, _P, _P]),
     "gprtSynthPerturb": (None, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_float]),
-    "gprtAccelTraceClosestHitMulti": (None, [_P, C.POINTER(C.c_void_p), C.c_uint32, _P, _P, C.c_uint32, _P]),
-    "gprtSynthBounceRays": (None, [_P, C.POINTER(C.c_void_p), C.c_uint32, _P, _P, _P, C.c_uint32, _P, C.c_uint32,
-                                   C.c_uint64, C.c_float]),
+    "gprtSynthBounceRays": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint64, C.c_float]),
+    "gprtInstanceAccelCreate": (_P, [_P, C.c_uint32, _P]),
+    "gprtAccelGetInstance": (GPRTInstance, [_P]),
+    "gprtAccelTraceClosestHitDetail": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P]),
+    "gprtProposalTrainLabels": (C.c_float, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float,
+                                            C.POINTER(C.c_float)]),
     "gprtComputeSynchronize": (C.c_uint64, [_P]),
     "gprtDeviceSynchronize": (C.c_uint64, [_P]),
     "gprtBeginProfile": (None, [_P]),
@@ -280,21 +309,50 @@ def build_batch(ctx: Context, accels, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE)
     ctx.lib.gprtAccelBuildBatch(ctx.h, arr, len(accels), GPRTBuildParams(mode))
 
 
-def _accel_array(accels):
-    return (C.c_void_p * len(accels))(*[a.h for a in accels])
+def _h(b):
+    return b.h if b is not None else None
 
 
-def trace_multi(ctx: Context, accels, rays_buf: Buffer, hits_buf: Buffer, count: int, ids_buf: Buffer | None = None):
-    """gprtAccelTraceClosestHitMulti: closest hit over several accels, order (t, instance, primID)."""
-    ctx.lib.gprtAccelTraceClosestHitMulti(ctx.h, _accel_array(accels), len(accels), rays_buf.h, hits_buf.h, count,
-                                          ids_buf.h if ids_buf is not None else None)
+class InstanceAccel:
+    """gprtInstanceAccelCreate over a device buffer of gprt::Instance records."""
+
+    def __init__(self, ctx: Context, records: np.ndarray):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.records = np.ascontiguousarray(records, dtype=INSTANCE_DTYPE)
+        self.rbuf = ctx.device_buffer(self.records)
+        self.h = self.lib.gprtInstanceAccelCreate(ctx.h, len(self.records), self.rbuf.h)
+        if not self.h:
+            raise RuntimeError("gprtInstanceAccelCreate failed: %s" % last_error())
+
+    def set_records(self, records: np.ndarray):
+        self.records = np.ascontiguousarray(records, dtype=INSTANCE_DTYPE)
+        self.rbuf.write(self.records)
+
+    def build(self, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE):
+        self.lib.gprtAccelBuild(self.ctx.h, self.h, GPRTBuildParams(mode))
+
+    def trace(self, rays_buf: Buffer, hits_buf: Buffer, count: int):
+        self.lib.gprtAccelTraceClosestHit(self.ctx.h, self.h, rays_buf.h, hits_buf.h, count)
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtAccelDestroy(self.h)
+            self.rbuf.destroy()
+            self.h = None
 
 
-def bounce_rays(ctx: Context, accels, rays_buf: Buffer, hits_buf: Buffer, ids_buf: Buffer | None, count: int,
+def trace_detail(ctx: Context, accel, rays_buf: Buffer, hits_buf: Buffer, count: int, inst_buf: Buffer = None,
+                 leaf_buf: Buffer = None, visits_buf: Buffer = None, proposal=None):
+    """gprtAccelTraceClosestHitDetail (any accel; optional instance / top-leaf / visit outputs)."""
+    ctx.lib.gprtAccelTraceClosestHitDetail(ctx.h, accel.h, _h(proposal), rays_buf.h, hits_buf.h, count,
+                                           _h(inst_buf), _h(leaf_buf), _h(visits_buf))
+
+
+def bounce_rays(ctx: Context, accel, rays_buf: Buffer, hits_buf: Buffer, inst_buf: Buffer | None, count: int,
                 out_buf: Buffer, out_count: int, seed: int, tmin: float = 1e-4):
     """gprtSynthBounceRays: cosine-weighted bounce rays from the hits of `count` traced rays."""
-    ctx.lib.gprtSynthBounceRays(ctx.h, _accel_array(accels), len(accels), rays_buf.h, hits_buf.h,
-                                ids_buf.h if ids_buf is not None else None, count, out_buf.h, out_count, seed, tmin)
+    ctx.lib.gprtSynthBounceRays(ctx.h, accel.h, rays_buf.h, hits_buf.h, _h(inst_buf), count, out_buf.h, out_count,
+                                seed, tmin)
 
 
 class Proposal:
@@ -323,6 +381,12 @@ class Proposal:
         acc = C.c_float(0.0)
         loss = self.lib.gprtProposalTrain(self.ctx.h, self.h, rays_buf.h, hits_buf.h, count, epochs, lr,
                                           C.byref(acc))
+        return float(loss), float(acc.value)
+
+    def train_labels(self, rays_buf: Buffer, labels_buf: Buffer, count: int, epochs: int = 1, lr: float = 1e-3):
+        acc = C.c_float(0.0)
+        loss = self.lib.gprtProposalTrainLabels(self.ctx.h, self.h, rays_buf.h, labels_buf.h, count, epochs, lr,
+                                                C.byref(acc))
         return float(loss), float(acc.value)
 
     def destroy(self):

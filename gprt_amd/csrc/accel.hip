@@ -103,6 +103,10 @@ GPRT_API GPRTAccel gprtTriangleAccelCreate(GPRTContext _ctx, GPRTGeom _g, unsign
   a->ctx = ctx;
   a->geom = g;
   a->flags = flags;
+  if (!blas_slot_acquire(ctx, a)) {
+    delete a;
+    return nullptr;
+  }
   return (GPRTAccel)a;
 }
 
@@ -110,7 +114,12 @@ GPRT_API void gprtAccelDestroy(GPRTAccel _a) {
   Accel *a = (Accel *)_a;
   if (!a) return;
   hipStreamSynchronize(a->ctx->stream);
-  accel_detach(a);
+  if (a->kind == AccelKind::Instances) {
+    tlas_release(a);
+  } else {
+    accel_detach(a);
+    blas_slot_release(a);
+  }
   delete a;
 }
 
@@ -140,8 +149,13 @@ GPRT_API void gprtAccelBuild(GPRTContext _ctx, GPRTAccel _a, GPRTBuildParams par
     fail("gprtAccelBuild: null argument");
     return;
   }
-  if (!validate_geom(a->geom, "gprtAccelBuild")) return;
   if (params.buildMode == GPRT_BUILD_MODE_UNINITIALIZED) params.buildMode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
+  if (a->kind == AccelKind::Instances) {
+    a->mode = params.buildMode;
+    a->built = tlas_build(ctx, ctx->stream, a);
+    return;
+  }
+  if (!validate_geom(a->geom, "gprtAccelBuild")) return;
   a->mode = params.buildMode;
   // A single build detaches the accel from any batch store it shared.
   if (a->store && a->store->refs > 1) accel_detach(a);
@@ -167,6 +181,10 @@ GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t
       fail("gprtAccelBuildBatch: accel %u appears twice", i);
       return;
     }
+    if (accels[i]->kind != AccelKind::Triangles) {
+      fail("gprtAccelBuildBatch: accel %u is not a triangle accel", i);
+      return;
+    }
     if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatch")) return;
   }
   if (params.buildMode == GPRT_BUILD_MODE_UNINITIALIZED) params.buildMode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
@@ -190,6 +208,11 @@ GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
     fail("Attempting to update an accel that was not built with an *_AND_UPDATE build mode");
     return;
   }
+  if (a->kind == AccelKind::Instances) {
+    // The top level is rebuilt from the records and the current sub-root boxes.
+    a->built = tlas_build(ctx, ctx->stream, a);
+    return;
+  }
   if (!validate_geom(a->geom, "gprtAccelUpdate")) return;
   if (a->geom->icount != a->n) {
     fail("gprtAccelUpdate: triangle count changed (%u -> %u); rebuild instead", a->n, a->geom->icount);
@@ -200,14 +223,19 @@ GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
 
 GPRT_API size_t gprtAccelGetSize(GPRTAccel _a, int) {
   Accel *a = (Accel *)_a;
+  if (a && a->kind == AccelKind::Instances) return tlas_size(a);
   // Nodes + leaf triangle records of this accel (the traversal-resident part).
   return (a && a->built) ? (size_t)(a->n > 1 ? a->n - 1 : 1) * sizeof(BVHNode) + (size_t)a->n * sizeof(TriRecord)
                          : 0;
 }
 
+// A triangle accel's address is its descriptor slot (what gprt::Instance::accelAddress names); it
+// stays the same for the accel's lifetime. An instance accel's is its top-level node array.
 GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel _a, int) {
   Accel *a = (Accel *)_a;
-  return a ? a->nodes : nullptr;
+  if (!a) return nullptr;
+  if (a->kind == AccelKind::Instances) return a->nodes;
+  return a->ctx->blasPool ? (const void *)(a->ctx->blasPool + a->slot) : nullptr;
 }
 
 GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel _a) {
@@ -236,7 +264,10 @@ GPRT_API void gprtAccelTraceClosestHit(GPRTContext _ctx, GPRTAccel _a, GPRTBuffe
     fail("gprtAccelTraceClosestHit: %u hits exceed the hit buffer (%zu B)", count, hits->size);
     return;
   }
-  trace_closest(ctx, ctx->stream, a, rays->d, hits->d, count);
+  if (a->kind == AccelKind::Instances)
+    tlas_trace(ctx, ctx->stream, a, rays->d, hits->d, count, nullptr, 0, nullptr, nullptr, nullptr);
+  else
+    trace_closest(ctx, ctx->stream, a, rays->d, hits->d, count);
 }
 
 GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _nodes, GPRTBuffer _aabbs,
@@ -250,6 +281,10 @@ GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _no
   }
   if (!a->built) {
     fail("gprtAccelExportLBVH: accel has not been built");
+    return;
+  }
+  if (a->kind != AccelKind::Triangles) {
+    fail("gprtAccelExportLBVH: expects a triangle accel");
     return;
   }
   const size_t n = a->n;
@@ -277,7 +312,11 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
   std::vector<Accel *> accels(count);
   for (uint32_t i = 0; i < count; ++i) {
     accels[i] = (Accel *)_accels[i];
-    if (!accels[i] || !validate_geom(accels[i]->geom, "gprtAccelBuildBatchAutotune")) return 0;
+    if (!accels[i] || accels[i]->kind != AccelKind::Triangles) {
+      fail("gprtAccelBuildBatchAutotune: accel %u is null or not a triangle accel", i);
+      return 0;
+    }
+    if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatchAutotune")) return 0;
   }
   const int bits[3] = {4, 6, 8}, tiles[3] = {2048, 4096, 8192}, waves[3] = {0, 8, 16};
   const bool timer = ctx->timer.enabled;

@@ -63,6 +63,12 @@ struct Context {
   size_t traceBytes = 0;
   // Small pinned host word for scan totals.
   uint32_t *pinnedWord = nullptr;
+  // Pool of triangle-accel descriptors (device, fixed capacity so addresses stay valid): the
+  // address an instance record names (gprt::Instance::accelAddress) is a slot of this pool, which
+  // lets the instance-accel build check every address on the device before reading through it.
+  struct BlasDesc *blasPool = nullptr;
+  std::vector<uint32_t> blasFree;
+  uint32_t blasNext = 0;
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -144,8 +150,27 @@ struct BVHStore {
   bool graphValid = false;
 };
 
+// Device descriptor of a built triangle accel (64 B, one pool slot per accel, rewritten after
+// every build): what an instance accel needs to traverse it and to read its triangles.
+struct BlasDesc {
+  const void *nodes;  // BVHNode[n - 1]
+  const void *tris;   // TriRecord[n]
+  const float *header;
+  const char *verts;
+  const char *idx;
+  uint32_t n, vstride, istride, nverts;
+  uint32_t magic;  // kBlasMagic while the accel is built and alive
+  uint32_t slot;
+};
+constexpr uint32_t kBlasMagic = 0x53414C42u;
+constexpr uint32_t kBlasPoolSlots = 65536;
+
+enum class AccelKind { Triangles, Instances };
+struct Tlas;
+
 struct Accel {
   Context *ctx = nullptr;
+  AccelKind kind = AccelKind::Triangles;
   Geom *geom = nullptr;
   unsigned flags = 0;
   GPRTBuildMode mode = GPRT_BUILD_MODE_UNINITIALIZED;
@@ -157,6 +182,11 @@ struct Accel {
   void *nodes = nullptr;
   void *tris = nullptr;
   float *header = nullptr;
+  // Triangle accels: descriptor pool slot and the host copy last written there.
+  uint32_t slot = 0xFFFFFFFFu;
+  BlasDesc hdesc{};
+  // Instance accels.
+  Tlas *tlas = nullptr;
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -195,6 +225,22 @@ bool sort_run(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt
 
 // Fill a device range with a 32-bit pattern (graph-safe; no hipMemset nodes).
 void fill_u32(Context *ctx, hipStream_t s, void *dst, uint32_t value, size_t words);
+
+// Trace work counter (first 256 B) + per-lane stack spill of `spillWordsPerLane` words.
+uint32_t *ensure_trace_scratch(Context *ctx, hipStream_t s, int spillWordsPerLane);
+
+// Triangle-accel descriptors (instance.hip).
+bool blas_slot_acquire(Context *ctx, Accel *a);
+void blas_slot_release(Accel *a);
+bool blas_desc_refresh(Context *ctx, hipStream_t s, Accel *a);
+
+// Instance accels (instance.hip).
+bool tlas_build(Context *ctx, hipStream_t s, Accel *a);
+void tlas_release(Accel *a);
+size_t tlas_size(const Accel *a);
+uint32_t tlas_leaves(const Accel *a);
+bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
+                const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits);
 
 // LBVH (segmented: one store may hold many meshes)
 void store_release(BVHStore *st);

@@ -875,6 +875,7 @@ void accel_detach(Accel *a) {
   BVHStore *st = a->store;
   a->store = nullptr;
   a->built = false;
+  blas_desc_refresh(a->ctx, a->ctx->stream, a);  // instance accels stop reading this storage
   if (--st->refs == 0) {
     hipStreamSynchronize(a->ctx->stream);
     store_release(st);
@@ -941,7 +942,10 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
                 "hipMemcpyAsync(tile table)"))
       return false;
   }
-  for (uint32_t m = 0; m < count; ++m) set_views(accels[m]);
+  for (uint32_t m = 0; m < count; ++m) {
+    set_views(accels[m]);
+    if (!blas_desc_refresh(ctx, s, accels[m])) return false;
+  }
 
   if (useGraph) {
     if (!st->graphValid) {

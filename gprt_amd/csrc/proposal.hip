@@ -13,20 +13,32 @@ using namespace gprt;
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_label_flags(const int32_t *__restrict__ labels, uint32_t n,
+// A label counts when it names one of the `limit` classes.
+__global__ void __launch_bounds__(256) k_label_flags(const int32_t *__restrict__ labels, uint32_t n, uint32_t limit,
                                                      uint32_t *__restrict__ flags) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flags[i] = labels[i] >= 0 ? 1u : 0u;
+  if (i < n) flags[i] = labels[i] >= 0 && (uint32_t)labels[i] < limit ? 1u : 0u;
 }
 
 __global__ void __launch_bounds__(256) k_compact(const GPRTRay *__restrict__ rays, const int32_t *__restrict__ labels,
+                                                 const uint32_t *__restrict__ flags,
                                                  const uint32_t *__restrict__ offs, uint32_t n,
                                                  GPRTRay *__restrict__ outRays, int32_t *__restrict__ outLabels) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || labels[i] < 0) return;
+  if (i >= n || !flags[i]) return;
   const uint32_t o = offs[i];
   outRays[o] = rays[i];
   outLabels[o] = labels[i];
+}
+
+__global__ void __launch_bounds__(256) k_inst_of_hits(const GPRTHit *__restrict__ hits, uint32_t n,
+                                                      int32_t *__restrict__ inst) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inst[i] = hits[i].primID >= 0 ? 0 : -1;
+}
+
+void instance_ids_of_hits(Context *ctx, hipStream_t s, const GPRTHit *hits, uint32_t n, int32_t *inst) {
+  launch(ctx, s, "trace_instance_ids", k_inst_of_hits, dim3((n + 255) / 256), dim3(256), 0, hits, n, inst);
 }
 
 uint64_t mix64(uint64_t z) {
@@ -117,6 +129,19 @@ GPRT_API uint32_t gprtProposalBindAccel(GPRTContext _ctx, GPRTProposal _p, GPRTA
     fail("gprtProposalBindAccel: accel has not been built");
     return 0;
   }
+  if (a->kind == AccelKind::Instances) {
+    const uint32_t leaves = tlas_leaves(a);
+    if (leaves > (uint32_t)kMlpOut) {
+      fail("gprtProposalBindAccel: the instance accel has %u top-level leaves; the proposal has %d classes", leaves,
+           kMlpOut);
+      return 0;
+    }
+    p->accel = a;
+    p->numClasses = leaves;
+    p->treePrims = leaves;
+    p->primClass = nullptr;
+    return leaves;
+  }
   if (cutDepth > (uint32_t)kMlpCutDepth) {
     fail("gprtProposalBindAccel: cutDepth %u exceeds %d (at most 4096 treelets)", cutDepth, kMlpCutDepth);
     return 0;
@@ -199,24 +224,13 @@ GPRT_API void gprtProposalGradients(GPRTContext _ctx, GPRTProposal _p, GPRTBuffe
   hipStreamSynchronize(s);
 }
 
-GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _hits,
-                                 uint32_t count, uint32_t epochs, float lr, float *accuracy) {
-  Context *ctx = (Context *)_ctx;
-  Proposal *p = (Proposal *)_p;
-  Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits;
-  if (!ctx || !p || !rays || !hits) {
-    fail("gprtProposalTrain: null argument");
-    return NAN;
-  }
-  if (!p->accel || !p->primClass) {
-    fail("gprtProposalTrain: bind the proposal to an accel first (gprtProposalBindAccel)");
-    return NAN;
-  }
-  if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * sizeof(GPRTHit) > hits->size) {
-    fail("gprtProposalTrain: %u rays exceed the ray or hit buffer", count);
-    return NAN;
-  }
-  if (count == 0) return 0.f;
+namespace {
+
+// Labels come from plain-trace hits (triangle accels: the treelet of the hit primitive) or are
+// given per ray; unlabelled rays are compacted away, then `epochs` passes of minibatch Adam run
+// over the rest. Returns the last epoch's mean loss.
+float train_core(Context *ctx, Proposal *p, const void *rays, const void *hits, const int32_t *userLabels,
+                 uint32_t count, uint32_t epochs, float lr, float *accuracy) {
   hipStream_t s = ctx->stream;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   // Scratch: labels | flags | offsets | compacted rays | compacted labels | scan state
@@ -230,11 +244,15 @@ GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _
   GPRTRay *cRays = (GPRTRay *)(b + 4 * al((size_t)count * 4));
   void *scanState = b + 4 * al((size_t)count * 4) + al((size_t)count * sizeof(GPRTRay));
   const unsigned g = (count + 255) / 256;
-  mlp_labels(ctx, s, p, hits->d, count, labels);
-  launch(ctx, s, "mlp_label_flags", k_label_flags, dim3(g), dim3(256), 0, (const int32_t *)labels, count, flags);
+  if (hits)
+    mlp_labels(ctx, s, p, hits, count, labels);
+  else
+    hip_ok(hipMemcpyAsync(labels, userLabels, (size_t)count * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+  launch(ctx, s, "mlp_label_flags", k_label_flags, dim3(g), dim3(256), 0, (const int32_t *)labels, count,
+         p->numClasses ? p->numClasses : (uint32_t)kMlpOut, flags);
   scan_run(ctx, s, flags, offs, count, 0, false, scanState, nullptr);
-  launch(ctx, s, "mlp_compact", k_compact, dim3(g), dim3(256), 0, (const GPRTRay *)rays->d, (const int32_t *)labels,
-         (const uint32_t *)offs, count, cRays, cLabels);
+  launch(ctx, s, "mlp_compact", k_compact, dim3(g), dim3(256), 0, (const GPRTRay *)rays, (const int32_t *)labels,
+         (const uint32_t *)flags, (const uint32_t *)offs, count, cRays, cLabels);
   uint32_t labelled = 0;
   hip_ok(hipMemcpyAsync(&labelled, scanState, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(labelled)");
   hipStreamSynchronize(s);
@@ -260,6 +278,52 @@ GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _
   }
   if (accuracy) *accuracy = acc;
   return lastLoss;
+}
+
+
+}  // namespace
+
+GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _hits,
+                                 uint32_t count, uint32_t epochs, float lr, float *accuracy) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits;
+  if (!ctx || !p || !rays || !hits) {
+    fail("gprtProposalTrain: null argument");
+    return NAN;
+  }
+  if (!p->accel) {
+    fail("gprtProposalTrain: bind the proposal to an accel first (gprtProposalBindAccel)");
+    return NAN;
+  }
+  if (!p->primClass) {
+    fail("gprtProposalTrain: hits of an instance accel carry no class; use gprtProposalTrainLabels with the "
+         "leafIDs of gprtAccelTraceClosestHitDetail");
+    return NAN;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * sizeof(GPRTHit) > hits->size) {
+    fail("gprtProposalTrain: %u rays exceed the ray or hit buffer", count);
+    return NAN;
+  }
+  if (count == 0) return 0.f;
+  return train_core(ctx, p, rays->d, hits->d, nullptr, count, epochs, lr, accuracy);
+}
+
+GPRT_API float gprtProposalTrainLabels(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _labels,
+                                       uint32_t count, uint32_t epochs, float lr, float *accuracy) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Buffer *rays = (Buffer *)_rays, *labels = (Buffer *)_labels;
+  if (!ctx || !p || !rays || !labels) {
+    fail("gprtProposalTrainLabels: null argument");
+    return NAN;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * 4 > labels->size) {
+    fail("gprtProposalTrainLabels: %u rays exceed the ray or label buffer", count);
+    return NAN;
+  }
+  if (count == 0) return 0.f;
+  return train_core(ctx, p, rays->d, nullptr, (const int32_t *)labels->d, count, epochs, lr, accuracy);
 }
 
 GPRT_API void gprtProposalInfer(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, uint32_t count,
@@ -288,6 +352,10 @@ GPRT_API void gprtProposalExportTreelets(GPRTContext _ctx, GPRTProposal _p, GPRT
     fail("gprtProposalExportTreelets: null argument or unbound proposal");
     return;
   }
+  if (!p->primClass) {
+    fail("gprtProposalExportTreelets: the proposal is bound to an instance accel (classes are its top-level leaves)");
+    return;
+  }
   if (cr->size < (size_t)p->numClasses * 4 && !buffer_realloc(cr, (size_t)p->numClasses * 4, false)) return;
   if (pc->size < (size_t)p->treePrims * 4 && !buffer_realloc(pc, (size_t)p->treePrims * 4, false)) return;
   hip_ok(hipMemcpyAsync(cr->d, p->classRoot, (size_t)p->numClasses * 4, hipMemcpyDeviceToDevice, ctx->stream),
@@ -296,31 +364,30 @@ GPRT_API void gprtProposalExportTreelets(GPRTContext _ctx, GPRTProposal _p, GPRT
          "hipMemcpyAsync");
 }
 
-GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
-                                         GPRTBuffer _hits, uint32_t count, GPRTBuffer _visits) {
-  Context *ctx = (Context *)_ctx;
-  Accel *a = (Accel *)_a;
-  Proposal *p = (Proposal *)_p;
-  Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits, *visits = (Buffer *)_visits;
+namespace {
+
+void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *rays, Buffer *hits, uint32_t count,
+                  Buffer *inst, Buffer *leaf, Buffer *visits) {
   if (!ctx || !a || !rays || !hits) {
-    fail("gprtAccelTraceClosestHitEx: null argument");
+    fail("%s: null argument", fn);
     return;
   }
   if (!a->built) {
-    fail("gprtAccelTraceClosestHitEx: accel has not been built");
+    fail("%s: accel has not been built", fn);
     return;
   }
   if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * sizeof(GPRTHit) > hits->size ||
-      (visits && (size_t)count * 4 > visits->size)) {
-    fail("gprtAccelTraceClosestHitEx: %u rays exceed a buffer", count);
+      (visits && (size_t)count * 4 > visits->size) || (inst && (size_t)count * 4 > inst->size) ||
+      (leaf && (size_t)count * 4 > leaf->size)) {
+    fail("%s: %u rays exceed a buffer", fn, count);
     return;
   }
   if (p && p->accel != a) {
-    fail("gprtAccelTraceClosestHitEx: the proposal is bound to another accel");
+    fail("%s: the proposal is bound to another accel", fn);
     return;
   }
   if (p && p->treePrims != a->n) {
-    fail("gprtAccelTraceClosestHitEx: the accel was rebuilt with a different size; bind the proposal again");
+    fail("%s: the accel was rebuilt with a different size; bind the proposal again", fn);
     return;
   }
   if (count == 0) return;
@@ -331,6 +398,32 @@ GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTPro
     cls = (int32_t *)p->rayScratch;
     mlp_infer(ctx, s, p, rays->d, count, cls, nullptr);
   }
-  trace_closest_ex(ctx, s, a, rays->d, hits->d, count, cls, p ? p->classRoot : nullptr, p ? p->numClasses : 0,
-                   visits ? (uint32_t *)visits->d : nullptr);
+  uint32_t *vis = visits ? (uint32_t *)visits->d : nullptr;
+  if (a->kind == AccelKind::Instances) {
+    tlas_trace(ctx, s, a, rays->d, hits->d, count, cls, p ? p->numClasses : 0, inst ? (int32_t *)inst->d : nullptr,
+               leaf ? (int32_t *)leaf->d : nullptr, vis);
+    return;
+  }
+  trace_closest_ex(ctx, s, a, rays->d, hits->d, count, cls, p ? p->classRoot : nullptr, p ? p->numClasses : 0, vis);
+  if (inst || leaf) {
+    // Triangle accel: instance 0 on a hit, no top-level leaf.
+    Buffer *hb = hits;
+    if (inst) instance_ids_of_hits(ctx, s, (const GPRTHit *)hb->d, count, (int32_t *)inst->d);
+    if (leaf) fill_u32(ctx, s, leaf->d, 0xFFFFFFFFu, count);
+  }
+}
+
+}  // namespace
+
+GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
+                                         GPRTBuffer _hits, uint32_t count, GPRTBuffer _visits) {
+  trace_detail("gprtAccelTraceClosestHitEx", (Context *)_ctx, (Accel *)_a, (Proposal *)_p, (Buffer *)_rays,
+               (Buffer *)_hits, count, nullptr, nullptr, (Buffer *)_visits);
+}
+
+GPRT_API void gprtAccelTraceClosestHitDetail(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
+                                             GPRTBuffer _hits, uint32_t count, GPRTBuffer _inst, GPRTBuffer _leaf,
+                                             GPRTBuffer _visits) {
+  trace_detail("gprtAccelTraceClosestHitDetail", (Context *)_ctx, (Accel *)_a, (Proposal *)_p, (Buffer *)_rays,
+               (Buffer *)_hits, count, (Buffer *)_inst, (Buffer *)_leaf, (Buffer *)_visits);
 }

@@ -225,6 +225,7 @@ GPRT_API void gprtContextDestroy(GPRTContext _ctx) {
   for (auto e : ctx->timer.pool) hipEventDestroy(e);
   if (ctx->arena) hipFree(ctx->arena);
   if (ctx->traceScratch) hipFree(ctx->traceScratch);
+  if (ctx->blasPool) hipFree(ctx->blasPool);
   if (ctx->pinnedWord) hipHostFree(ctx->pinnedWord);
   hipEventDestroy(ctx->profBegin);
   hipEventDestroy(ctx->profEnd);

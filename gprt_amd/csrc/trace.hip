@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(kTraceBlock)
   __shared__ uint32_t s_stack[kLdsStack][kTraceBlock];
   const int tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * kTraceBlock + tid;
-  uint32_t *mySpill = spill + (size_t)gtid * (kMaxStack - kLdsStack);
+  const LaneStack st{s_stack, tid, spill + (size_t)gtid * (kMaxStack - kLdsStack)};
   const int l = tid & 63;
 
   while (true) {
@@ -46,19 +46,21 @@ __global__ void __launch_bounds__(kTraceBlock)
       const GPRTRay ray = rays[ri];
       const RayPre r = ray_setup(ray.ox, ray.oy, ray.oz, ray.tmin, ray.dx, ray.dy, ray.dz);
       HitState h;
-      h.t = ray.tmax;
-      h.u = h.v = 0.f;
-      h.prim = -1;
+      hit_init(h, ray.tmax);
       uint32_t visits = 0;
+      int skip = kNoSkip;
       if (proposal) {
         const int32_t c = proposal[ri];
         if (c >= 0 && (uint32_t)c < numClasses) {
           const int32_t root = classRoot[c];
           // A treelet rooted at the tree's root (cut depth 0) would only repeat the full pass.
-          if (root != 0) visits += traverse(r, nodes, tris, nprims, root, h, s_stack, tid, mySpill);
+          if (root != 0) {
+            visits += traverse<kMaxStack>(r, nodes, tris, nprims, root, h, st, 0, 0u, -1);
+            skip = root;  // searched completely: the full pass does not enter it again
+          }
         }
       }
-      visits += traverse(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, s_stack, tid, mySpill);
+      visits += traverse<kMaxStack>(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, st, 0, 0u, -1, skip);
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
       o.primID = h.prim;
@@ -72,11 +74,9 @@ __global__ void __launch_bounds__(kTraceBlock)
 
 }  // namespace
 
-bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
-                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits) {
-  if (count == 0) return true;
+uint32_t *ensure_trace_scratch(Context *ctx, hipStream_t s, int spillWordsPerLane) {
   const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
-  const size_t need = (size_t)blocks * kTraceBlock * (kMaxStack - kLdsStack) * 4 + 256;
+  const size_t need = (size_t)blocks * kTraceBlock * (size_t)spillWordsPerLane * 4 + 256;
   if (need > ctx->traceBytes) {
     if (ctx->traceScratch) {
       hipStreamSynchronize(s);
@@ -84,15 +84,22 @@ bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, v
       ctx->traceScratch = nullptr;
       ctx->traceBytes = 0;
     }
-    if (!hip_ok(hipMalloc(&ctx->traceScratch, need), "hipMalloc(trace scratch)")) return false;
+    if (!hip_ok(hipMalloc(&ctx->traceScratch, need), "hipMalloc(trace scratch)")) return nullptr;
     ctx->traceBytes = need;
   }
-  uint32_t *work = (uint32_t *)ctx->traceScratch;
-  void *spill = (char *)ctx->traceScratch + 256;
+  return (uint32_t *)ctx->traceScratch;
+}
+
+bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
+                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits) {
+  if (count == 0) return true;
+  uint32_t *work = ensure_trace_scratch(ctx, s, kMaxStack - kLdsStack);
+  if (!work) return false;
+  const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
   fill_u32(ctx, s, work, 0u, 1);
   launch(ctx, s, proposal ? "trace_closest_proposal" : "trace_closest", k_trace, dim3(blocks), dim3(kTraceBlock), 0,
          (const BVHNode *)a->nodes, (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count,
-         work, (uint32_t *)spill, proposal, classRoot, numClasses, visits);
+         work, work + 64, proposal, classRoot, numClasses, visits);
   return true;
 }
 

@@ -1,5 +1,13 @@
-// Device-side closest-hit traversal shared by the single-accel and multi-instance trace kernels.
+// Device-side closest-hit traversal shared by the triangle-accel and instance-accel trace kernels.
+//
+// One node fetch decides both children (64-byte node with both child boxes); leaf children are
+// handed to a leaf callback at once; the near inner child is descended and the far one pushed on
+// a per-lane stack whose first kLdsStack entries live in LDS and the rest in a per-lane global
+// spill area. The stack is shared by nested traversals (instance accels: the bottom-level walk
+// starts above the top-level entries), so a traversal takes the stack slot it starts at.
 #pragma once
+
+#include <climits>
 
 #include "bvh.hpp"
 
@@ -8,43 +16,76 @@ namespace gprt {
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 20;
 constexpr int kBlocksPerCU = 14;  // 28 waves per CU: the register-limited occupancy at ~60 VGPRs
+// A binary radix tree over distinct 64-bit keys is at most 64 levels deep, so a single-level walk
+// needs at most 64 entries and a two-level walk at most 128.
 constexpr int kMaxStack = 64;
+constexpr int kMaxStack2 = 128;
 
+// Closest hit so far. Hits are ordered by (t, inst, prim) with unsigned comparisons, so the
+// result does not depend on the order in which subtrees or instances are visited.
 struct HitState {
   float t, u, v;
-  int prim;
+  int prim;      // -1: none yet
+  uint32_t inst; // instance index (0 for a triangle accel); 0xFFFFFFFF: none yet
+  int leaf;      // instance accels: index of the top-level leaf (sub-root) holding the hit
+};
+
+__device__ __forceinline__ void hit_init(HitState &h, float tmax) {
+  h.t = tmax;
+  h.u = h.v = 0.f;
+  h.prim = -1;
+  h.inst = 0xFFFFFFFFu;
+  h.leaf = -1;
+}
+
+__device__ __forceinline__ bool hit_closer(float t, uint32_t inst, int prim, const HitState &h) {
+  return t < h.t || (t == h.t && (inst < h.inst || (inst == h.inst && (unsigned)prim < (unsigned)h.prim)));
+}
+
+struct LaneStack {
+  uint32_t (*lds)[kTraceBlock];
+  int tid;
+  uint32_t *spill;  // per-lane global entries [kLdsStack, maxStack)
 };
 
 __device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord *__restrict__ tris, int leaf,
-                                               uint32_t nprims, HitState &h) {
+                                               uint32_t nprims, uint32_t inst, int tag, HitState &h) {
   if ((uint32_t)leaf >= nprims) return;
   const TriRecord tr = tris[leaf];
   float t, u, v;
   if (tri_hit(r, tr.v0, tr.v1, tr.v2, t, u, v)) {
     const int prim = (int)__float_as_uint(tr.v0.w);
-    if (t < h.t || (t == h.t && (unsigned)prim < (unsigned)h.prim)) {
+    if (hit_closer(t, inst, prim, h)) {
       h.t = t;
       h.prim = prim;
       h.u = u;
       h.v = v;
+      h.inst = inst;
+      h.leaf = tag;
     }
   }
 }
 
-// Depth-first traversal of the subtree rooted at `start` (an inner node index, or ~leaf).
-__device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__restrict__ nodes,
-                                             const TriRecord *__restrict__ tris, uint32_t nprims, int start,
-                                             HitState &h, uint32_t (*s_stack)[kTraceBlock], int tid,
-                                             uint32_t *__restrict__ mySpill) {
+// Depth-first walk of the subtree rooted at `start` (an inner node index, or ~leaf position) of a
+// tree with `nprims` leaves, using stack slots [sp0, MaxStack). leaf(position, spTop) is called
+// for every leaf whose box the ray meets; spTop is the first free stack slot. A child equal to
+// `skip` (a subtree already searched completely; kNoSkip for none) is not entered. Returns the
+// number of inner nodes fetched.
+constexpr int kNoSkip = INT_MIN;
+
+template <int MaxStack, class LeafFn>
+__device__ __forceinline__ uint32_t walk(const RayPre &r, const BVHNode *__restrict__ nodes, uint32_t nprims,
+                                         int start, const HitState &h, const LaneStack &st, int sp0, int skip,
+                                         LeafFn &&leaf) {
   if (start < 0) {
-    intersect_leaf(r, tris, ~start, nprims, h);
+    leaf(~start, sp0);
     return 0;
   }
   int node = start;
-  int sp = 0;
+  int sp = sp0;
   uint32_t visits = 0;
-  // Each inner node is fetched at most once per traversal; the step bound and the index check
-  // only guard against a malformed tree.
+  // Each inner node is fetched at most once per walk; the step bound and the index check only
+  // guard against a malformed tree.
   for (uint32_t step = 0; step < nprims; ++step) {
     if ((uint32_t)node >= nprims - 1) break;
     const BVHNode nd = nodes[node];
@@ -53,12 +94,16 @@ __device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__r
     bool h0 = box_hit(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, h.t, tn0);
     bool h1 = box_hit(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, h.t, tn1);
     const int c0 = nd.d.x, c1 = nd.d.y;
+    h0 = h0 && c0 != skip;
+    h1 = h1 && c1 != skip;
     if (h0 && c0 < 0) {
-      intersect_leaf(r, tris, ~c0, nprims, h);
+      leaf(~c0, sp);
       h0 = false;
     }
     if (h1 && c1 < 0) {
-      intersect_leaf(r, tris, ~c1, nprims, h);
+      // The first leaf may have tightened h.t; the box test above used the older bound, which
+      // only costs a redundant leaf test.
+      leaf(~c1, sp);
       h1 = false;
     }
     if (h0 && h1) {
@@ -66,9 +111,9 @@ __device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__r
       const int nearC = firstNear ? c0 : c1;
       const int farC = firstNear ? c1 : c0;
       if (sp < kLdsStack)
-        s_stack[sp][tid] = (uint32_t)farC;
-      else if (sp < kMaxStack)
-        mySpill[sp - kLdsStack] = (uint32_t)farC;
+        st.lds[sp][st.tid] = (uint32_t)farC;
+      else if (sp < MaxStack)
+        st.spill[sp - kLdsStack] = (uint32_t)farC;
       ++sp;
       node = nearC;
     } else if (h0) {
@@ -76,13 +121,25 @@ __device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__r
     } else if (h1) {
       node = c1;
     } else {
-      if (sp == 0) break;
+      if (sp == sp0) break;
       --sp;
-      if (sp >= kMaxStack) break;  // unreachable for trees of depth <= 64
-      node = (int)(sp < kLdsStack ? s_stack[sp][tid] : mySpill[sp - kLdsStack]);
+      if (sp >= MaxStack) break;  // unreachable within the depth bounds above
+      node = (int)(sp < kLdsStack ? st.lds[sp][st.tid] : st.spill[sp - kLdsStack]);
     }
   }
   return visits;
+}
+
+// Triangle-accel walk: leaves are triangles of instance `inst`; `tag` is recorded with a hit.
+template <int MaxStack>
+__device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__restrict__ nodes,
+                                             const TriRecord *__restrict__ tris, uint32_t nprims, int start,
+                                             HitState &h, const LaneStack &st, int sp0, uint32_t inst, int tag,
+                                             int skip = kNoSkip) {
+  return walk<MaxStack>(r, nodes, nprims, start, h, st, sp0, skip,
+                        [&](int leaf, int) __attribute__((always_inline)) {
+                          intersect_leaf(r, tris, leaf, nprims, inst, tag, h);
+                        });
 }
 
 }  // namespace gprt

@@ -131,16 +131,54 @@ def bounce_rays(prim_rays: np.ndarray, hits: np.ndarray, verts: np.ndarray, idx:
 
 
 def bounce_rays_multi(prim_rays: np.ndarray, hits: np.ndarray, inst: np.ndarray, meshes, count: int,
-                      seed: int = 3, tmin: float = 1e-4) -> np.ndarray:
-    """bounce_rays over several meshes: hit i lies on triangle hits[i].primID of meshes[inst[i]]."""
-    offs_v = np.cumsum([0] + [len(v) for v, _ in meshes])
+                      seed: int = 3, tmin: float = 1e-4, transforms=None) -> np.ndarray:
+    """bounce_rays over instances: hit i lies on triangle hits[i].primID of meshes[inst[i]], whose
+    vertices are carried to world space by transforms[inst[i]] (3x4 [A | b], FP64, fixed order)."""
+    world = []
+    for k, (v, idx) in enumerate(meshes):
+        if transforms is None:
+            world.append(v)
+            continue
+        m = np.asarray(transforms[k], dtype=np.float32).astype(np.float64)
+        vd = v.astype(np.float64)
+        w = np.empty_like(vd)
+        for c in range(3):
+            w[:, c] = ((m[c, 0] * vd[:, 0] + m[c, 1] * vd[:, 1]) + m[c, 2] * vd[:, 2]) + m[c, 3]
+        world.append(w)
+    offs_v = np.cumsum([0] + [len(v) for v in world])
     offs_t = np.cumsum([0] + [len(i) for _, i in meshes])
-    verts = np.concatenate([v for v, _ in meshes])
+    verts = np.concatenate(world)
     idx = np.concatenate([i.astype(np.int64) + offs_v[k] for k, (_, i) in enumerate(meshes)])
     h = hits.copy()
     hit = h["primID"] >= 0
     h["primID"][hit] = h["primID"][hit] + offs_t[inst[hit]]
     return bounce_rays(prim_rays, h, verts, idx, count, seed, tmin)
+
+
+def object_rays(rays: np.ndarray, transform) -> np.ndarray:
+    """Rays in the object space of an instance with 3x4 transform [A | b] (world = A x + b): the
+    inverse in FP64 (adjugate / determinant), rounded to FP32, applied in FP32 in a fixed order."""
+    A = np.asarray(transform, dtype=np.float32).astype(np.float64)
+    a, b = A[:, :3], A[:, 3]
+    det = (a[0, 0] * (a[1, 1] * a[2, 2] - a[1, 2] * a[2, 1]) - a[0, 1] * (a[1, 0] * a[2, 2] - a[1, 2] * a[2, 0])
+           + a[0, 2] * (a[1, 0] * a[2, 1] - a[1, 1] * a[2, 0]))
+    idt = 1.0 / det
+    inv = np.array([
+        [(a[1, 1] * a[2, 2] - a[1, 2] * a[2, 1]) * idt, (a[0, 2] * a[2, 1] - a[0, 1] * a[2, 2]) * idt,
+         (a[0, 1] * a[1, 2] - a[0, 2] * a[1, 1]) * idt],
+        [(a[1, 2] * a[2, 0] - a[1, 0] * a[2, 2]) * idt, (a[0, 0] * a[2, 2] - a[0, 2] * a[2, 0]) * idt,
+         (a[0, 2] * a[1, 0] - a[0, 0] * a[1, 2]) * idt],
+        [(a[1, 0] * a[2, 1] - a[1, 1] * a[2, 0]) * idt, (a[0, 1] * a[2, 0] - a[0, 0] * a[2, 1]) * idt,
+         (a[0, 0] * a[1, 1] - a[0, 1] * a[1, 0]) * idt]])
+    binv = -(inv[:, 0] * b[0] + inv[:, 1] * b[1] + inv[:, 2] * b[2])
+    m = np.concatenate([inv, binv[:, None]], 1).astype(np.float32)
+    out = rays.copy()
+    o = [rays["ox"], rays["oy"], rays["oz"]]
+    d = [rays["dx"], rays["dy"], rays["dz"]]
+    for c, (on, dn) in enumerate([("ox", "dx"), ("oy", "dy"), ("oz", "dz")]):
+        out[on] = ((m[c, 0] * o[0] + m[c, 1] * o[1]) + m[c, 2] * o[2]) + m[c, 3]
+        out[dn] = (m[c, 0] * d[0] + m[c, 1] * d[1]) + m[c, 2] * d[2]
+    return out
 
 
 def random_rays(n: int, seed: int, lo=-0.5, hi=1.5) -> np.ndarray:

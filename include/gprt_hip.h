@@ -236,7 +236,9 @@ GPRT_API void gprtAccelTraceClosestHit(GPRTContext context, GPRTAccel accel, GPR
 typedef struct GPRTProposal_T *GPRTProposal;
 GPRT_API GPRTProposal gprtProposalCreate(GPRTContext context, uint32_t maxBatch, uint64_t seed);
 GPRT_API void gprtProposalDestroy(GPRTContext context, GPRTProposal proposal);
-/* Cuts the accel's tree at cutDepth; returns the number of treelets. Rebind after a rebuild. */
+/* Cuts the accel's tree at cutDepth; returns the number of treelets. Rebind after a rebuild.
+ * For an instance accel the classes are its top-level leaves (cutDepth is ignored; at most
+ * 4096). */
 GPRT_API uint32_t gprtProposalBindAccel(GPRTContext context, GPRTProposal proposal, GPRTAccel accel,
                                         uint32_t cutDepth);
 GPRT_API size_t gprtProposalParamCount(void);
@@ -246,6 +248,11 @@ GPRT_API void gprtProposalGetParams(GPRTContext context, GPRTProposal proposal, 
  * written by gprtAccelTraceClosestHit). Returns the last epoch's mean loss; *accuracy = top-1. */
 GPRT_API float gprtProposalTrain(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, GPRTBuffer hits,
                                  uint32_t count, uint32_t epochs, float learningRate, float *accuracy);
+/* Same with explicit labels (int32 class per ray, < 0 = not labelled), e.g. the leafIDs of an
+ * instance-accel trace, whose classes are its top-level leaves. */
+GPRT_API float gprtProposalTrainLabels(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays,
+                                       GPRTBuffer labels, uint32_t count, uint32_t epochs, float learningRate,
+                                       float *accuracy);
 /* Top-1 treelet per ray (int32), fused: the 4096 logits are never written to memory. */
 GPRT_API void gprtProposalInfer(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, uint32_t count,
                                 GPRTBuffer classes);
@@ -262,6 +269,35 @@ GPRT_API void gprtProposalExportTreelets(GPRTContext context, GPRTProposal propo
  * fetched (visits, uint32, may be NULL). */
 GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext context, GPRTAccel accel, GPRTProposal proposal,
                                          GPRTBuffer rays, GPRTBuffer hits, uint32_t count, GPRTBuffer visits);
+
+/* Instance accels (two-level scenes). gprtInstanceAccelCreate / gprtAccelGetInstance keep the
+ * reference's names and the gprt::Instance record (gprt/gprt_host.h:1418, :1570;
+ * gprt/gprt_shared.h:92-101): a 3x4 row-major transform [A | b] (world = A * object + b),
+ * instanceCustomIndex:24, mask:8 (0 hides the instance), geometryIndex:24, flags:8, and
+ * accelAddress = gprtAccelGetDeviceAddress of a built triangle accel. Build with gprtAccelBuild
+ * after the triangle accels it names are built (and again after they are rebuilt or the
+ * records change). Traces (gprtAccelTraceClosestHit / Ex / Detail) return, per ray, the closest
+ * hit over all instances under the order (t, instance index, primID), primID being local to
+ * the instance's geometry; t is the ray parameter of the world-space ray. At most 65536
+ * instances; an address that does not name a live, built triangle accel of this context
+ * disables its instance. */
+typedef struct GPRTInstance {
+  float transform[3][4];
+  uint32_t instanceCustomIndex : 24;
+  uint32_t mask : 8;
+  uint32_t geometryIndex : 24;
+  uint32_t flags : 8;
+  uint64_t accelAddress;
+} GPRTInstance;
+GPRT_API GPRTAccel gprtInstanceAccelCreate(GPRTContext context, uint32_t numInstances, GPRTBuffer instances);
+GPRT_API GPRTInstance gprtAccelGetInstance(GPRTAccel blas);
+/* Closest hit with optional outputs (each may be NULL): instanceIDs (int32, -1 on a miss; 0 for
+ * a triangle accel), leafIDs (int32: for an instance accel the top-level leaf holding the hit,
+ * the class the proposal learns for such accels; -1 otherwise) and visits (uint32, inner nodes
+ * fetched). `proposal` may be NULL. */
+GPRT_API void gprtAccelTraceClosestHitDetail(GPRTContext context, GPRTAccel accel, GPRTProposal proposal,
+                                             GPRTBuffer rays, GPRTBuffer hits, uint32_t count,
+                                             GPRTBuffer instanceIDs, GPRTBuffer leafIDs, GPRTBuffer visits);
 
 /* Export of the built LBVH in the reference's development layout (gprt/development/gprt_lbvh.h:28-55):
  * nodes int4[2N-1] {left, right, parent, leaf primID or -1} with inner nodes [0, N-2] and leaf
@@ -290,19 +326,16 @@ GPRT_API void gprtSynthBlobMeshes(GPRTContext context, const int64_t *sizes, uin
 GPRT_API void gprtSynthPerturb(GPRTContext context, GPRTBuffer src, GPRTBuffer dst, uint64_t floats, uint64_t seed,
                                float amplitude);
 
-/* Scenes of several triangle accels (identity transforms; no reference counterpart at buffer
- * level: the reference traces instances through gprtInstanceAccelCreate, gprt/gprt_host.h:1418,
- * and the driver's TLAS). gprtAccelTraceClosestHitMulti returns, per ray, the closest hit over
- * all `count` accels under the order (t, instance index, primID), and the instance index
- * (int32, -1 on a miss) in instanceIDs (may be NULL). gprtSynthBounceRays writes `outCount`
- * cosine-weighted bounce rays (tmin given, tmax 1e30) from the hit points of the first `count`
- * traced rays: misses are skipped, hits are cycled in ray order, seeded as
- * gprt_amd/scenes.bounce_rays; with no hit at all every output ray is empty (tmax < tmin). */
-GPRT_API void gprtAccelTraceClosestHitMulti(GPRTContext context, GPRTAccel *accels, uint32_t count, GPRTBuffer rays,
-                                            GPRTBuffer hits, uint32_t rayCount, GPRTBuffer instanceIDs);
-GPRT_API void gprtSynthBounceRays(GPRTContext context, GPRTAccel *accels, uint32_t count, GPRTBuffer rays,
-                                  GPRTBuffer hits, GPRTBuffer instanceIDs, uint32_t rayCount, GPRTBuffer out,
-                                  uint32_t outCount, uint64_t seed, float tmin);
+/* Bounce rays for a traced batch (benchmarks, tests): `outCount` cosine-weighted rays (tmin as
+ * given, tmax 1e30) from the hit points of the first `rayCount` rays of `rays` / `hits` (as
+ * written by a closest-hit trace of `accel`). Misses are skipped and hits are cycled in ray
+ * order; the normal is the world-space geometric normal facing the incoming ray; seeded as
+ * gprt_amd/scenes.bounce_rays. For an instance accel, instanceIDs holds each hit's instance
+ * (from gprtAccelTraceClosestHitDetail). With no hit at all every output ray is empty
+ * (tmax < tmin). */
+GPRT_API void gprtSynthBounceRays(GPRTContext context, GPRTAccel accel, GPRTBuffer rays, GPRTBuffer hits,
+                                  GPRTBuffer instanceIDs, uint32_t rayCount, GPRTBuffer out, uint32_t outCount,
+                                  uint64_t seed, float tmin);
 
 /* Library version string, for logs. */
 GPRT_API const char *gprtVersion(void);

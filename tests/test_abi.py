@@ -75,3 +75,30 @@ def test_build_params_layout():
 
 def test_version_string():
     assert b"gfx950" in api.load().gprtVersion()
+
+
+def test_instance_record_layout(tmp_path):
+    """GPRTInstance as a C compiler lays it out equals api.INSTANCE_DTYPE / api.GPRTInstance
+    (gprt::Instance, gprt/gprt_shared.h:92-101: 48 B transform, two bitfield words, address)."""
+    import shutil
+    import subprocess
+    import numpy as np
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <string.h>\n#include <stddef.h>\n#include "gprt_hip.h"\n'
+        "int main(void) { GPRTInstance x; memset(&x, 0, sizeof x);\n"
+        "  x.transform[1][3] = 2.5f; x.instanceCustomIndex = 0x123456; x.mask = 0xAB;\n"
+        "  x.geometryIndex = 7; x.flags = 3; x.accelAddress = 0x1122334455667788ull;\n"
+        "  fwrite(&x, sizeof x, 1, stdout); return 0; }\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    raw = subprocess.run([str(exe)], check=True, capture_output=True).stdout
+    assert len(raw) == 64 == api.INSTANCE_DTYPE.itemsize == C.sizeof(api.GPRTInstance)
+    rec = np.frombuffer(raw, dtype=api.INSTANCE_DTYPE)[0]
+    assert rec["transform"][1, 3] == 2.5
+    assert rec["word0"] == 0xAB123456 and rec["word1"] == 0x03000007
+    assert rec["accelAddress"] == 0x1122334455667788
+    c = api.GPRTInstance.from_buffer_copy(raw)
+    assert (c.instanceCustomIndex, c.mask, c.geometryIndex, c.flags) == (0x123456, 0xAB, 7, 3)
