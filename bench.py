@@ -279,10 +279,13 @@ def batched_workload(args):
     per_mesh_ms = timed(per_mesh, args.steps)
     lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
     batch_ms = timed(batch, args.steps)
-    rep = C.create_string_buffer(1 << 14)
-    best = lib.gprtAccelBuildBatchAutotune(ctx.h, ev, len(accs), 3, rep, len(rep))
-    tune = json.loads(rep.value.decode())
-    tuned_ms = timed(batch, args.steps)
+    if args.no_autotune:
+        tune, best, tuned_ms = {"best": None, "configs": []}, -1, batch_ms
+    else:
+        rep = C.create_string_buffer(1 << 14)
+        best = lib.gprtAccelBuildBatchAutotune(ctx.h, ev, len(accs), 3, rep, len(rep))
+        tune = json.loads(rep.value.decode())
+        tuned_ms = timed(batch, args.steps)
     # device-event time of one graph replay (the batch build alone)
     lib.gprtKernelTimerReset(ctx.h)
     lib.gprtKernelTimerEnable(ctx.h, True)
@@ -337,6 +340,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--meshes", type=int, default=4096)
+    ap.add_argument("--no-autotune", action="store_true", help="batched: skip the autotuner sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "

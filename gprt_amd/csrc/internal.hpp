@@ -135,14 +135,18 @@ struct BVHStore {
   void *tris = nullptr;   // TriRecord[prims]
   uint32_t *parentLeaf = nullptr, *parentInner = nullptr;
   float *headers = nullptr;  // [meshes][16]
-  uint32_t *entries = nullptr;  // [prims][2] refit edges into nodes spanning tiles
+  uint32_t *entries = nullptr;  // [prims] per-mesh lists of refit edges into nodes spanning sub-tiles
+  void *topo = nullptr;         // int4[inner]: {child0, child1, other end of the range, 0}
+  uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into nodes spanning regions
+  uint32_t *edgeCount = nullptr;    // [meshes] (in the zeroed range)
+  uint32_t *regionCount = nullptr;  // [tiles] (in the zeroed range)
   // Zeroed at the start of every build (one contiguous range):
   uint32_t *zeroBase = nullptr;
   size_t zeroWords = 0;
   uint32_t *counters = nullptr;  // [inner]
   uint32_t *hist = nullptr;      // [meshes][passes][R]
   uint32_t *status = nullptr;    // [passes][tiles][R]
-  uint32_t *tickets = nullptr;   // [passes]; word 32 counts refit edges
+  uint32_t *tickets = nullptr;   // [passes]
   // Host copies of the plan (to detect when a cached graph can be replayed).
   std::vector<MeshDesc> desc;
   std::vector<uint32_t> tileMesh;

@@ -51,8 +51,14 @@ struct BuildArgs {
   uint32_t *parentLeaf, *parentInner, *counters;
   float *headers;
   uint32_t *hist, *status, *tickets;
-  uint32_t *entries, *entryCount;  // refit edges into nodes spanning tiles: {mesh, link}
-  uint32_t entryCap;
+  int4 *topo;            // per inner node {child0, child1, other end of the leaf range, 0}
+  // Refit edges (links into nodes that a refit level cannot finish), at the mesh's primOffset:
+  // entries: from sub-tiles into nodes spanning sub-tiles, one list per 65536-key region (count
+  // at regionCount[first tile of the region]); entriesTop: from regions into nodes spanning
+  // regions, one list per mesh (count at edgeCount[mesh]).
+  uint32_t *entries, *entriesTop;
+  uint32_t *regionCount;
+  uint32_t *edgeCount;
   int passes;
   int digitBits;
 };
@@ -408,64 +414,81 @@ __global__ void __launch_bounds__(256) k_copy_back(BuildArgs a, uint32_t prims) 
 // ------------------------------------------------------------------------------------------------
 // Karras 2012 split search (gprt_lbvh.hlsl:64-103, :287-338). One thread per inner node.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ int delta64(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
-                                       uint64_t ki, int64_t j, int64_t n) {
-  if (j < 0 || j >= n) return -1;
-  return __clzll(ki ^ (((uint64_t)codes[j] << 32) | ids[j]));
-}
+// One node per thread over 1024-key sub-tiles; the sub-tile's keys and 1024 keys on each side
+// are staged in LDS as 64-bit keys, so nearly every probe of the three searches is an LDS read
+// (a probe outside the staged window reads the key arrays).
+constexpr int kKarrasSub = 1024;
+constexpr int kKarrasHalo = 1024;
 
-__global__ void __launch_bounds__(kThreads) k_karras(BuildArgs a, uint32_t tileBase) {
-  const uint32_t t = tileBase + blockIdx.x;
+__global__ void __launch_bounds__(kKarrasSub) k_karras(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
+  __shared__ uint64_t s_key[kKarrasSub + 2 * kKarrasHalo];
+  const uint32_t t = tileBase + blockIdx.x / subPerTile, sub = blockIdx.x % subPerTile;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const TileRange r = tile_range(a, d, t, m);
+  const uint32_t lo = r.lo + sub * kKarrasSub;
+  if (lo >= r.hi) return;  // uniform over the workgroup
+  const uint32_t hi = min(lo + (uint32_t)kKarrasSub, r.hi);
   const int64_t N = d.ntri;
   const uint32_t *__restrict__ codes = a.codes + d.primOffset;
   const uint32_t *__restrict__ ids = a.ids + d.primOffset;
-  BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
+  int4 *__restrict__ topo = a.topo + d.innerOffset;
   uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
   uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
-  const uint32_t hiInner = min(r.hi, d.ntri - 1);
-  for (uint32_t ii = r.lo + threadIdx.x; ii < hiInner; ii += kThreads) {
-    const int64_t i = ii;
-    const uint64_t ki = ((uint64_t)codes[i] << 32) | ids[i];
-    const int dir = delta64(codes, ids, ki, i + 1, N) >= delta64(codes, ids, ki, i - 1, N) ? 1 : -1;
-    const int dmin = delta64(codes, ids, ki, i - dir, N);
-    int64_t lmax = 2;
-    while (delta64(codes, ids, ki, i + lmax * dir, N) > dmin) lmax *= 2;
-    int64_t len = 0;
-    for (int64_t s = lmax >> 1; s >= 1; s >>= 1)
-      if (delta64(codes, ids, ki, i + (len + s) * dir, N) > dmin) len += s;
-    const int64_t j = i + len * dir;
-    const int dnode = delta64(codes, ids, ki, j, N);
-    int64_t s = 0, step = len;
-    do {
-      step = (step + 1) >> 1;
-      if (delta64(codes, ids, ki, i + (s + step) * dir, N) > dnode) s += step;
-    } while (step > 1);
-    const int64_t split = i + s * dir + (dir < 0 ? -1 : 0);
-    const int64_t first = dir > 0 ? i : j;
-    const int64_t last = dir > 0 ? j : i;
-    if (split < first || split >= last) continue;  // only possible with repeated keys
-    int c0, c1;
-    if (split == first) {
-      c0 = ~(int)split;
-      parentLeaf[split] = ((uint32_t)i << 1) | 0u;
-    } else {
-      c0 = (int)split;
-      parentInner[split] = ((uint32_t)i << 1) | 0u;
-    }
-    if (split + 1 == last) {
-      c1 = ~(int)(split + 1);
-      parentLeaf[split + 1] = ((uint32_t)i << 1) | 1u;
-    } else {
-      c1 = (int)(split + 1);
-      parentInner[split + 1] = ((uint32_t)i << 1) | 1u;
-    }
-    // d.z keeps the other end of the node's leaf range [min(i,j), max(i,j)] for the refit.
-    nodes[i].d = make_int4(c0, c1, (int)j, 0);
-    if (i == 0) parentInner[0] = 0xFFFFFFFFu;
+  const int64_t w0 = (int64_t)lo - kKarrasHalo;  // first staged position
+  for (int q = threadIdx.x; q < kKarrasSub + 2 * kKarrasHalo; q += kKarrasSub) {
+    const int64_t j = w0 + q;
+    s_key[q] = (j >= 0 && j < N) ? (((uint64_t)codes[j] << 32) | ids[j]) : 0ull;
   }
+  __syncthreads();
+  const int64_t wEnd = min((int64_t)hi + kKarrasHalo, N);
+  auto key = [&](int64_t j) -> uint64_t {
+    return (j >= w0 && j < wEnd) ? s_key[j - w0] : (((uint64_t)codes[j] << 32) | ids[j]);
+  };
+  auto delta = [&](uint64_t ki, int64_t j) -> int {
+    if (j < 0 || j >= N) return -1;
+    return __clzll(ki ^ key(j));
+  };
+  const int64_t i = (int64_t)lo + threadIdx.x;
+  if (i >= (int64_t)hi || i >= N - 1) return;
+  const uint64_t ki = s_key[i - w0];
+  const int dir = delta(ki, i + 1) >= delta(ki, i - 1) ? 1 : -1;
+  const int dmin = delta(ki, i - dir);
+  int64_t lmax = 2;
+  while (delta(ki, i + lmax * dir) > dmin) lmax *= 2;
+  int64_t len = 0;
+  for (int64_t s = lmax >> 1; s >= 1; s >>= 1)
+    if (delta(ki, i + (len + s) * dir) > dmin) len += s;
+  const int64_t j = i + len * dir;
+  const int dnode = delta(ki, j);
+  int64_t s = 0, step = len;
+  do {
+    step = (step + 1) >> 1;
+    if (delta(ki, i + (s + step) * dir) > dnode) s += step;
+  } while (step > 1);
+  const int64_t split = i + s * dir + (dir < 0 ? -1 : 0);
+  const int64_t first = dir > 0 ? i : j;
+  const int64_t last = dir > 0 ? j : i;
+  if (split < first || split >= last) return;  // only possible with repeated keys
+  int c0, c1;
+  if (split == first) {
+    c0 = ~(int)split;
+    parentLeaf[split] = ((uint32_t)i << 1) | 0u;
+  } else {
+    c0 = (int)split;
+    parentInner[split] = ((uint32_t)i << 1) | 0u;
+  }
+  if (split + 1 == last) {
+    c1 = ~(int)(split + 1);
+    parentLeaf[split + 1] = ((uint32_t)i << 1) | 1u;
+  } else {
+    c1 = (int)(split + 1);
+    parentInner[split + 1] = ((uint32_t)i << 1) | 1u;
+  }
+  // Compact topology (coalesced 16-byte writes); the refit writes each full node once. z keeps
+  // the other end of the node's leaf range [min(i,j), max(i,j)].
+  topo[i] = make_int4(c0, c1, (int)j, 0);
+  if (i == 0) parentInner[0] = 0xFFFFFFFFu;
 }
 
 __device__ __forceinline__ void store_child_box(BVHNode *nodes, uint32_t node, uint32_t side, const float b[6]) {
@@ -496,27 +519,54 @@ __device__ __forceinline__ void merge_box(float box[6], const float sib[6]) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Refit, phase 1 (one workgroup per tile): leaf records and every inner node whose leaf range
-// lies inside the tile. Arrival counters for those nodes live in LDS and the boxes are ordered
-// with workgroup-scope acquire/release, which costs no cache maintenance. A climb that reaches a
-// node whose range leaves the tile writes its box into that node and records the edge for
-// phase 2.
+// Refit, phase 1 (one 1024-thread workgroup per 1024-key sub-tile, one leaf per thread): leaf
+// records and every inner node whose leaf range lies inside the sub-tile. The climb runs
+// entirely in LDS: the topology of the sub-tile's candidate nodes is loaded once (coalesced), a
+// finished node's box and each leaf box stay in LDS, arrival counters are LDS atomics ordered by
+// LDS-only fences (no wait on global stores), and the finished nodes' child boxes are written in
+// one coalesced pass at the end. A climb that reaches a node whose range leaves the sub-tile
+// writes its box into that node's child slot and records the edge for phase 2.
 // ------------------------------------------------------------------------------------------------
-template <int TILE>
-__global__ void __launch_bounds__(kThreads) k_refit_local(BuildArgs a, uint32_t tileBase) {
-  __shared__ uint32_t s_cnt[TILE];
-  const uint32_t t = tileBase + blockIdx.x;
+constexpr int kRefitSub = 1024;
+constexpr uint32_t kRefitRegion = 65536;  // keys per phase-2 region (a multiple of every tile size)
+
+__device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
+__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+
+__global__ void __launch_bounds__(kRefitSub) k_refit_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
+  __shared__ float s_box[kRefitSub][6];   // box of a finished local inner node (index - lo)
+  __shared__ float s_leaf[kRefitSub][6];  // leaf boxes (position - lo)
+  __shared__ uint32_t s_cnt[kRefitSub];
+  __shared__ uint32_t s_par[kRefitSub];
+  __shared__ int s_c0[kRefitSub], s_c1[kRefitSub], s_other[kRefitSub];
+  const uint32_t t = tileBase + blockIdx.x / subPerTile, sub = blockIdx.x % subPerTile;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const TileRange r = tile_range(a, d, t, m);
+  const uint32_t lo = r.lo + sub * kRefitSub;
+  if (lo >= r.hi) return;  // uniform over the workgroup
+  const uint32_t hi = min(lo + (uint32_t)kRefitSub, r.hi);
   const uint32_t n = d.ntri;
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
   const uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
   const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
   float *__restrict__ header = a.headers + (size_t)m * 16;
-  for (uint32_t i = threadIdx.x; i < (uint32_t)TILE; i += kThreads) s_cnt[i] = 0u;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t i = lo + tid;  // candidate inner node (index inside the sub-tile)
+  const bool cand = i < hi && i + 1 < n;
+  s_cnt[tid] = 0u;
+  int4 dd = make_int4(0, 0, 0, 0);
+  if (cand) {
+    dd = a.topo[d.innerOffset + i];
+    s_c0[tid] = dd.x;
+    s_c1[tid] = dd.y;
+    s_other[tid] = dd.z;
+    s_par[tid] = parentInner[i];
+  }
   __syncthreads();
-  for (uint32_t k = r.lo + threadIdx.x; k < r.hi; k += kThreads) {
+  const uint32_t k = i;  // leaf position
+  uint32_t edge = 0xFFFFFFFFu;  // link into a node spanning sub-tiles, for phase 2
+  if (k < hi) {
     const uint32_t prim = a.ids[d.primOffset + k];
     float3 p, q, s;
     load_triangle(d, prim, p, q, s);
@@ -534,75 +584,147 @@ __global__ void __launch_bounds__(kThreads) k_refit_local(BuildArgs a, uint32_t 
     box[5] = fmax_ord(p.z, fmax_ord(q.z, s.z));
     if (n == 1) {
       for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-      continue;
-    }
-    uint32_t link = parentLeaf[k];
-    // A radix tree over distinct 64-bit keys is at most 65 levels deep; the bound and the index
-    // checks only guard against a malformed topology.
-    for (int level = 0; level < 72; ++level) {
-      const uint32_t node = link >> 1, side = link & 1u;
-      if (node >= n - 1) break;
-      store_child_box(nodes, node, side, box);
-      const uint32_t other = (uint32_t)nodes[node].d.z;
-      const uint32_t first = min(node, other), last = max(node, other);
-      if (first < r.lo || last >= r.hi) {
-        // The node's range leaves this tile: phase 2 finishes it.
-        const uint32_t e = atomicAdd(a.entryCount, 1u);
-        if (e < a.entryCap) {
-          a.entries[2 * e] = m;
-          a.entries[2 * e + 1] = link;
+    } else {
+      for (int c = 0; c < 6; ++c) s_leaf[tid][c] = box[c];
+      uint32_t link = parentLeaf[k];
+      // A radix tree over distinct 64-bit keys is at most 65 levels deep; the bound and the
+      // index checks only guard against a malformed topology.
+      for (int level = 0; level < 72; ++level) {
+        const uint32_t node = link >> 1, side = link & 1u;
+        if (node >= n - 1) break;
+        bool local = node >= lo && node < hi;
+        if (local) {
+          const uint32_t other = (uint32_t)s_other[node - lo];
+          local = other >= lo && other < hi;
         }
-        break;
-      }
-      const uint32_t old =
-          __hip_atomic_fetch_add(&s_cnt[node - r.lo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old != 1u) break;  // first arrival: the sibling's thread finishes this node
-      float sib[6];
-      load_child_box(nodes, node, side ^ 1u, sib);
-      merge_box(box, sib);
-      link = parentInner[node];
-      if (link == 0xFFFFFFFFu) {
-        for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-        break;
+        if (!local) {
+          // The node's range leaves this sub-tile: phase 2 finishes it.
+          store_child_box(nodes, node, side, box);
+          edge = link;
+          break;
+        }
+        lds_release();
+        const uint32_t old =
+            __hip_atomic_fetch_add(&s_cnt[node - lo], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        lds_acquire();
+        if (old != 1u) break;  // first arrival: the sibling's thread finishes this node
+        // Both children lie inside the node's range, hence inside the sub-tile.
+        const int sib = side ? s_c0[node - lo] : s_c1[node - lo];
+        const float *sb = sib < 0 ? s_leaf[(uint32_t)(~sib) - lo] : s_box[(uint32_t)sib - lo];
+        float sbox[6];
+        for (int c = 0; c < 6; ++c) sbox[c] = sb[c];
+        merge_box(box, sbox);
+        for (int c = 0; c < 6; ++c) s_box[node - lo][c] = box[c];
+        link = s_par[node - lo];
+        if (link == 0xFFFFFFFFu) {
+          for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
+          break;
+        }
       }
     }
+  }
+  // Append the edges of this wave to the list of its 65536-key region: one atomic per wave.
+  {
+    const uint64_t mask = __ballot(edge != 0xFFFFFFFFu);
+    if (mask) {
+      const int lane = (int)(threadIdx.x & 63u);
+      const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      const uint32_t region = lo / kRefitRegion;
+      const uint32_t regionLo = region * kRefitRegion;
+      uint32_t *cnt = &a.regionCount[d.tileOffset + regionLo / a.tileKeys];
+      uint32_t base = 0;
+      if (lane == __ffsll((long long)mask) - 1) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+      base = __shfl(base, __ffsll((long long)mask) - 1, 64);
+      // A region's list holds at most one edge per key of the region (each edge roots a disjoint
+      // subtree of the region's leaves); the guard keeps writes inside the mesh's entries.
+      if (edge != 0xFFFFFFFFu && base + before < min(kRefitRegion, n - regionLo))
+        a.entries[d.primOffset + regionLo + base + before] = edge;
+    }
+  }
+  __syncthreads();
+  // Every candidate node is written whole (64 bytes): finished local nodes with both child boxes;
+  // nodes spanning sub-tiles get their topology here and their child boxes from the climbs.
+  if (cand) {
+    const uint32_t other = (uint32_t)dd.z;
+    float4 *dst = (float4 *)&nodes[i];
+    if (other >= lo && other < hi) {
+      const int c0 = dd.x, c1 = dd.y;
+      const float *b0 = c0 < 0 ? s_leaf[(uint32_t)(~c0) - lo] : s_box[(uint32_t)c0 - lo];
+      const float *b1 = c1 < 0 ? s_leaf[(uint32_t)(~c1) - lo] : s_box[(uint32_t)c1 - lo];
+      dst[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
+      dst[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
+      dst[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
+    }
+    dst[3] = make_float4(__int_as_float(dd.x), __int_as_float(dd.y), __int_as_float(dd.z), 0.0f);
   }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Refit, phase 2: the nodes whose range spans several tiles. Each recorded edge arrives at its
-// node through a device-scope acquire/release counter; the second arrival merges both child
-// boxes and climbs. Only these few nodes pay for cross-XCD ordering.
+// Refit, phases 2 and 3: the nodes whose range spans several sub-tiles. Phase 2 runs one
+// workgroup per 65536-key region and finishes the nodes whose range lies inside the region;
+// phase 3 runs one workgroup per mesh for the few nodes that span regions. Each edge climbs
+// through arrival counters ordered at workgroup scope (the workgroup's waves share one CU, so this
+// needs no L2 maintenance across XCDs); the second arrival merges both child boxes and climbs on.
+// Kernel boundaries order the phases.
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_refit_top(BuildArgs a) {
-  const uint32_t count = min(*a.entryCount, a.entryCap);
-  for (uint32_t e = blockIdx.x * kThreads + threadIdx.x; e < count; e += gridDim.x * kThreads) {
-    const uint32_t m = a.entries[2 * e];
-    uint32_t link = a.entries[2 * e + 1];
-    const MeshDesc d = a.desc[m];
-    const uint32_t n = d.ntri;
-    BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
-    const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
-    uint32_t *__restrict__ counters = a.counters + d.innerOffset;
-    float *__restrict__ header = a.headers + (size_t)m * 16;
-    for (int level = 0; level < 72; ++level) {
-      const uint32_t node = link >> 1, side = link & 1u;
-      if (node >= n - 1) break;
-      const uint32_t old =
-          __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (old != 1u) break;
-      float box[6], sib[6];
-      load_child_box(nodes, node, side, box);
-      load_child_box(nodes, node, side ^ 1u, sib);
-      merge_box(box, sib);
-      link = parentInner[node];
-      if (link == 0xFFFFFFFFu) {
-        for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-        break;
-      }
-      store_child_box(nodes, link >> 1, link & 1u, box);
+constexpr int kRefitTop = 512;
+
+// Climbs from `link` while the nodes' leaf ranges lie in [regionLo, regionHi); returns the link
+// into the first node outside (0xFFFFFFFF when the climb ended).
+__device__ __forceinline__ uint32_t refit_climb(const BuildArgs &a, const MeshDesc &d, uint32_t m, uint32_t link,
+                                                uint32_t regionLo, uint32_t regionHi) {
+  const uint32_t n = d.ntri;
+  BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
+  const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
+  uint32_t *__restrict__ counters = a.counters + d.innerOffset;
+  const int4 *__restrict__ topo = a.topo + d.innerOffset;
+  for (int level = 0; level < 72; ++level) {
+    const uint32_t node = link >> 1;
+    if (node >= n - 1) return 0xFFFFFFFFu;
+    const uint32_t other = (uint32_t)topo[node].z;
+    if (min(node, other) < regionLo || max(node, other) >= regionHi) return link;
+    const uint32_t old = __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old != 1u) return 0xFFFFFFFFu;
+    float box[6], sib[6];
+    load_child_box(nodes, node, 0, box);
+    load_child_box(nodes, node, 1, sib);
+    merge_box(box, sib);
+    link = parentInner[node];
+    if (link == 0xFFFFFFFFu) {
+      float *__restrict__ header = a.headers + (size_t)m * 16;
+      for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
+      return 0xFFFFFFFFu;
+    }
+    store_child_box(nodes, link >> 1, link & 1u, box);
+  }
+  return 0xFFFFFFFFu;
+}
+
+__global__ void __launch_bounds__(kRefitTop) k_refit_region(BuildArgs a, uint32_t tileBase) {
+  const uint32_t t = tileBase + blockIdx.x;
+  const uint32_t m = a.tileMesh[t];
+  const MeshDesc d = a.desc[m];
+  const uint32_t lt = t - d.tileOffset;
+  const uint32_t perRegion = kRefitRegion / a.tileKeys;
+  if (lt % perRegion != 0) return;  // one workgroup per region: its first tile
+  const uint32_t regionLo = lt * a.tileKeys;
+  const uint32_t regionHi = min(regionLo + kRefitRegion, d.ntri);
+  const uint32_t count = min(a.regionCount[t], regionHi - regionLo);
+  for (uint32_t e = threadIdx.x; e < count; e += kRefitTop) {
+    const uint32_t up = refit_climb(a, d, m, a.entries[d.primOffset + regionLo + e], regionLo, regionHi);
+    if (up != 0xFFFFFFFFu) {
+      const uint32_t slot = atomicAdd(&a.edgeCount[m], 1u);
+      if (slot < d.ntri) a.entriesTop[d.primOffset + slot] = up;
     }
   }
+}
+
+__global__ void __launch_bounds__(kRefitTop) k_refit_top(BuildArgs a, uint32_t meshBase) {
+  const uint32_t m = meshBase + blockIdx.x;
+  const MeshDesc d = a.desc[m];
+  const uint32_t count = min(a.edgeCount[m], d.ntri);
+  for (uint32_t e = threadIdx.x; e < count; e += kRefitTop)
+    refit_climb(a, d, m, a.entriesTop[d.primOffset + e], 0u, d.ntri);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -682,22 +804,22 @@ BuildArgs args_of(BVHStore *st) {
   a.status = st->status;
   a.tickets = st->tickets;
   a.entries = st->entries;
-  a.entryCount = st->tickets + 32;
-  a.entryCap = st->prims;
+  a.topo = (int4 *)st->topo;
+  a.entriesTop = st->entriesTop;
+  a.regionCount = st->regionCount;
+  a.edgeCount = st->edgeCount;
   a.passes = st->passes;
   a.digitBits = st->cfg.digitBits;
   return a;
 }
 
-void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a, uint32_t tileBase, uint32_t tiles) {
-  if (st->cfg.tileKeys == 2048)
-    launch(ctx, s, "lbvh_refit_local", k_refit_local<2048>, dim3(tiles), dim3(kThreads), 0, a, tileBase);
-  else if (st->cfg.tileKeys == 4096)
-    launch(ctx, s, "lbvh_refit_local", k_refit_local<4096>, dim3(tiles), dim3(kThreads), 0, a, tileBase);
-  else
-    launch(ctx, s, "lbvh_refit_local", k_refit_local<8192>, dim3(tiles), dim3(kThreads), 0, a, tileBase);
-  launch(ctx, s, "lbvh_refit_top", k_refit_top, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tiles, 1024))),
-         dim3(kThreads), 0, a);
+void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a, uint32_t tileBase, uint32_t tiles,
+                  uint32_t meshBase, uint32_t meshes) {
+  const uint32_t subPerTile = (uint32_t)st->cfg.tileKeys / kRefitSub;
+  launch(ctx, s, "lbvh_refit_local", k_refit_sub, dim3(tiles * subPerTile), dim3(kRefitSub), 0, a, tileBase,
+         subPerTile);
+  launch(ctx, s, "lbvh_refit_region", k_refit_region, dim3(tiles), dim3(kRefitTop), 0, a, tileBase);
+  launch(ctx, s, "lbvh_refit_top", k_refit_top, dim3(meshes), dim3(kRefitTop), 0, a, meshBase);
 }
 
 // Extra dynamic LDS that caps a sort pass at `wavesPerCU` resident waves per CU (160 KB of LDS
@@ -756,8 +878,9 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
   }
   if (st->passes & 1)
     launch(ctx, s, "lbvh_copy_back", k_copy_back, dim3(grid_for(st->prims, 256, 8192)), dim3(256), 0, a, st->prims);
-  launch(ctx, s, "lbvh_karras", k_karras, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
-  launch_refit(ctx, s, st, a, 0u, st->tiles);
+  launch(ctx, s, "lbvh_karras", k_karras, dim3(st->tiles * ((uint32_t)st->cfg.tileKeys / kKarrasSub)),
+         dim3(kKarrasSub), 0, a, 0u, (uint32_t)st->cfg.tileKeys / kKarrasSub);
+  launch_refit(ctx, s, st, a, 0u, st->tiles, 0u, st->meshes);
 }
 
 bool same_plan(const BVHStore *st, const std::vector<MeshDesc> &desc, const SortConfig &cfg) {
@@ -810,13 +933,19 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   const size_t oHdr = off;
   off += al((size_t)M * 16 * 4);
   const size_t oEnt = off;
-  off += al((size_t)prims * 8);
+  off += al((size_t)prims * 4);
+  const size_t oEntTop = off;
+  off += al((size_t)prims * 4);
+  const size_t oTopo = off;
+  off += al((size_t)std::max<uint32_t>(I, 1) * 16);
   const size_t oZero = off;
   const size_t cntWords = std::max<uint32_t>(I, 1);
   const size_t histWords = (size_t)M * passes * R;
   const size_t statWords = (size_t)passes * T * R;
   const size_t tickWords = 64;
-  const size_t zeroWords = cntWords + histWords + statWords + tickWords;
+  const size_t edgeWords = M;
+  const size_t regionWords = T;
+  const size_t zeroWords = cntWords + histWords + statWords + tickWords + edgeWords + regionWords;
   off += al(zeroWords * 4);
   if (off > st->blockBytes) {
     free_block(st);
@@ -842,12 +971,16 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->parentInner = (uint32_t *)(b + oPI);
   st->headers = (float *)(b + oHdr);
   st->entries = (uint32_t *)(b + oEnt);
+  st->topo = (int4 *)(b + oTopo);
+  st->entriesTop = (uint32_t *)(b + oEntTop);
   st->zeroBase = (uint32_t *)(b + oZero);
   st->zeroWords = zeroWords;
   st->counters = st->zeroBase;
   st->hist = st->counters + cntWords;
   st->status = st->hist + histWords;
   st->tickets = st->status + statWords;
+  st->edgeCount = st->tickets + tickWords;
+  st->regionCount = st->edgeCount + edgeWords;
   st->desc = desc;
   st->tileMesh = tileMesh;
   return true;
@@ -991,9 +1124,10 @@ bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   BVHStore *st = acc->store;
   const MeshDesc &d = st->desc[acc->mesh];
   if (d.ntri > 1) fill_u32(ctx, s, st->counters + d.innerOffset, 0u, d.ntri - 1);
-  fill_u32(ctx, s, st->tickets + 32, 0u, 1);  // refit edge count
+  fill_u32(ctx, s, st->edgeCount + acc->mesh, 0u, 1);
+  fill_u32(ctx, s, st->regionCount + d.tileOffset, 0u, d.ntiles);
   const BuildArgs a = args_of(st);
-  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles);
+  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u);
   return true;
 }
 
