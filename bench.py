@@ -276,7 +276,7 @@ def batched_workload(args):
     def batch():
         lib.gprtAccelBuildBatch(ctx.h, ev, len(accs), params)
 
-    per_mesh_ms = timed(per_mesh, args.steps)
+    per_mesh_ms = timed(per_mesh, args.steps) if not args.no_per_mesh else None
     lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
     batch_ms = timed(batch, args.steps)
     if args.no_autotune:
@@ -312,7 +312,8 @@ def batched_workload(args):
         "config": {"workload": f"{len(accs)} meshes, {total} triangles", "meshes": len(accs), "triangles": total},
         "components": {"per_mesh_launches_ms": per_mesh_ms, "batch_graph_default_ms": batch_ms,
                        "batch_graph_tuned_ms": tuned_ms, "batch_graph_device_ms": graph_ms,
-                       "speedup_vs_per_mesh": per_mesh_ms / tuned_ms, "autotune_best": tune["best"],
+                       "speedup_vs_per_mesh": per_mesh_ms / tuned_ms if per_mesh_ms else None,
+                       "autotune_best": tune["best"],
                        "autotune_best_index": int(best)},
         "roofline": {"kernel": "lbvh batch graph (all build kernels)", "bound": "hbm", "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
@@ -326,13 +327,35 @@ def batched_workload(args):
     ctx.close()
 
 
+def copy_workload(args):
+    """Calibration for HBM byte counters: a plain device-to-device copy of 1 GiB (torch's copy
+    kernel), timed with HIP events; under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its counters
+    give the factor between counted and moved bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * 4 * n / (ms * 1e-3) / 1e9,
+                      "unit": "GB/s", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                      "ms_per_copy": ms, "steps": args.steps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="scene", choices=["scene", "batched"])
+    ap.add_argument("--workload", default="scene", choices=["scene", "batched", "copy"])
     ap.add_argument("--cpu-rays", type=int, default=16384)
     ap.add_argument("--cpu-build-tris", type=int, default=1 << 20)
     ap.add_argument("--train-rays", type=int, default=4 << 20)
@@ -341,6 +364,7 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--meshes", type=int, default=4096)
     ap.add_argument("--no-autotune", action="store_true", help="batched: skip the autotuner sweep")
+    ap.add_argument("--no-per-mesh", action="store_true", help="batched: skip the per-mesh launch loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} requested; this build measures one GPU only "
@@ -348,6 +372,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         batched_workload(args)
+    elif args.workload == "copy":
+        copy_workload(args)
     else:
         scene_workload(args)
 

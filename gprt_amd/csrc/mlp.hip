@@ -19,6 +19,7 @@
 //   bf16 copies of W1t..W4t for the MFMA kernels; Adam moments m, v in the fp32 layout.
 //   Input features per ray (102): o.xyz, d.xyz, then for each of those 6 values v and each band
 //   k = 0..7: sin(pi * 2^k * v), cos(pi * 2^k * v); padded with zeros to 128.
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -292,7 +293,7 @@ constexpr int kChunk = 64;     // W4t rows (output classes) per staged chunk
 constexpr int kActStride = 136;  // bf16 per activation row in LDS (128 + 8 pad)
 constexpr int kWStride = 136;
 
-__global__ void __launch_bounds__(256) k_infer_top1(const GPRTRay *__restrict__ rays, uint32_t count,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_infer_top1(const GPRTRay *__restrict__ rays, uint32_t count,
                                                     const uint16_t *__restrict__ W1t,
                                                     const uint16_t *__restrict__ W2t,
                                                     const uint16_t *__restrict__ W3t,
@@ -371,48 +372,90 @@ __global__ void __launch_bounds__(256) k_infer_top1(const GPRTRay *__restrict__ 
     for (int ks = 0; ks < 4; ++ks)
       a[i][ks].u = *(const uint4 *)&myAct[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
   float best[4][4];
-  int bestIdx[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      best[i][q] = -INFINITY;
-      bestIdx[i][q] = 0x7FFFFFFF;
-    }
-  const float *b4 = bias + 3 * kMlpHid;
-  for (int c0 = 0; c0 < kMlpOut; c0 += kChunk) {
-    __syncthreads();
-    for (int i = tid; i < kChunk * 16; i += 256) {
-      const int row = i >> 4, q = i & 15;
-      *(uint4 *)&wbuf[row * kWStride + q * 8] = *(const uint4 *)&W4t[(size_t)(c0 + row) * 128 + q * 8];
-    }
-    __syncthreads();
+    for (int q = 0; q < 4; ++q) best[i][q] = -FLT_MAX;
+  // W4t streams through a two-chunk ring in the activation region (free once every wave holds
+  // its A fragments) by direct global-to-LDS loads: chunk c + 1 is in flight while chunk c is
+  // multiplied, and the copies use no registers. A chunk is 128 rows x 256 bytes, unpadded, with
+  // the 16-byte units of row r stored at position q ^ (r & 15), so the 16 rows read by one
+  // B-fragment load fall into distinct units. The output bias is staged in LDS (the weight
+  // region), so the multiply loop issues no vector-memory loads of its own and the explicit
+  // vmcnt waits below only count the ring copies.
+  constexpr int kOutChunk = 128;
+  constexpr int kChunks = kMlpOut / kOutChunk;
+  constexpr int kChunkBytes = kOutChunk * 256;
+  static_assert(2 * kChunkBytes <= kInferRays * kActStride * 2, "ring fits the activation region");
+  static_assert(kMlpOut * 4 <= 128 * kWStride * 2, "bias fits the weight region");
+  // Classes beyond numClasses get a bias of -FLT_MAX, so they never win and the multiply loop needs
+  // no per-element check (a finite value: -inf would encode to a signalling NaN below, which
+  // v_max_f32 turns into a quiet NaN that poisons the running maximum); the accumulators start at
+  // the bias, so no add is needed either.
+  float *sBias = (float *)wbuf;
+  __syncthreads();  // every wave holds its A fragments; the activation region becomes the ring
+  for (int i = tid; i < kMlpOut; i += 256) sBias[i] = (uint32_t)i < numClasses ? bias[3 * kMlpHid + i] : -FLT_MAX;
+  const uint32_t ringBase = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void *)smem;
+  // Wave w copies 1 KB blocks 8w .. 8w + 7 of a chunk (4 rows each).
+  auto issue = [&](int chunk, int buf) {
 #pragma unroll
-    for (int j = 0; j < kChunk / 16; ++j) {
-      f32x4_t acc[4];
+    for (int sblk = 0; sblk < 8; ++sblk) {
+      const int blk = w * 8 + sblk;
+      const int r = 4 * blk + (l >> 4);
+      const int q = (l & 15) ^ (r & 15);
+      const uint16_t *g = W4t + ((size_t)(chunk * kOutChunk + r) * 128 + q * 8);
+      const uint32_t dst = __builtin_amdgcn_readfirstlane(ringBase + (uint32_t)(buf * kChunkBytes + blk * 1024));
+      asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(dst), "v"(g) : "memory", "m0");
+    }
+  };
+  issue(0, 0);
+  for (int c = 0; c < kChunks; ++c) {
+    if (c + 1 < kChunks) {
+      issue(c + 1, (c + 1) & 1);
+      __builtin_amdgcn_s_waitcnt(0x0F78);  // vmcnt(8): this wave's copies of chunk c have landed
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    }
+    __syncthreads();  // every wave's copies of chunk c are visible
+    const uint16_t *cur = (const uint16_t *)((const char *)smem + (c & 1) * kChunkBytes);
+    const int c0 = c * kOutChunk;
+    // Column block j (16 classes) of this chunk: 16 MFMAs into accumulators seeded with the bias.
+    auto block = [&](int j, f32x4_t (&acc)[4]) __attribute__((always_inline)) {
+      const int r = j * 16 + (l & 15);
+      const float b = sBias[c0 + r];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      for (int i = 0; i < 4; ++i) acc[i] = f32x4_t{b, b, b, b};
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         Frag bf;
-        bf.u = *(const uint4 *)&wbuf[(j * 16 + (l & 15)) * kWStride + ks * 32 + 8 * (l >> 4)];
+        const int q = (ks * 4 + (l >> 4)) ^ (r & 15);
+        bf.u = *(const uint4 *)&cur[r * 128 + q * 8];
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = mfma16(a[i][ks], bf, acc[i]);
       }
-      const int col = c0 + j * 16 + (l & 15);
-      const float bcol = b4[col];
-      const bool valid = (uint32_t)col < numClasses;
+    };
+    // Running maximum with the class carried in the value: the low 12 mantissa bits of each
+    // logit are replaced by 4095 - class (2 VALU operations per logit). Among logits equal after
+    // the truncation (relative difference below 2^-11) the lower class wins when positive.
+    auto keep = [&](int j, const f32x4_t (&acc)[4]) __attribute__((always_inline)) {
+      const uint32_t code = 4095u - (uint32_t)(c0 + j * 16 + (l & 15));
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float v = acc[i][q] + bcol;
-          if (valid && v > best[i][q]) {
-            best[i][q] = v;
-            bestIdx[i][q] = col;
-          }
-        }
+        for (int q = 0; q < 4; ++q)
+          best[i][q] = fmaxf(best[i][q], __uint_as_float((__float_as_uint(acc[i][q]) & 0xFFFFF000u) | code));
+    };
+    // Two accumulator sets: the MFMAs of block j + 1 run while block j's maxima are taken.
+    f32x4_t accA[4], accB[4];
+    block(0, accA);
+#pragma unroll
+    for (int j = 0; j < kOutChunk / 16; j += 2) {
+      block(j + 1, accB);
+      keep(j, accA);
+      if (j + 2 < kOutChunk / 16) block(j + 2, accA);
+      keep(j + 1, accB);
     }
+    __syncthreads();  // chunk c's buffer is refilled in iteration c + 1
   }
   // Reduce over the 16 lanes that share rows (lane bits 0..3 index the column).
 #pragma unroll
@@ -420,16 +463,10 @@ __global__ void __launch_bounds__(256) k_infer_top1(const GPRTRay *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       float v = best[i][q];
-      int c = bestIdx[i][q];
 #pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        const float ov = __shfl_xor(v, o, 64);
-        const int oc = __shfl_xor(c, o, 64);
-        if (ov > v || (ov == v && oc < c)) {
-          v = ov;
-          c = oc;
-        }
-      }
+      for (int o = 1; o < 16; o <<= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+      int c = 4095 - (int)(__float_as_uint(v) & 0xFFFu);
+      if ((uint32_t)c >= numClasses) c = 0;  // only without any finite logit
       const uint32_t ri = ray0 + w * kInferRaysPerWave + i * 16 + 4 * (l >> 4) + q;
       if ((l & 15) == 0 && ri < count) {
         outClass[ri] = c;
@@ -734,7 +771,7 @@ void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint3
   const size_t lds = ((size_t)kInferRays * kActStride + (size_t)128 * kWStride) * 2;
   launch(ctx, s, "mlp_infer_top1", k_infer_top1, dim3(blocks_for(n, kInferRays)), dim3(256), lds,
          (const GPRTRay *)rays, n, mlp_wbf(p, 0), mlp_wbf(p, 1), mlp_wbf(p, 2), mlp_wbf(p, 3),
-         (const float *)p->bias, cls, logit, p->numClasses);
+         (const float *)p->bias, cls, logit, p->numClasses ? p->numClasses : (uint32_t)kMlpOut);
 }
 
 bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth) {

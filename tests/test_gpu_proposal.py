@@ -45,6 +45,32 @@ def test_forward_matches_cpu(ctx):
     prop.destroy()
 
 
+def test_fused_top1_matches_logits(ctx, scene):
+    """gprtProposalInfer (fused: logits never leave registers) picks the top logit of the GEMM path.
+    The fused kernel compares logits with their low 12 mantissa bits replaced by the class (relative
+    resolution 2^-11) and accumulates in another order, so a near-tie may resolve either way."""
+    prop = api.Proposal(ctx, max_batch=1024, seed=1)
+    prop.set_params(M.he_init(11).astype(np.float32))
+    rays = np.concatenate([scenes.random_rays(700, seed=13), scenes.pinhole_rays(17, 19)])  # 1023 rays
+    n = len(rays)
+    z = _logits(ctx, prop, rays)
+    rb, cb = ctx.device_buffer(rays), ctx.device_buffer(nbytes=4 * n)
+    for classes in (4096, None):
+        if classes is None:
+            classes = prop.bind(scene[2], 10)  # fewer than 4096 treelets: the argmax is restricted
+        ctx.lib.gprtProposalInfer(ctx.h, prop.h, rb.h, n, cb.h)
+        c = cb.read(np.int32, n)
+        assert ((c >= 0) & (c < classes)).all()
+        zz = z[:, :classes]
+        top = zz.max(1)
+        picked = zz[np.arange(n), c]
+        assert (top - picked <= 1e-3 * np.maximum(1.0, np.abs(top))).all()
+        assert np.mean(c == np.argmax(zz, 1)) > 0.97
+    rb.destroy()
+    cb.destroy()
+    prop.destroy()
+
+
 def test_gradients_match_cpu(ctx):
     prop = api.Proposal(ctx, max_batch=512, seed=1)
     p = M.he_init(8).astype(np.float32)
