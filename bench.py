@@ -327,6 +327,144 @@ def batched_workload(args):
     ctx.close()
 
 
+def dynamic_workload(args):
+    """Dynamic scene: 64 blob meshes of 65,536 triangles (mesh i: seed 5000 + i) under one instance
+    accel. Every frame f: all vertices = base + U(-1e-3, 1e-3) (seed 7000 + f), the 64 triangle
+    accels are rebuilt in one graph-captured batch and the instance accel on top of them, then
+    1024^2 pinhole primary rays and 1024^2 bounce rays from their hits (2,097,152 rays) are traced
+    plainly and with the learned proposal (classes = the instance accel's 4,096 top-level leaves).
+    Every 16 frames the proposal is fine-tuned on that frame's plain-traversal labels. Frame times
+    come from the library's per-kernel HIP events: plain frame = perturb + builds + bounce
+    generation + plain traces; learned frame = the same with inference + proposal-first traces
+    (+ the fine-tune in the frames that have one)."""
+    from gprt_amd import api, scenes
+
+    lib = api.load()
+    ctx = api.Context(0)
+    meshes, per = 64, 65536
+    sizes = np.full(meshes, per, dtype=np.int64)
+    base, ibuf = api.synth_blob_meshes(ctx, sizes, 5000, 0.002)
+    cur = ctx.device_buffer(nbytes=base.nbytes)
+    floats = base.nbytes // 4
+    lib.gprtSynthPerturb(ctx.h, base.h, cur.h, floats, 7000, 1e-3)
+    blas = api.mesh_accels(ctx, cur, ibuf, sizes)
+    api.build_batch(ctx, blas)
+    tlas = api.InstanceAccel(ctx, api.instance_records(blas))
+    tlas.build()
+    side = 1024
+    prim = scenes.pinhole_rays(side, side)
+    n = len(prim)
+    pb = ctx.device_buffer(prim)
+    bb = ctx.device_buffer(nbytes=32 * n)
+    hp, hb, hp2, hb2 = (ctx.device_buffer(nbytes=16 * n) for _ in range(4))
+    ip, ib_, lp, lb = (ctx.device_buffer(nbytes=4 * n) for _ in range(4))
+    prop = api.Proposal(ctx, max_batch=args.batch, seed=1)
+    classes = prop.bind(tlas)
+    leaves_per_class = int(lib.gprtAccelGetPrimCount(tlas.h)) // classes
+    tune_n = min(args.tune_rays, n)
+    log(f"dynamic scene: {meshes} x {per} triangles, {2 * n} rays/frame, {classes} proposal classes")
+
+    def frame(f, timed_frames):
+        lib.gprtSynthPerturb(ctx.h, base.h, cur.h, floats, 7000 + f, 1e-3)
+        api.build_batch(ctx, blas)
+        tlas.build()
+        api.trace_detail(ctx, tlas, pb, hp, n, ip, lp)
+        api.bounce_rays(ctx, tlas, pb, hp, ip, n, bb, n, seed=7000 + f)
+        api.trace_detail(ctx, tlas, bb, hb, n, ib_, lb)
+        tuned = None
+        if f % 16 == 0:
+            # Fine-tune on this frame's plain labels (a sample of the primary and bounce rays).
+            lib.gprtKernelTimerEnable(ctx.h, False)
+            ctx.sync()
+            k = tune_n // 2
+            l1 = prop.train_labels(pb, lp, k, epochs=args.tune_epochs, lr=args.lr)
+            l2 = prop.train_labels(bb, lb, k, epochs=args.tune_epochs, lr=args.lr)
+            tuned = (l1, l2)
+            lib.gprtKernelTimerEnable(ctx.h, timed_frames)
+        api.trace_detail(ctx, tlas, pb, hp2, n, proposal=prop)
+        api.trace_detail(ctx, tlas, bb, hb2, n, proposal=prop)
+        return tuned
+
+    for f in range(args.warmup):
+        frame(f, False)
+    ctx.sync()
+    frames = args.frames
+    plain_ms, learned_ms, build_ms, tplain, tlearn, tune_ms = [], [], [], [], [], []
+    mism = 0
+    hit_rates = []
+    t0 = time.perf_counter()
+    for f in range(frames):
+        lib.gprtKernelTimerReset(ctx.h)
+        lib.gprtKernelTimerEnable(ctx.h, True)
+        ts = time.perf_counter()
+        tuned = frame(f, True)
+        ctx.sync()
+        tune_wall = 0.0
+        lib.gprtKernelTimerEnable(ctx.h, False)
+        pk = kernel_report(lib, ctx)
+        common = ms_of(pk, "synth_")[0] + ms_of(pk, "lbvh_")[0] + ms_of(pk, "tlas_")[0] + ms_of(pk, "fill_u32")[0] \
+            + ms_of(pk, "scan_")[0]
+        tp = ms_of(pk, "trace_closest_instances")[0]
+        tl = ms_of(pk, "trace_closest_proposal_instances")[0] + ms_of(pk, "mlp_infer")[0]
+        plain_ms.append(common + tp)
+        learned_ms.append(common + tl)
+        build_ms.append(ms_of(pk, "lbvh_")[0] + ms_of(pk, "tlas_")[0])
+        tplain.append(tp)
+        tlearn.append(tl)
+        if tuned is not None:
+            tune_ms.append((time.perf_counter() - ts) * 1e3)
+        if f % 16 == 0 or f == frames - 1:
+            a1, a2 = hp.read(api.HIT_DTYPE, n), hp2.read(api.HIT_DTYPE, n)
+            b1, b2 = hb.read(api.HIT_DTYPE, n), hb2.read(api.HIT_DTYPE, n)
+            mism += int((a1.view(np.uint32).reshape(n, 4) != a2.view(np.uint32).reshape(n, 4)).any(1).sum())
+            mism += int((b1.view(np.uint32).reshape(n, 4) != b2.view(np.uint32).reshape(n, 4)).any(1).sum())
+            # Proposal hit rate: the proposed top-level leaf holds the closest hit.
+            cls = ctx.device_buffer(nbytes=4 * n)
+            lib.gprtProposalInfer(ctx.h, prop.h, pb.h, n, cls.h)
+            lab = lp.read(np.int32, n)
+            c = cls.read(np.int32, n)
+            m = lab >= 0
+            hit_rates.append(float((c[m] == lab[m] // leaves_per_class).mean()) if m.any() else 0.0)
+            cls.destroy()
+    wall = time.perf_counter() - t0
+    pct = lambda a, q: float(np.percentile(np.asarray(a), q))
+    rays = 2 * n
+    out = {
+        "metric": "dynamic scene frame time (64 x 65,536 tris rebuilt per frame + 2,097,152 rays), plain vs "
+                  "learned proposal, 1 GPU",
+        "value": rays / (pct(plain_ms, 50) * 1e3),
+        "unit": "Mrays/s (whole frame, plain, p50)",
+        "n_gpus": 1, "steps": frames, "warmup": args.warmup, "ms_per_step": wall * 1e3 / frames,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (geometry, traversal); "
+        "bf16 MFMA with f32 accumulate (proposal MLP)",
+        "data": "synthetic (device-generated blob meshes, seeded per-frame perturbation)",
+        "config": {"workload": f"{meshes} meshes x {per} triangles, {rays} rays/frame (1024^2 primary + 1024^2 "
+                               f"bounce), {frames} frames, fine-tune every 16 frames on {tune_n} labelled rays",
+                   "meshes": meshes, "triangles_per_mesh": per, "rays_per_frame": rays, "frames": frames,
+                   "proposal_classes": classes},
+        "components": {
+            "frame_ms_plain_p50": pct(plain_ms, 50), "frame_ms_plain_p99": pct(plain_ms, 99),
+            "frame_ms_learned_p50": pct(learned_ms, 50), "frame_ms_learned_p99": pct(learned_ms, 99),
+            "build_ms_p50": pct(build_ms, 50), "build_mprims_s_p50": meshes * per / (pct(build_ms, 50) * 1e3),
+            "trace_ms_plain_p50": pct(tplain, 50), "trace_mrays_s_plain_p50": rays / (pct(tplain, 50) * 1e3),
+            "trace_ms_learned_p50": pct(tlearn, 50), "trace_mrays_s_learned_p50": rays / (pct(tlearn, 50) * 1e3),
+            "learned_vs_plain_trace": pct(tplain, 50) / pct(tlearn, 50),
+            "fine_tune_frame_wall_ms_mean": float(np.mean(tune_ms)) if tune_ms else None,
+            "proposal_hit_rate_by_check": hit_rates, "hit_mismatches_learned_vs_plain": mism},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    for b in (pb, bb, hp, hb, hp2, hb2, ip, ib_, lp, lb, cur):
+        b.destroy()
+    prop.destroy()
+    tlas.destroy()
+    for a in blas:
+        a.destroy()
+    base.destroy()
+    ibuf.destroy()
+    ctx.close()
+
+
 def copy_workload(args):
     """Calibration for HBM byte counters: a plain device-to-device copy of 1 GiB (torch's copy
     kernel), timed with HIP events; under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its counters
@@ -355,7 +493,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="scene", choices=["scene", "batched", "copy"])
+    ap.add_argument("--workload", default="scene", choices=["scene", "batched", "dynamic", "copy"])
+    ap.add_argument("--frames", type=int, default=256, help="dynamic: timed frames")
+    ap.add_argument("--tune-rays", type=int, default=262144, help="dynamic: labelled rays per fine-tune")
+    ap.add_argument("--tune-epochs", type=int, default=2, help="dynamic: epochs per fine-tune")
     ap.add_argument("--cpu-rays", type=int, default=16384)
     ap.add_argument("--cpu-build-tris", type=int, default=1 << 20)
     ap.add_argument("--train-rays", type=int, default=4 << 20)
@@ -374,6 +515,8 @@ def main():
         batched_workload(args)
     elif args.workload == "copy":
         copy_workload(args)
+    elif args.workload == "dynamic":
+        dynamic_workload(args)
     else:
         scene_workload(args)
 

@@ -249,7 +249,8 @@ def last_error() -> str | None:
 class TriangleAccel:
     """gprtGeomTypeCreate + gprtGeomCreate + gprtTriangles* + gprtTriangleAccelCreate."""
 
-    def __init__(self, ctx: Context, verts: np.ndarray = None, idx: np.ndarray = None, *, buffers=None):
+    def __init__(self, ctx: Context, verts: np.ndarray = None, idx: np.ndarray = None, *, buffers=None,
+                 offsets=(0, 0), owns_buffers: bool = True):
         self.ctx, self.lib = ctx, ctx.lib
         if buffers is None:
             verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
@@ -259,10 +260,11 @@ class TriangleAccel:
             self.ibuf = ctx.device_buffer(idx)
         else:
             self.vbuf, self.ibuf, self.nv, self.nt = buffers
+        self.owns_buffers = owns_buffers
         self.gtype = self.lib.gprtGeomTypeCreate(ctx.h, GPRT_TRIANGLES, 0)
         self.geom = self.lib.gprtGeomCreate(ctx.h, self.gtype)
-        self.lib.gprtTrianglesSetVertices(self.geom, self.vbuf.h, self.nv, 12, 0)
-        self.lib.gprtTrianglesSetIndices(self.geom, self.ibuf.h, self.nt, 12, 0)
+        self.lib.gprtTrianglesSetVertices(self.geom, self.vbuf.h, self.nv, 12, offsets[0])
+        self.lib.gprtTrianglesSetIndices(self.geom, self.ibuf.h, self.nt, 12, offsets[1])
         self.h = self.lib.gprtTriangleAccelCreate(ctx.h, self.geom, 0)
 
     @classmethod
@@ -298,8 +300,9 @@ class TriangleAccel:
             self.lib.gprtAccelDestroy(self.h)
             self.lib.gprtGeomDestroy(self.geom)
             self.lib.gprtGeomTypeDestroy(self.gtype)
-            self.vbuf.destroy()
-            self.ibuf.destroy()
+            if self.owns_buffers:
+                self.vbuf.destroy()
+                self.ibuf.destroy()
             self.h = None
 
 
@@ -307,6 +310,28 @@ def build_batch(ctx: Context, accels, mode=GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE)
     """gprtAccelBuildBatch over a list of TriangleAccel objects."""
     arr = (C.c_void_p * len(accels))(*[a.h for a in accels])
     ctx.lib.gprtAccelBuildBatch(ctx.h, arr, len(accels), GPRTBuildParams(mode))
+
+
+def synth_blob_meshes(ctx: Context, sizes, seed0: int, offset: float = 0.002):
+    """gprtSynthBlobMeshes: blob meshes generated on the device into one vertex and one index
+    buffer (mesh i: seed seed0 + i, indices local to the mesh). Returns (vbuf, ibuf)."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+    total = int(sizes.sum())
+    vb, ib = ctx.device_buffer(nbytes=36 * total), ctx.device_buffer(nbytes=12 * total)
+    ctx.lib.gprtSynthBlobMeshes(ctx.h, sizes.ctypes.data_as(C.c_void_p), len(sizes), seed0, offset, vb.h, ib.h)
+    return vb, ib
+
+
+def mesh_accels(ctx: Context, vbuf: Buffer, ibuf: Buffer, sizes):
+    """One TriangleAccel per mesh of concatenated geometry (3 vertices per triangle, as written
+    by synth_blob_meshes), referencing the shared buffers at the mesh's offsets."""
+    accs, t0 = [], 0
+    for n in sizes:
+        n = int(n)
+        accs.append(TriangleAccel(ctx, buffers=(vbuf, ibuf, 3 * n, n), offsets=(36 * t0, 12 * t0),
+                                  owns_buffers=False))
+        t0 += n
+    return accs
 
 
 def _h(b):

@@ -2,14 +2,17 @@
 // records (gprt/gprt_host.h:1418, gprt/gprt_shared.h:92-101), gprtAccelGetInstance (:1570), build,
 // closest-hit trace, and bounce-ray generation over such scenes.
 //
-// The reference hands instances to the Vulkan driver's TLAS. Here the top level is an LBVH whose
-// leaves are the sub-roots of every instance's own LBVH cut at a fixed depth D (2^D slots per
-// instance, 64 * 2^D <= 4096 by default), with each sub-root's box carried into world space.
-// Meshes that overlap in space (every blob mesh spans the unit cube) then cost a ray about one
-// traversal of the union instead of one traversal per instance. A ray walks the top tree and, at
-// a leaf, walks that sub-root's subtree in the instance's object space; closest hits are ordered
-// by (t, instance index, primID), so the result equals the per-instance closest hits combined in
-// that order, whatever the visiting order.
+// The reference hands instances to the Vulkan driver's TLAS. Here the top level opens every
+// instance down to its triangles: an LBVH over one world-space box per triangle of every enabled
+// instance, built by the same builder as triangle accels. Instances that overlap in space (every
+// blob mesh spans the unit cube) then cost a ray what one merged tree over all their triangles
+// costs; leaves of one mesh only (whole instances, subtrees of 64 triangles or runs of 4
+// consecutive triangles were all tried) left the top level several times less selective. A ray
+// walks the top tree and intersects a leaf's triangle in its instance's object space; closest hits
+// are ordered by (t, instance index, primID), so the result equals the per-instance closest hits
+// combined in that order, whatever the visiting order. The instances' own trees serve the learned
+// proposal: its classes are fixed groups of consecutive (Morton-ordered) triangles of an instance,
+// each searched through the smallest node of the instance's tree that covers it.
 //
 // Instance records name triangle accels by gprt::Instance::accelAddress, which is a slot of the
 // context's descriptor pool (gprtAccelGetDeviceAddress); the build and the trace check every
@@ -45,7 +48,9 @@ struct TlasInst {
   const TriRecord *tris;
   const char *verts;
   const char *idx;
-  uint32_t n, flags, vstride, istride, nverts, slot, pad0, pad1;
+  uint32_t n, flags, vstride, istride, nverts, slot;
+  uint32_t first;      // first top-level leaf of this instance
+  uint32_t classSpan;  // triangles per proposal class of this instance
   float inv[12];  // object = inv[:, :3] * world + inv[:, 3]
   float fwd[12];  // the record's transform
 };
@@ -53,7 +58,17 @@ static_assert(sizeof(TlasInst) == 160, "TlasInst layout");
 
 struct Tlas {
   Buffer *instances = nullptr;
-  uint32_t numInstances = 0, depth = 0, leaves = 0;
+  uint32_t numInstances = 0, leaves = 0, leafCap = 0;
+  // Proposal classes: 2^classDepth per instance, class c of instance c >> classDepth holding its
+  // triangles [g * span, (g + 1) * span) in the instance's leaf order, g = c & (2^classDepth - 1);
+  // stable across rebuilds, unlike any numbering of the top tree.
+  uint32_t classDepth = 0, classes = 0;
+  int2 *classNodes = nullptr;
+  int32_t *leafClass = nullptr;   // [leaves] class of each top-level leaf, or -1
+  uint32_t *instFirst = nullptr;  // [numInstances + 1] first top-level leaf of each instance
+  std::vector<uint32_t> hostFirst;
+  std::vector<unsigned char> hostRecords;
+  uint64_t builtEpoch = ~0ull;  // the context's descriptor epoch when the table was last built
   void *block = nullptr;
   size_t blockBytes = 0;
   TlasInst *insts = nullptr;
@@ -78,17 +93,13 @@ __device__ __forceinline__ bool blas_live(const BlasDesc *__restrict__ pool, con
          d.idx == ti.idx;
 }
 
-// One thread per top-level slot s = instance * 2^D + path. The path bits (most significant first)
-// pick child 0 / 1 from the instance's root down D levels; a leaf reached early is the sub-root of
-// every slot sharing that prefix, kept by the slot whose remaining bits are zero (the others are
-// disabled but keep the same box, so the top tree only sees finite boxes).
-__global__ void __launch_bounds__(256) k_tlas_cut(const InstanceRecord *__restrict__ recs, uint32_t ninst,
-                                                  uint32_t depth, const BlasDesc *__restrict__ pool,
-                                                  TlasInst *__restrict__ insts, int2 *__restrict__ subroots,
-                                                  float *__restrict__ topVerts) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= (ninst << depth)) return;
-  const uint32_t i = s >> depth, path = s & ((1u << depth) - 1u);
+// One thread per instance: resolve the record's address to a live triangle accel of this context,
+// invert the transform (FP64, one rounding to FP32) and fill the instance table.
+__global__ void __launch_bounds__(256) k_tlas_inst(const InstanceRecord *__restrict__ recs, uint32_t ninst,
+                                                   uint32_t classDepth, const uint32_t *__restrict__ instFirst,
+                                                   const BlasDesc *__restrict__ pool, TlasInst *__restrict__ insts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
   const InstanceRecord rec = recs[i];
   const uint64_t base = (uint64_t)pool;
   bool valid = rec.addr >= base && rec.addr < base + (uint64_t)kBlasPoolSlots * sizeof(BlasDesc) &&
@@ -99,7 +110,6 @@ __global__ void __launch_bounds__(256) k_tlas_cut(const InstanceRecord *__restri
     d = pool[slot];
     valid = d.magic == kBlasMagic && d.n > 0;
   }
-  // Transform and its inverse (FP64, one rounding to FP32).
   double A[3][3], b[3];
   bool identity = true, finite = true;
   for (int r = 0; r < 3; ++r) {
@@ -130,80 +140,133 @@ __global__ void __launch_bounds__(256) k_tlas_cut(const InstanceRecord *__restri
     inv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
   }
   const uint32_t mask = rec.w0 >> 24;
-  if (path == 0) {
-    TlasInst t;
-    t.nodes = (const BVHNode *)d.nodes;
-    t.tris = (const TriRecord *)d.tris;
-    t.verts = d.verts;
-    t.idx = d.idx;
-    t.n = valid ? d.n : 0u;
-    t.flags = (valid && mask != 0 ? kInstEnabled : 0u) | (identity ? kInstIdentity : 0u);
-    t.vstride = d.vstride;
-    t.istride = d.istride;
-    t.nverts = d.nverts;
-    t.slot = slot;
-    t.pad0 = t.pad1 = 0;
-    for (int r = 0; r < 3; ++r) {
-      for (int c = 0; c < 3; ++c) t.inv[4 * r + c] = valid ? (float)inv[r][c] : 0.f;
-      t.inv[4 * r + 3] = valid ? (float)(-(inv[r][0] * b[0] + inv[r][1] * b[1] + inv[r][2] * b[2])) : 0.f;
-      for (int c = 0; c < 4; ++c) t.fwd[4 * r + c] = rec.xf[4 * r + c];
-    }
-    insts[i] = t;
+  TlasInst t;
+  t.nodes = (const BVHNode *)d.nodes;
+  t.tris = (const TriRecord *)d.tris;
+  t.verts = d.verts;
+  t.idx = d.idx;
+  t.n = valid ? d.n : 0u;
+  t.flags = (valid && mask != 0 ? kInstEnabled : 0u) | (identity ? kInstIdentity : 0u);
+  t.vstride = d.vstride;
+  t.istride = d.istride;
+  t.nverts = d.nverts;
+  t.slot = slot;
+  // The host laid the leaves out from its own view of the instances; an instance the device finds
+  // different (never for a consistent context) keeps its leaves disabled.
+  const uint32_t hostCount = instFirst[i + 1] - instFirst[i];
+  if (hostCount != t.n) {
+    t.n = 0;
+    t.flags &= ~kInstEnabled;
   }
-  bool enabled = valid && mask != 0;
-  int node = 0;
+  t.first = instFirst[i];
+  t.classSpan = t.n ? (t.n + (1u << classDepth) - 1u) >> classDepth : 0u;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) t.inv[4 * r + c] = valid ? (float)inv[r][c] : 0.f;
+    t.inv[4 * r + 3] = valid ? (float)(-(inv[r][0] * b[0] + inv[r][1] * b[1] + inv[r][2] * b[2])) : 0.f;
+    for (int c = 0; c < 4; ++c) t.fwd[4 * r + c] = rec.xf[4 * r + c];
+  }
+  insts[i] = t;
+}
+
+// World box of an object-space box under [A | b]: centre / half-extent form in FP64, widened by
+// 2^-18 of its magnitude so that rounding of the object-space ray cannot cull a hit.
+__device__ __forceinline__ void world_box(const float *fwd, float box[6]) {
+  double lo[3], hi[3];
+  for (int r = 0; r < 3; ++r) {
+    double c = fwd[4 * r + 3], e = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double ck = 0.5 * ((double)box[k] + (double)box[3 + k]);
+      const double ek = 0.5 * ((double)box[3 + k] - (double)box[k]);
+      c += (double)fwd[4 * r + k] * ck;
+      e += fabs((double)fwd[4 * r + k]) * ek;
+    }
+    const double pad = (fabs(c) + e) * 3.814697265625e-06 + 1e-30;
+    lo[r] = c - e - pad;
+    hi[r] = c + e + pad;
+  }
+  for (int r = 0; r < 3; ++r) {
+    box[r] = (float)lo[r];
+    box[3 + r] = (float)hi[r];
+    if ((double)box[r] > lo[r]) box[r] = nextafterf(box[r], -INFINITY);
+    if ((double)box[3 + r] < hi[r]) box[3 + r] = nextafterf(box[3 + r], INFINITY);
+  }
+}
+
+// One thread per top-level leaf s: the instance holding it (binary search over the instances' first
+// leaves) and its triangle p; the leaf's box is the world box of that triangle record. Leaves of a
+// disabled instance become a point (rays practically never enter them).
+__global__ void __launch_bounds__(256) k_tlas_cut(const TlasInst *__restrict__ insts, uint32_t ninst,
+                                                  const uint32_t *__restrict__ instFirst, uint32_t leaves,
+                                                  uint32_t classDepth, int2 *__restrict__ subroots,
+                                                  int32_t *__restrict__ leafClass, float *__restrict__ topVerts) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= leaves) return;
+  uint32_t lo = 0, hi = ninst;  // largest i with instFirst[i] <= s
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (instFirst[mid] <= s) lo = mid; else hi = mid;
+  }
+  const uint32_t i = lo;
+  const TlasInst t = insts[i];
+  const uint32_t p = s - instFirst[i];
+  const bool enabled = (t.flags & kInstEnabled) && p < t.n;
   float box[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (valid) {
-    node = d.n == 1 ? ~0 : 0;
-    for (int c = 0; c < 6; ++c) box[c] = d.header[8 + c];
-    const BVHNode *nodes = (const BVHNode *)d.nodes;
-    for (uint32_t lvl = 0; lvl < depth; ++lvl) {
-      if (node < 0 || (uint32_t)node >= d.n - 1) {
-        if (node >= 0) enabled = false;  // malformed tree: never reached for a built accel
-        if ((path & ((1u << (depth - lvl)) - 1u)) != 0) enabled = false;
-        break;
-      }
-      const uint32_t bit = (path >> (depth - 1 - lvl)) & 1u;
-      const BVHNode nd = nodes[node];
-      if (bit) {
-        box[0] = nd.b.z; box[1] = nd.b.w; box[2] = nd.c.x; box[3] = nd.c.y; box[4] = nd.c.z; box[5] = nd.c.w;
-        node = nd.d.y;
-      } else {
-        box[0] = nd.a.x; box[1] = nd.a.y; box[2] = nd.a.z; box[3] = nd.a.w; box[4] = nd.b.x; box[5] = nd.b.y;
-        node = nd.d.x;
-      }
-    }
-    if (!identity) {
-      // World box of the object box: centre / half-extent form in FP64, widened by 2^-18 of its
-      // magnitude so that rounding of the object-space ray cannot cull a hit.
-      double lo[3], hi[3];
-      for (int r = 0; r < 3; ++r) {
-        double c = b[r], e = 0.0;
-        for (int k = 0; k < 3; ++k) {
-          const double ck = 0.5 * ((double)box[k] + (double)box[3 + k]);
-          const double ek = 0.5 * ((double)box[3 + k] - (double)box[k]);
-          c += A[r][k] * ck;
-          e += fabs(A[r][k]) * ek;
-        }
-        const double pad = (fabs(c) + e) * 3.814697265625e-06 + 1e-30;
-        lo[r] = c - e - pad;
-        hi[r] = c + e + pad;
-      }
-      for (int r = 0; r < 3; ++r) {
-        box[r] = (float)lo[r];
-        box[3 + r] = (float)hi[r];
-        if ((double)box[r] > lo[r]) box[r] = nextafterf(box[r], -INFINITY);
-        if ((double)box[3 + r] < hi[r]) box[3 + r] = nextafterf(box[3 + r], INFINITY);
-      }
-    }
+  if (p < t.n) {
+    const TriRecord tr = t.tris[p];
+    box[0] = fmin_ord(tr.v0.x, fmin_ord(tr.v1.x, tr.v2.x));
+    box[1] = fmin_ord(tr.v0.y, fmin_ord(tr.v1.y, tr.v2.y));
+    box[2] = fmin_ord(tr.v0.z, fmin_ord(tr.v1.z, tr.v2.z));
+    box[3] = fmax_ord(tr.v0.x, fmax_ord(tr.v1.x, tr.v2.x));
+    box[4] = fmax_ord(tr.v0.y, fmax_ord(tr.v1.y, tr.v2.y));
+    box[5] = fmax_ord(tr.v0.z, fmax_ord(tr.v1.z, tr.v2.z));
+    if (!(t.flags & kInstIdentity)) world_box(t.fwd, box);
   }
-  subroots[s] = make_int2(enabled ? (int)i : -1, node);
+  if (!enabled) {
+    for (int c = 0; c < 3; ++c) box[c] = box[3 + c] = box[c] * 0.5f + box[3 + c] * 0.5f;
+  }
+  subroots[s] = make_int2(enabled ? (int)i : -1, (int)p);
+  leafClass[s] = enabled ? (int32_t)((i << classDepth) + p / t.classSpan) : -1;
   float *v = topVerts + (size_t)9 * s;
   for (int c = 0; c < 3; ++c) {
     v[c] = box[c];
     v[3 + c] = box[3 + c];
     v[6 + c] = box[c] * 0.5f + box[3 + c] * 0.5f;
   }
+}
+
+// Node of each proposal class: class c of instance i = c >> Dc covers `classSpan` consecutive leaves
+// of the instance's tree; its node is the smallest node whose leaf range holds them, found by
+// descending on the split positions (a left child's range ends at its index, or at ~index for a
+// leaf).
+__global__ void __launch_bounds__(256) k_tlas_classes(const TlasInst *__restrict__ insts, uint32_t ninst,
+                                                      uint32_t classDepth, int2 *__restrict__ classNodes) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (ninst << classDepth)) return;
+  const uint32_t i = c >> classDepth, g = c & ((1u << classDepth) - 1u);
+  const TlasInst t = insts[i];
+  const uint32_t span = t.classSpan;
+  const uint32_t a = g * span;
+  bool enabled = (t.flags & kInstEnabled) && a < t.n;
+  int node = 0;
+  if (enabled) {
+    const uint32_t b = min(a + span, t.n) - 1;
+    node = t.n == 1 ? ~0 : 0;
+    for (int level = 0; level < 72 && node >= 0; ++level) {
+      if ((uint32_t)node >= t.n - 1) {
+        enabled = false;  // malformed tree: never reached for a built accel
+        break;
+      }
+      const int4 d = t.nodes[node].d;
+      const uint32_t split = d.x >= 0 ? (uint32_t)d.x : (uint32_t)~d.x;
+      if (b <= split)
+        node = d.x;
+      else if (a > split)
+        node = d.y;
+      else
+        break;
+    }
+  }
+  classNodes[c] = make_int2(enabled ? (int)i : -1, node);
 }
 
 __global__ void __launch_bounds__(256) k_tlas_index(uint32_t *__restrict__ idx, uint32_t words) {
@@ -223,17 +286,20 @@ __device__ __forceinline__ RayPre object_ray(const TlasInst &ti, const GPRTRay &
   return ray_setup(ox, oy, oz, ray.tmin, dx, dy, dz);
 }
 
+// kCheck: the instance table may be stale (a triangle accel was destroyed or moved since this
+// instance accel was built), so every use of an instance first checks its descriptor.
+template <bool kCheck>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace_tlas(const BVHNode *__restrict__ topNodes, const TriRecord *__restrict__ topTris, uint32_t leaves,
                  const int2 *__restrict__ subroots, const TlasInst *__restrict__ insts,
                  const BlasDesc *__restrict__ pool, const GPRTRay *__restrict__ rays, GPRTHit *__restrict__ hits,
                  uint32_t count, uint32_t *__restrict__ work, uint32_t *__restrict__ spill,
-                 const int32_t *__restrict__ proposal, uint32_t numClasses, int32_t *__restrict__ instOut,
-                 int32_t *__restrict__ leafOut, uint32_t *__restrict__ visitsOut) {
+                 const int32_t *__restrict__ proposal, const int2 *__restrict__ classNodes, uint32_t numClasses,
+                 int32_t *__restrict__ instOut, int32_t *__restrict__ leafOut, uint32_t *__restrict__ visitsOut) {
   __shared__ uint32_t s_stack[kLdsStack][kTraceBlock];
   const int tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * kTraceBlock + tid;
-  const LaneStack st{s_stack, tid, spill + (size_t)gtid * (kMaxStack2 - kLdsStack)};
+  const LaneStack st{s_stack, tid, spill + (size_t)gtid * (kMaxStack - kLdsStack)};
   const int l = tid & 63;
   while (true) {
     uint32_t base = 0;
@@ -247,33 +313,44 @@ __global__ void __launch_bounds__(kTraceBlock)
       HitState h;
       hit_init(h, ray.tmax);
       uint32_t visits = 0;
-      int skip = -1;
-      auto enter = [&](int k, int sp0) __attribute__((always_inline)) {
-        if ((uint32_t)k >= leaves || k == skip) return;
+      // A top-level leaf: one triangle of one instance, intersected in the instance's space.
+      auto cluster = [&](int k) __attribute__((always_inline)) {
+        if ((uint32_t)k >= leaves) return;
         const int2 sr = subroots[k];
         if (sr.x < 0) return;
         const TlasInst &ti = insts[sr.x];
-        const BVHNode *nodes = ti.nodes;
-        const uint32_t n = ti.n;
-        if (!(ti.flags & kInstEnabled) || !blas_live(pool, ti)) return;
+        if (kCheck && !blas_live(pool, ti)) return;
         if (ti.flags & kInstIdentity) {
-          visits += traverse<kMaxStack2>(rw, nodes, ti.tris, n, sr.y, h, st, sp0, (uint32_t)sr.x, k);
+          intersect_leaf(rw, ti.tris, sr.y, ti.n, (uint32_t)sr.x, k, h);
         } else {
           const RayPre ro = object_ray(ti, ray);
-          visits += traverse<kMaxStack2>(ro, nodes, ti.tris, n, sr.y, h, st, sp0, (uint32_t)sr.x, k);
+          intersect_leaf(ro, ti.tris, sr.y, ti.n, (uint32_t)sr.x, k, h);
         }
       };
+      // With a proposal: the proposed class (a subtree of one instance's own tree) first, whose hit
+      // tightens t; then the top-level walk, which finds that hit again and records its leaf.
       if (proposal) {
         const int32_t c = proposal[ri];
         if (c >= 0 && (uint32_t)c < numClasses) {
-          enter(c, 0);
-          skip = c;  // its subtree has been searched completely
+          const int2 cn = classNodes[c];
+          if (cn.x >= 0) {
+            const TlasInst &ti = insts[cn.x];
+            if ((ti.flags & kInstEnabled) && (!kCheck || blas_live(pool, ti))) {
+              if (ti.flags & kInstIdentity) {
+                visits += traverse<kMaxStack>(rw, ti.nodes, ti.tris, ti.n, cn.y, h, st, 0, (uint32_t)cn.x, -1);
+              } else {
+                const RayPre ro = object_ray(ti, ray);
+                visits += traverse<kMaxStack>(ro, ti.nodes, ti.tris, ti.n, cn.y, h, st, 0, (uint32_t)cn.x, -1);
+              }
+            }
+          }
         }
       }
-      visits += walk<kMaxStack2>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip, [&](int pos, int spTop) __attribute__((always_inline)) {
-        if ((uint32_t)pos >= leaves) return;
-        enter((int)__float_as_uint(topTris[pos].v0.w), spTop);
-      });
+      visits += walk<kMaxStack>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip,
+                                [&](int pos, int) __attribute__((always_inline)) {
+                                  if ((uint32_t)pos >= leaves) return;
+                                  cluster((int)__float_as_uint(topTris[pos].v0.w));
+                                });
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
       o.primID = h.prim;
@@ -393,8 +470,8 @@ __global__ void __launch_bounds__(256) k_bounce(const TlasInst *__restrict__ ins
   out[j] = o;
 }
 
-// Cut depth: the largest D <= 6 with instances * 2^D <= 4096 (0 beyond 4096 instances).
-uint32_t cut_depth(uint32_t ninst) {
+// Proposal class depth: the largest Dc <= 6 with instances * 2^Dc <= 4096.
+uint32_t class_depth(uint32_t ninst) {
   uint32_t d = 0;
   while (d < 6 && ((uint64_t)ninst << (d + 1)) <= 4096) ++d;
   return d;
@@ -448,6 +525,8 @@ bool blas_slot_acquire(Context *ctx, Accel *a) {
     fail("gprtTriangleAccelCreate: more than %u live triangle accels in one context", kBlasPoolSlots);
     return false;
   }
+  if (ctx->slotAccel.size() <= a->slot) ctx->slotAccel.resize((size_t)a->slot + 1, nullptr);
+  ctx->slotAccel[a->slot] = a;
   std::memset(&a->hdesc, 0, sizeof(a->hdesc));
   return true;
 }
@@ -459,7 +538,9 @@ void blas_slot_release(Accel *a) {
          "hipMemcpyAsync(descriptor)");
   hipStreamSynchronize(ctx->stream);
   ctx->blasFree.push_back(a->slot);
+  if (a->slot < ctx->slotAccel.size()) ctx->slotAccel[a->slot] = nullptr;
   a->slot = 0xFFFFFFFFu;
+  ++ctx->blasEpoch;
 }
 
 bool blas_desc_refresh(Context *ctx, hipStream_t s, Accel *a) {
@@ -482,6 +563,7 @@ bool blas_desc_refresh(Context *ctx, hipStream_t s, Accel *a) {
   }
   if (std::memcmp(&d, &a->hdesc, sizeof(d)) == 0) return true;
   a->hdesc = d;
+  ++ctx->blasEpoch;  // instance accels built before this see a moved or retired triangle accel
   // Pageable source: the copy is staged before the call returns, so a->hdesc may change later.
   return hip_ok(hipMemcpyAsync(ctx->blasPool + a->slot, &a->hdesc, sizeof(BlasDesc), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync(descriptor)");
@@ -501,6 +583,8 @@ void tlas_release(Accel *a) {
 }
 
 uint32_t tlas_leaves(const Accel *a) { return a->tlas ? a->tlas->leaves : 0; }
+uint32_t tlas_classes(const Accel *a) { return a->tlas ? a->tlas->classes : 0; }
+const int32_t *tlas_leaf_class(const Accel *a) { return a->tlas ? a->tlas->leafClass : nullptr; }
 
 size_t tlas_size(const Accel *a) {
   const Tlas *t = a->tlas;
@@ -517,48 +601,109 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
     return false;
   }
   if (!pool_ensure(ctx)) return false;
-  if (!t->block) {
+  // Lay the leaves out from the records (read back: 64 bytes per instance): instance i owns leaves
+  // [first[i], first[i] + n_i) for the triangle accel its address names (0 if it names none).
+  t->hostRecords.resize((size_t)N * sizeof(InstanceRecord));
+  if (!hip_ok(hipMemcpyAsync(t->hostRecords.data(), t->instances->d, t->hostRecords.size(), hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(instance records)") ||
+      !hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize(instance records)"))
+    return false;
+  const uint64_t base = (uint64_t)(uintptr_t)ctx->blasPool;
+  t->hostFirst.assign((size_t)N + 1, 0u);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    InstanceRecord rec;
+    std::memcpy(&rec, t->hostRecords.data() + (size_t)i * sizeof(InstanceRecord), sizeof(rec));
+    uint32_t n = 0;
+    if (rec.addr >= base && rec.addr < base + (uint64_t)kBlasPoolSlots * sizeof(BlasDesc) &&
+        (rec.addr - base) % sizeof(BlasDesc) == 0) {
+      const uint32_t slot = (uint32_t)((rec.addr - base) / sizeof(BlasDesc));
+      const Accel *b = slot < ctx->slotAccel.size() ? ctx->slotAccel[slot] : nullptr;
+      if (b && b->built && (rec.w0 >> 24) != 0) n = b->n;
+    }
+    t->hostFirst[i] = (uint32_t)total;
+    total += n;
+  }
+  if (total >= (1ull << 30)) {
+    fail("gprtAccelBuild: the instances hold %llu triangles; the limit is 2^30", (unsigned long long)total);
+    return false;
+  }
+  t->hostFirst[N] = (uint32_t)total;
+  const uint32_t L = total ? (uint32_t)total : 1u;  // an empty scene keeps one disabled leaf
+  if (!t->block || L > t->leafCap) {
+    if (t->block) {
+      hipStreamSynchronize(s);
+      hipFree(t->block);
+      t->block = nullptr;
+    }
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const uint32_t L = t->leaves;
-    const size_t oSub = al((size_t)N * sizeof(TlasInst));
-    const size_t oVerts = oSub + al((size_t)L * sizeof(int2));
-    const size_t oIdx = oVerts + al((size_t)L * 36);
-    t->blockBytes = oIdx + al((size_t)L * 12);
-    if (!hip_ok(hipMalloc(&t->block, t->blockBytes), "hipMalloc(instance accel)")) return false;
+    const uint32_t cap = L + L / 8;  // room for some growth without a new allocation
+    size_t off = al((size_t)N * sizeof(TlasInst));
+    const size_t oFirst = off;
+    off += al(((size_t)N + 1) * 4);
+    const size_t oSub = off;
+    off += al((size_t)cap * sizeof(int2));
+    const size_t oLeafCls = off;
+    off += al((size_t)cap * 4);
+    const size_t oVerts = off;
+    off += al((size_t)cap * 36);
+    const size_t oIdx = off;
+    off += al((size_t)cap * 12);
+    const size_t oCls = off;
+    off += al((size_t)t->classes * sizeof(int2));
+    if (!hip_ok(hipMalloc(&t->block, off), "hipMalloc(instance accel)")) return false;
+    t->blockBytes = off;
+    t->leafCap = cap;
     char *b = (char *)t->block;
     t->insts = (TlasInst *)b;
+    t->instFirst = (uint32_t *)(b + oFirst);
     t->subroots = (int2 *)(b + oSub);
+    t->leafClass = (int32_t *)(b + oLeafCls);
+    t->classNodes = (int2 *)(b + oCls);
     t->vbuf.ctx = t->ibuf.ctx = ctx;
     t->vbuf.d = b + oVerts;
-    t->vbuf.size = (size_t)L * 36;
     t->ibuf.d = b + oIdx;
-    t->ibuf.size = (size_t)L * 12;
     t->gtype = GeomType{ctx, GPRT_TRIANGLES, 0};
     t->geom.ctx = ctx;
     t->geom.type = &t->gtype;
     t->geom.vbuf = &t->vbuf;
-    t->geom.vcount = 3 * L;
     t->geom.vstride = 12;
     t->geom.voffset = 0;
     t->geom.ibuf = &t->ibuf;
-    t->geom.icount = L;
     t->geom.istride = 12;
     t->geom.ioffset = 0;
     t->top.ctx = ctx;
     t->top.geom = &t->geom;
     t->top.mode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
+    t->leaves = 0;
+  }
+  if (L != t->leaves) {
+    t->leaves = L;
+    t->vbuf.size = (size_t)L * 36;
+    t->ibuf.size = (size_t)L * 12;
+    t->geom.vcount = 3 * L;
+    t->geom.icount = L;
     launch(ctx, s, "tlas_index", k_tlas_index, dim3((3 * L + 255) / 256), dim3(256), 0, (uint32_t *)t->ibuf.d,
            3 * L);
   }
-  const uint32_t L = t->leaves;
-  launch(ctx, s, "tlas_cut", k_tlas_cut, dim3((L + 255) / 256), dim3(256), 0,
-         (const InstanceRecord *)t->instances->d, N, t->depth, (const BlasDesc *)ctx->blasPool, t->insts, t->subroots,
-         (float *)t->vbuf.d);
+  a->n = L;
+  // Pageable source: the copy is staged before the call returns.
+  if (!hip_ok(hipMemcpyAsync(t->instFirst, t->hostFirst.data(), ((size_t)N + 1) * 4, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(instance layout)"))
+    return false;
+  launch(ctx, s, "tlas_inst", k_tlas_inst, dim3((N + 255) / 256), dim3(256), 0,
+         (const InstanceRecord *)t->instances->d, N, t->classDepth, (const uint32_t *)t->instFirst,
+         (const BlasDesc *)ctx->blasPool, t->insts);
+  launch(ctx, s, "tlas_cut", k_tlas_cut, dim3((L + 255) / 256), dim3(256), 0, (const TlasInst *)t->insts, N,
+         (const uint32_t *)t->instFirst, L, t->classDepth, t->subroots, t->leafClass, (float *)t->vbuf.d);
+  launch(ctx, s, "tlas_classes", k_tlas_classes, dim3((t->classes + 255) / 256), dim3(256), 0,
+         (const TlasInst *)t->insts, N, t->classDepth, t->classNodes);
   Accel *top = &t->top;
   if (!lbvh_build_batch(ctx, s, &top, 1, true)) return false;
   a->nodes = top->nodes;
   a->tris = top->tris;
   a->header = top->header;
+  t->builtEpoch = ctx->blasEpoch;
   return true;
 }
 
@@ -566,15 +711,25 @@ bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *h
                 const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits) {
   if (count == 0) return true;
   Tlas *t = a->tlas;
-  uint32_t *work = ensure_trace_scratch(ctx, s, kMaxStack2 - kLdsStack);
+  uint32_t *work = ensure_trace_scratch(ctx, s, kMaxStack - kLdsStack);
   if (!work) return false;
   const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
   fill_u32(ctx, s, work, 0u, 1);
-  launch(ctx, s, proposal ? "trace_closest_proposal_instances" : "trace_closest_instances", k_trace_tlas,
-         dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes, (const TriRecord *)t->top.tris, t->leaves,
-         (const int2 *)t->subroots, (const TlasInst *)t->insts, (const BlasDesc *)ctx->blasPool,
-         (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal, numClasses, instOut, leafOut,
-         visits);
+  const uint32_t nc = proposal ? std::min(numClasses, t->classes) : 0u;
+  // The unchecked kernel trusts the instance table: no triangle accel of this context was
+  // destroyed or moved since the instance accel was built.
+  const bool checked = t->builtEpoch != ctx->blasEpoch;
+  const char *name = proposal ? "trace_closest_proposal_instances" : "trace_closest_instances";
+  if (checked)
+    launch(ctx, s, name, k_trace_tlas<true>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
+           (const TriRecord *)t->top.tris, t->leaves, (const int2 *)t->subroots, (const TlasInst *)t->insts,
+           (const BlasDesc *)ctx->blasPool, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
+           (const int2 *)t->classNodes, nc, instOut, leafOut, visits);
+  else
+    launch(ctx, s, name, k_trace_tlas<false>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
+           (const TriRecord *)t->top.tris, t->leaves, (const int2 *)t->subroots, (const TlasInst *)t->insts,
+           (const BlasDesc *)ctx->blasPool, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
+           (const int2 *)t->classNodes, nc, instOut, leafOut, visits);
   return true;
 }
 
@@ -608,10 +763,10 @@ GPRT_API GPRTAccel gprtInstanceAccelCreate(GPRTContext _ctx, uint32_t numInstanc
   Tlas *t = new Tlas();
   t->instances = inst;
   t->numInstances = numInstances;
-  t->depth = cut_depth(numInstances);
-  t->leaves = numInstances << t->depth;
+  t->classDepth = class_depth(numInstances);
+  t->classes = numInstances << t->classDepth;
   a->tlas = t;
-  a->n = t->leaves;
+  a->n = 0;
   return (GPRTAccel)a;
 }
 

@@ -69,6 +69,8 @@ struct Context {
   struct BlasDesc *blasPool = nullptr;
   std::vector<uint32_t> blasFree;
   uint32_t blasNext = 0;
+  uint64_t blasEpoch = 0;  // bumped whenever a descriptor is retired or rewritten
+  std::vector<struct Accel *> slotAccel;  // host view: pool slot -> live triangle accel
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -243,6 +245,9 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a);
 void tlas_release(Accel *a);
 size_t tlas_size(const Accel *a);
 uint32_t tlas_leaves(const Accel *a);
+// Proposal classes of an instance accel and the class of each top-level leaf (-1: disabled).
+uint32_t tlas_classes(const Accel *a);
+const int32_t *tlas_leaf_class(const Accel *a);
 bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
                 const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits);
 

@@ -553,6 +553,15 @@ __global__ void __launch_bounds__(256) k_labels(const GPRTHit *__restrict__ hits
   labels[i] = (p >= 0 && (uint32_t)p < nprims) ? primClass[p] : -1;
 }
 
+__global__ void __launch_bounds__(256) k_labels_of_leaves(const int32_t *__restrict__ leaves, uint32_t count,
+                                                          const int32_t *__restrict__ leafClass, uint32_t nleaves,
+                                                          int32_t *__restrict__ labels) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int32_t k = leaves[i];
+  labels[i] = (k >= 0 && (uint32_t)k < nleaves) ? leafClass[k] : -1;
+}
+
 unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void gemm(Context *ctx, hipStream_t s, const GemmArgs &g, const char *name) {
@@ -760,6 +769,12 @@ void mlp_adam(Context *ctx, hipStream_t s, Proposal *p, float lr) {
   launch(ctx, s, "mlp_adam", k_adam, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, p->w, (const float *)p->g, p->m,
          p->v, (uint32_t)kMlpParams, lr, b1, b2, eps, c1, c2, (const uint32_t *)p->bfMap, p->wbf);
   mlp_refresh_bias(ctx, s, p);
+}
+
+void mlp_labels_of_leaves(Context *ctx, hipStream_t s, Proposal *p, const int32_t *leaves, uint32_t n,
+                          int32_t *labels) {
+  launch(ctx, s, "mlp_labels", k_labels_of_leaves, dim3(blocks_for(n, 256)), dim3(256), 0, leaves, n,
+         tlas_leaf_class(p->accel), p->accel->n, labels);
 }
 
 void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint32_t n, int32_t *labels) {

@@ -47,6 +47,7 @@ struct Proposal {
   int32_t *classRoot = nullptr, *innerClass = nullptr, *leafClass = nullptr, *primClass = nullptr;
   uint32_t *numClassesDev = nullptr;
   Accel *accel = nullptr;
+  bool instanceBound = false;  // bound to an instance accel: classes of its top-level leaves
   // Per-ray scratch for proposal-first traces.
   void *rayScratch = nullptr;
   size_t rayScratchBytes = 0;
@@ -61,6 +62,9 @@ void mlp_forward_batch(Context *ctx, hipStream_t s, Proposal *p, const void *ray
 void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, uint32_t labelled);
 void mlp_adam(Context *ctx, hipStream_t s, Proposal *p, float lr);
 void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint32_t n, int32_t *labels);
+// Classes of top-level leaves (instance accels): labels[i] = leafClass[leaves[i]], or -1.
+void mlp_labels_of_leaves(Context *ctx, hipStream_t s, Proposal *p, const int32_t *leaves, uint32_t n,
+                          int32_t *labels);
 void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit);
 bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth);
 

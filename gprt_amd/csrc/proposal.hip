@@ -129,23 +129,20 @@ GPRT_API uint32_t gprtProposalBindAccel(GPRTContext _ctx, GPRTProposal _p, GPRTA
     fail("gprtProposalBindAccel: accel has not been built");
     return 0;
   }
-  if (a->kind == AccelKind::Instances) {
-    const uint32_t leaves = tlas_leaves(a);
-    if (leaves > (uint32_t)kMlpOut) {
-      fail("gprtProposalBindAccel: the instance accel has %u top-level leaves; the proposal has %d classes", leaves,
-           kMlpOut);
-      return 0;
-    }
-    p->accel = a;
-    p->numClasses = leaves;
-    p->treePrims = leaves;
-    p->primClass = nullptr;
-    return leaves;
-  }
   if (cutDepth > (uint32_t)kMlpCutDepth) {
     fail("gprtProposalBindAccel: cutDepth %u exceeds %d (at most 4096 treelets)", cutDepth, kMlpCutDepth);
     return 0;
   }
+  if (a->kind == AccelKind::Instances) {
+    // Classes are fixed groups of each instance's triangles (stable across rebuilds); cutDepth is
+    // unused.
+    p->accel = a;
+    p->instanceBound = true;
+    p->numClasses = tlas_classes(a);
+    p->treePrims = a->n;
+    return p->numClasses;
+  }
+  p->instanceBound = false;
   if (!mlp_treelets(ctx, ctx->stream, p, a, (int)cutDepth)) return 0;
   return p->numClasses;
 }
@@ -246,6 +243,8 @@ float train_core(Context *ctx, Proposal *p, const void *rays, const void *hits, 
   const unsigned g = (count + 255) / 256;
   if (hits)
     mlp_labels(ctx, s, p, hits, count, labels);
+  else if (p->instanceBound)
+    mlp_labels_of_leaves(ctx, s, p, userLabels, count, labels);  // top-level leaf -> its treelet
   else
     hip_ok(hipMemcpyAsync(labels, userLabels, (size_t)count * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
   launch(ctx, s, "mlp_label_flags", k_label_flags, dim3(g), dim3(256), 0, (const int32_t *)labels, count,
@@ -296,9 +295,9 @@ GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _
     fail("gprtProposalTrain: bind the proposal to an accel first (gprtProposalBindAccel)");
     return NAN;
   }
-  if (!p->primClass) {
-    fail("gprtProposalTrain: hits of an instance accel carry no class; use gprtProposalTrainLabels with the "
-         "leafIDs of gprtAccelTraceClosestHitDetail");
+  if (p->instanceBound) {
+    fail("gprtProposalTrain: hits of an instance accel do not name their top-level leaf; use "
+         "gprtProposalTrainLabels with the leafIDs of gprtAccelTraceClosestHitDetail");
     return NAN;
   }
   if ((size_t)count * sizeof(GPRTRay) > rays->size || (size_t)count * sizeof(GPRTHit) > hits->size) {
@@ -352,10 +351,12 @@ GPRT_API void gprtProposalExportTreelets(GPRTContext _ctx, GPRTProposal _p, GPRT
     fail("gprtProposalExportTreelets: null argument or unbound proposal");
     return;
   }
-  if (!p->primClass) {
-    fail("gprtProposalExportTreelets: the proposal is bound to an instance accel (classes are its top-level leaves)");
+  if (p->instanceBound) {
+    fail("gprtProposalExportTreelets: bound to an instance accel, whose classes are groups of its instances' "
+         "triangles");
     return;
   }
+
   if (cr->size < (size_t)p->numClasses * 4 && !buffer_realloc(cr, (size_t)p->numClasses * 4, false)) return;
   if (pc->size < (size_t)p->treePrims * 4 && !buffer_realloc(pc, (size_t)p->treePrims * 4, false)) return;
   hip_ok(hipMemcpyAsync(cr->d, p->classRoot, (size_t)p->numClasses * 4, hipMemcpyDeviceToDevice, ctx->stream),

@@ -62,6 +62,8 @@ __device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord 
       h.v = v;
       h.inst = inst;
       h.leaf = tag;
+    } else if (t == h.t && inst == h.inst && prim == h.prim && h.leaf < 0) {
+      h.leaf = tag;  // the same hit, first found by a walk that did not know its top-level leaf
     }
   }
 }
