@@ -47,6 +47,16 @@ N_TRIS = 4_194_304
 IMG = 2048
 N_BOUNCE = 4_194_304
 LBVH_BYTES_PER_PRIM = 368
+# HBM-side bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
+# (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
+# The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
+TRAFFIC_BYTES = {
+    "trace_closest": 11929.0e6 + 69.8e6,
+    "mlp_infer_top1": 278.5e6 + 33.6e6,
+    "lbvh_sort_pass": 40.0e6 + 42.6e6,
+    "build": (48.0 + 48.0 + 8.8 + 4 * (9.5 + 10.2) + 27.0 + 24.3 + 272.6 + 113.2 + 3.8 + 1.1) * N_TRIS,
+    "batched_build_per_prim": 48.0 + 48.1 + 9.3 + 4 * (9.1 + 10.2) + 27.2 + 24.4 + 177.5 + 113.0 + 4.4 + 1.2,
+}
 
 
 def log(msg):
@@ -157,20 +167,25 @@ def scene_workload(args):
     sw_ms, sw_n = ms_of(per_kernel, "lbvh_sort_pass")
     sw_avg = sw_ms / max(sw_n, 1)
     roof_sort = {"kernel": "lbvh_sort_pass", "bound": "hbm", "achieved": N_TRIS * 16 / (sw_avg * 1e-3) / 1e9,
-                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "avg_launch_ms": sw_avg,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": TRAFFIC_BYTES["lbvh_sort_pass"],
+                 "avg_launch_ms": sw_avg,
                  "bytes_per_launch": N_TRIS * 16}
     roof_sort["frac"] = roof_sort["achieved"] / HBM_PEAK_GBS
-    roof_build = {"bound": "hbm", "achieved": N_TRIS * LBVH_BYTES_PER_PRIM / (build_ms * 1e-3) / 1e9,
-                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": LBVH_BYTES_PER_PRIM}
+    roof_build = {"kernel": "lbvh build (all build kernels)", "bound": "hbm",
+                  "achieved": N_TRIS * LBVH_BYTES_PER_PRIM / (build_ms * 1e-3) / 1e9,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": LBVH_BYTES_PER_PRIM,
+                  "traffic": TRAFFIC_BYTES["build"]}
     roof_build["frac"] = roof_build["achieved"] / HBM_PEAK_GBS
     infer_avg = infer_ms_tot / max(infer_n, 1)
     roof_mlp = {"kernel": "mlp_infer_top1", "bound": "mfma",
                 "achieved": n_rays * MLP_FLOP_PER_RAY / (infer_avg * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "avg_launch_ms": infer_avg}
+                "unit": "TFLOP/s", "avg_launch_ms": infer_avg, "flop_per_ray": MLP_FLOP_PER_RAY,
+                "traffic": TRAFFIC_BYTES["mlp_infer_top1"]}
     roof_mlp["frac"] = roof_mlp["achieved"] / BF16_PEAK_TFLOPS
 
     cpu = None
     roof_trace = None
+    dominant = roof_mlp if infer_ms >= plain_ms else None
     if not args.no_cpu:
         from oracle import binding as orc
         keys, nodes, aabbs = acc.export()
@@ -194,12 +209,18 @@ def scene_workload(args):
                "trace_mismatches_vs_gpu": mism}
         trace_avg = plain_ms
         ach = n_rays * trace_bytes / (trace_avg * 1e-3) / 1e9
+        # Algorithmic bytes exceed what HBM delivers: most node fetches hit in L2 / MALL, so the HBM
+        # roofline does not bind traversal (measured HBM-side traffic is `traffic`).
         roof_trace = {"kernel": "trace_closest", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": trace_avg,
+                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["trace_closest"],
+                      "traffic_gbs": TRAFFIC_BYTES["trace_closest"] / (trace_avg * 1e-3) / 1e9,
+                      "avg_launch_ms": trace_avg,
                       "bytes_per_ray": trace_bytes, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray}
         log(f"cpu: build {cpu['value']:.3f} Mprims/s, trace {cpu['trace_mrays_s']:.4f} Mrays/s; "
             f"{nodes_per_ray:.1f} nodes + {tris_per_ray:.1f} tris per ray")
 
+    if dominant is None:
+        dominant = roof_trace if roof_trace else roof_build
     out = {
         "metric": "LBVH build Mprims/s + closest-hit Mrays/s, plain vs learned-proposal, 1 GPU",
         "value": N_TRIS / (build_ms * 1e3),
@@ -224,7 +245,8 @@ def scene_workload(args):
                        "learned_vs_plain": plain_ms / learned_ms, "treelet_cut_ms": cut_ms,
                        "hit_mismatches_learned_vs_plain": mismatches, "hit_rate": hit_rate,
                        "train_s": train_s, "train_loss": loss, "train_top1": train_acc},
-        "roofline": roof_trace if roof_trace else roof_sort,
+        # The roofline of the kernel that takes the most time in the step.
+        "roofline": dominant,
         "roofline_build": roof_build,
         "roofline_sort_pass": roof_sort,
         "roofline_mlp": roof_mlp,
@@ -316,7 +338,8 @@ def batched_workload(args):
                        "autotune_best": tune["best"],
                        "autotune_best_index": int(best)},
         "roofline": {"kernel": "lbvh batch graph (all build kernels)", "bound": "hbm", "achieved": ach,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES["batched_build_per_prim"] * total,
                      "bytes_per_prim": LBVH_BYTES_PER_PRIM},
         "autotune": tune,
         "cpu_baseline": None,
