@@ -282,34 +282,26 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ w, co
 }
 
 // ------------------------------------------------------------------------------------------------
-// Fused inference: encode -> 3 hidden layers -> 4096 logits with a running argmax, never
-// writing the logits. A workgroup of 4 waves handles 256 rays (64 per wave, 4 row blocks of 16);
-// hidden weights and 64-column chunks of W4t are staged in LDS and shared by the 4 waves, so
-// each chunk read from L2 serves 256 rays.
+// Inference in two kernels. k_infer_hidden: one wave per 64 rays encodes them and runs the three
+// 128x128 hidden layers (activations in 17 KB of LDS, weight fragments read from L2, so eight
+// waves share a CU), writing the bf16 layer-3 activations (256 B per ray). k_infer_top1: the
+// 128x4096 output layer with a running argmax over 256 rays per workgroup, never writing the
+// logits; 80 KB of LDS and at most 256 registers let two workgroups (two waves per SIMD) share a
+// CU and hide each other's stalls.
 // ------------------------------------------------------------------------------------------------
 constexpr int kInferRaysPerWave = 64;
 constexpr int kInferRays = 4 * kInferRaysPerWave;
-constexpr int kChunk = 64;     // W4t rows (output classes) per staged chunk
 constexpr int kActStride = 136;  // bf16 per activation row in LDS (128 + 8 pad)
-constexpr int kWStride = 136;
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_infer_top1(const GPRTRay *__restrict__ rays, uint32_t count,
-                                                    const uint16_t *__restrict__ W1t,
-                                                    const uint16_t *__restrict__ W2t,
-                                                    const uint16_t *__restrict__ W3t,
-                                                    const uint16_t *__restrict__ W4t, const float *__restrict__ bias,
-                                                    int32_t *__restrict__ outClass, float *__restrict__ outLogit,
-                                                    uint32_t numClasses) {
-  extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
-  uint16_t *act = smem;                                   // [256 rays][kActStride]
-  uint16_t *wbuf = smem + kInferRays * kActStride;        // [128 rows][kWStride]
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  const uint32_t ray0 = blockIdx.x * kInferRays;
-  uint16_t *myAct = act + (size_t)w * kInferRaysPerWave * kActStride;
-
-  // Encode: one ray per lane.
+__global__ void __launch_bounds__(64) k_infer_hidden(const GPRTRay *__restrict__ rays, uint32_t count,
+                                                     const uint16_t *__restrict__ W1t,
+                                                     const uint16_t *__restrict__ W2t,
+                                                     const uint16_t *__restrict__ W3t,
+                                                     const float *__restrict__ bias, uint16_t *__restrict__ h3) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[kInferRaysPerWave * kActStride];
+  const int l = threadIdx.x;
+  const uint32_t ri = blockIdx.x * kInferRaysPerWave + l;
   {
-    const uint32_t ri = ray0 + w * kInferRaysPerWave + l;
     uint32_t wds[kMlpInPad / 2];
     if (ri < count) {
       encode_ray(rays[ri], wds);
@@ -317,83 +309,88 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 #pragma unroll
       for (int i = 0; i < kMlpInPad / 2; ++i) wds[i] = 0u;
     }
-    store_features(wds, &myAct[l * kActStride]);
+    store_features(wds, &act[l * kActStride]);
   }
-
-  // Hidden layers.
+  __builtin_amdgcn_wave_barrier();
   const uint16_t *Ws[3] = {W1t, W2t, W3t};
   for (int layer = 0; layer < 3; ++layer) {
-    __syncthreads();  // previous users of wbuf are done
-    // Stage Wt[128][128] -> wbuf (each thread copies 64 bf16).
-    for (int i = tid; i < 128 * 16; i += 256) {
-      const int row = i >> 4, q = i & 15;
-      *(uint4 *)&wbuf[row * kWStride + q * 8] = *(const uint4 *)&Ws[layer][row * 128 + q * 8];
-    }
-    __syncthreads();
+    const uint16_t *W = Ws[layer];
     const float *b = bias + layer * kMlpHid;
-    f32x4_t acc[4][8];
+    // The wave's whole 64 x 128 input tile goes to registers first, so each 16-column block of
+    // outputs can be written back over it as soon as it is done (16 accumulators live at a time).
+    Frag a[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < 4; ++ks)
+        a[i][ks].u = *(const uint4 *)&act[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll 2
+    for (int j = 0; j < 8; ++j) {
+      f32x4_t acc[4];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      Frag a[4];
+      for (int i = 0; i < 4; ++i) acc[i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        Frag bf;
+        bf.u = *(const uint4 *)&W[(j * 16 + (l & 15)) * 128 + ks * 32 + 8 * (l >> 4)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = mfma16(a[i][ks], bf, acc[i]);
+      }
+      const int col = j * 16 + (l & 15);
+      const float bc = b[col];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        a[i].u = *(const uint4 *)&myAct[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        Frag bf;
-        bf.u = *(const uint4 *)&wbuf[(j * 16 + (l & 15)) * kWStride + ks * 32 + 8 * (l >> 4)];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i][j] = mfma16(a[i], bf, acc[i][j]);
-      }
-    }
-    // All lanes of this wave have read their activations; overwrite them with the outputs.
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int row = i * 16 + 4 * (l >> 4) + q, col = j * 16 + (l & 15);
-          myAct[row * kActStride + col] = f2bf(fmaxf(acc[i][j][q] + b[col], 0.f));
+          const int row = i * 16 + 4 * (l >> 4) + q;
+          act[row * kActStride + col] = __builtin_bit_cast(uint16_t, (__bf16)fmaxf(acc[i][q] + bc, 0.f));
         }
+    }
     __builtin_amdgcn_wave_barrier();
   }
+  if (ri < count) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) *(uint4 *)&h3[(size_t)ri * 128 + q * 8] = *(const uint4 *)&act[l * kActStride + q * 8];
+  }
+}
 
-  // Output layer with running argmax. A fragments (h3) stay in registers.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_infer_top1(uint32_t count, const uint16_t *__restrict__ W4t, const float *__restrict__ bias,
+                 const uint16_t *__restrict__ h3, int32_t *__restrict__ outClass, float *__restrict__ outLogit,
+                 uint32_t numClasses) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const uint32_t ray0 = blockIdx.x * kInferRays;
+  // A fragments (layer-3 activations of this wave's 64 rays) stay in registers.
   Frag a[4][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t ri = ray0 + w * kInferRaysPerWave + i * 16 + (l & 15);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
-      a[i][ks].u = *(const uint4 *)&myAct[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
+      a[i][ks].u = ri < count ? *(const uint4 *)&h3[(size_t)ri * 128 + ks * 32 + 8 * (l >> 4)]
+                              : make_uint4(0u, 0u, 0u, 0u);
+  }
   float best[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int q = 0; q < 4; ++q) best[i][q] = -FLT_MAX;
-  // W4t streams through a two-chunk ring in the activation region (free once every wave holds
-  // its A fragments) by direct global-to-LDS loads: chunk c + 1 is in flight while chunk c is
+  // W4t streams through a two-chunk LDS ring by direct global-to-LDS loads: chunk c + 1 is in flight while chunk c is
   // multiplied, and the copies use no registers. A chunk is 128 rows x 256 bytes, unpadded, with
   // the 16-byte units of row r stored at position q ^ (r & 15), so the 16 rows read by one
-  // B-fragment load fall into distinct units. The output bias is staged in LDS (the weight
-  // region), so the multiply loop issues no vector-memory loads of its own and the explicit
-  // vmcnt waits below only count the ring copies.
+  // B-fragment load fall into distinct units. The output bias is staged in LDS after the ring,
+  // so the multiply loop issues no vector-memory loads of its own and the explicit vmcnt waits
+  // below only count the ring copies.
   constexpr int kOutChunk = 128;
   constexpr int kChunks = kMlpOut / kOutChunk;
   constexpr int kChunkBytes = kOutChunk * 256;
-  static_assert(2 * kChunkBytes <= kInferRays * kActStride * 2, "ring fits the activation region");
-  static_assert(kMlpOut * 4 <= 128 * kWStride * 2, "bias fits the weight region");
   // Classes beyond numClasses get a bias of -FLT_MAX, so they never win and the multiply loop needs
   // no per-element check (a finite value: -inf would encode to a signalling NaN below, which
   // v_max_f32 turns into a quiet NaN that poisons the running maximum); the accumulators start at
   // the bias, so no add is needed either.
-  float *sBias = (float *)wbuf;
-  __syncthreads();  // every wave holds its A fragments; the activation region becomes the ring
+  float *sBias = (float *)((char *)smem + 2 * 128 * 256);
   for (int i = tid; i < kMlpOut; i += 256) sBias[i] = (uint32_t)i < numClasses ? bias[3 * kMlpHid + i] : -FLT_MAX;
   const uint32_t ringBase = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void *)smem;
   // Wave w copies 1 KB blocks 8w .. 8w + 7 of a chunk (4 rows each).
@@ -472,8 +469,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         outClass[ri] = c;
         if (outLogit) outLogit[ri] = v;
       }
-    }
-}
+    }}
 
 // ------------------------------------------------------------------------------------------------
 // Treelets
@@ -658,7 +654,9 @@ bool mlp_alloc(Context *ctx, Proposal *p, uint32_t maxBatch) {
 void mlp_release(Proposal *p) {
   if (p->block) hipFree(p->block);
   if (p->treeBlock) hipFree(p->treeBlock);
-  p->block = p->treeBlock = nullptr;
+  if (p->h3) hipFree(p->h3);
+  p->block = p->treeBlock = p->h3 = nullptr;
+  p->h3Bytes = 0;
 }
 
 // Copies fp32 parameters to the device and refreshes the bf16 weights and the bias vector.
@@ -783,10 +781,30 @@ void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint
 }
 
 void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit) {
-  const size_t lds = ((size_t)kInferRays * kActStride + (size_t)128 * kWStride) * 2;
-  launch(ctx, s, "mlp_infer_top1", k_infer_top1, dim3(blocks_for(n, kInferRays)), dim3(256), lds,
-         (const GPRTRay *)rays, n, mlp_wbf(p, 0), mlp_wbf(p, 1), mlp_wbf(p, 2), mlp_wbf(p, 3),
-         (const float *)p->bias, cls, logit, p->numClasses ? p->numClasses : (uint32_t)kMlpOut);
+  const uint32_t classes = p->numClasses ? p->numClasses : (uint32_t)kMlpOut;
+  // Two kernels through a bf16 activation buffer of at most 2^22 rays (1 GiB) per round.
+  const uint32_t chunk = std::min<uint32_t>(n, 1u << 22);
+  const size_t need = (size_t)chunk * 256;
+  if (need > p->h3Bytes) {
+    if (p->h3) {
+      hipStreamSynchronize(s);
+      hipFree(p->h3);
+      p->h3 = nullptr;
+      p->h3Bytes = 0;
+    }
+    if (!hip_ok(hipMalloc(&p->h3, need), "hipMalloc(proposal activations)")) return;
+    p->h3Bytes = need;
+  }
+  const size_t ldsOut = 2 * 128 * 256 + (size_t)kMlpOut * 4;  // W4t ring + output bias
+  for (uint32_t r0 = 0; r0 < n; r0 += chunk) {
+    const uint32_t m = std::min(chunk, n - r0);
+    launch(ctx, s, "mlp_infer_hidden", k_infer_hidden, dim3(blocks_for(m, kInferRaysPerWave)), dim3(64), 0,
+           (const GPRTRay *)rays + r0, m, mlp_wbf(p, 0), mlp_wbf(p, 1), mlp_wbf(p, 2), (const float *)p->bias,
+           (uint16_t *)p->h3);
+    launch(ctx, s, "mlp_infer_top1", k_infer_top1, dim3(blocks_for(m, kInferRays)), dim3(256), ldsOut, m,
+           mlp_wbf(p, 3), (const float *)p->bias, (const uint16_t *)p->h3, cls + r0, logit ? logit + r0 : nullptr,
+           classes);
+  }
 }
 
 bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth) {

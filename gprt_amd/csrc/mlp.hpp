@@ -48,6 +48,9 @@ struct Proposal {
   uint32_t *numClassesDev = nullptr;
   Accel *accel = nullptr;
   bool instanceBound = false;  // bound to an instance accel: classes of its top-level leaves
+  // bf16 layer-3 activations between the two inference kernels.
+  void *h3 = nullptr;
+  size_t h3Bytes = 0;
   // Per-ray scratch for proposal-first traces.
   void *rayScratch = nullptr;
   size_t rayScratchBytes = 0;
