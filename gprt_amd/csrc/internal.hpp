@@ -135,17 +135,17 @@ struct BVHStore {
   uint32_t *codes = nullptr, *ids = nullptr, *codesTmp = nullptr, *idsTmp = nullptr;
   void *nodes = nullptr;  // BVHNode[inner]
   void *tris = nullptr;   // TriRecord[prims]
-  uint32_t *parentLeaf = nullptr, *parentInner = nullptr;
+  uint32_t *parentLeaf = nullptr, *parentInner = nullptr;  // filled on demand by lbvh_parents
   float *headers = nullptr;  // [meshes][16]
-  uint32_t *entries = nullptr;  // [prims] per-mesh lists of refit edges into nodes spanning sub-tiles
-  void *topo = nullptr;         // int4[inner]: {child0, child1, other end of the range, 0}
-  uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into nodes spanning regions
+  uint32_t *entries = nullptr;  // [prims] per-region lists of build edges into splits on sub-tile edges
+  void *pending = nullptr;      // [inner][2] x 32 B: child records waiting at a split (box, ref, far end)
+  uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into splits on region edges
   uint32_t *edgeCount = nullptr;    // [meshes] (in the zeroed range)
   uint32_t *regionCount = nullptr;  // [tiles] (in the zeroed range)
   // Zeroed at the start of every build (one contiguous range):
   uint32_t *zeroBase = nullptr;
   size_t zeroWords = 0;
-  uint32_t *counters = nullptr;  // [inner]
+  uint32_t *counters = nullptr;  // [inner] child arrivals per split
   uint32_t *hist = nullptr;      // [meshes][passes][R]
   uint32_t *status = nullptr;    // [passes][tiles][R]
   uint32_t *tickets = nullptr;   // [passes]
@@ -259,6 +259,8 @@ void accel_detach(Accel *a);
 bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph);
 bool lbvh_refit(Context *ctx, hipStream_t s, Accel *a);
 bool lbvh_export(Context *ctx, hipStream_t s, Accel *a, int32_t *nodes, float *aabbs, uint64_t *keys);
+// Fills the store's parent links of one built triangle accel from its nodes.
+void lbvh_parents(Context *ctx, hipStream_t s, Accel *a);
 
 // Trace
 bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count);

@@ -17,12 +17,10 @@
 //  * The sort is a segmented one-sweep radix sort: per-mesh digit histograms are built by the
 //    key kernel, tiles never cross a mesh, the decoupled look-back stops at the mesh's first tile,
 //    and a mesh that fits in one tile is sorted entirely in LDS with no look-back at all.
-//  * Karras: delta(i,j) = clz64(key_i ^ key_j), -1 outside the mesh (hlsl:45-59); the split
-//    search is the binary search of hlsl:84-97 without its trailing t=1 iterations (they cannot
-//    move s once the search has converged).
-//  * Refit: one thread per leaf climbs while it is the second child to arrive at a node (atomic
-//    arrival counter, acquire/release at device scope): every node is combined exactly once,
-//    O(N), versus every leaf doing six CAS atomics on every ancestor (hlsl:400-428).
+//  * Hierarchy: the Karras radix tree (hlsl:287-338) with the same node numbering, built bottom-up
+//    together with the boxes in one pass (agglomerative construction): every node is combined
+//    exactly once when its second child arrives, O(N), with no split search and no topology
+//    pass, versus every leaf doing six CAS atomics on every ancestor (hlsl:400-428).
 #include <algorithm>
 #include <cstring>
 
@@ -40,6 +38,11 @@ constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr int kMortonBits = 30;
 constexpr int kThreads = 256;  // non-sort kernels
 
+struct Pending {  // a child waiting at its parent's split: box[0..3], {box[4], box[5], ref, far end}
+  float4 a, b;
+};
+static_assert(sizeof(Pending) == 32, "Pending must be 32 bytes");
+
 struct BuildArgs {
   const MeshDesc *desc;
   const uint32_t *tileMesh;
@@ -51,11 +54,12 @@ struct BuildArgs {
   uint32_t *parentLeaf, *parentInner, *counters;
   float *headers;
   uint32_t *hist, *status, *tickets;
-  int4 *topo;            // per inner node {child0, child1, other end of the leaf range, 0}
-  // Refit edges (links into nodes that a refit level cannot finish), at the mesh's primOffset:
-  // entries: from sub-tiles into nodes spanning sub-tiles, one list per 65536-key region (count
-  // at regionCount[first tile of the region]); entriesTop: from regions into nodes spanning
-  // regions, one list per mesh (count at edgeCount[mesh]).
+  Pending *pending;      // [inner][2]: child records waiting at a split, per side
+  // Build edges (split << 1 | side of a child whose parent a level cannot finish), at the mesh's
+  // primOffset: entries: from sub-tiles into splits on sub-tile edges, one list per 65536-key
+  // region (count at regionCount[first tile of the region]); entriesTop: from regions into
+  // splits on region edges, one list per mesh (count at edgeCount[mesh]). counters: arrivals
+  // per split.
   uint32_t *entries, *entriesTop;
   uint32_t *regionCount;
   uint32_t *edgeCount;
@@ -248,7 +252,8 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
   constexpr int W = THREADS / 64;
   constexpr int TILE = THREADS * ITEMS;
   constexpr int DPT = (R + THREADS - 1) / THREADS;
-  __shared__ __attribute__((aligned(16))) uint32_t s_buf[TILE];
+  __shared__ __attribute__((aligned(16))) uint32_t s_buf[TILE];   // keys in tile order
+  __shared__ __attribute__((aligned(16))) uint32_t s_vbuf[TILE];  // their payloads
   __shared__ uint32_t s_whist[W][R];
   __shared__ uint32_t s_tdigit[R];
   __shared__ uint32_t s_gbase[R];
@@ -279,26 +284,30 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
   const uint32_t segEnd = d.primOffset + d.ntri;
   const bool single = d.ntiles == 1;
 
-  uint32_t k[ITEMS], dg[ITEMS], pos[ITEMS];
+  // Keys and payloads are loaded together up front, so the payload loads overlap the ranking and
+  // the look-back; digits are recomputed from the keys (fewer live registers).
+  auto digit = [&](uint32_t x) { return (x >> shift) & dmask; };
+  uint32_t k[ITEMS], v[ITEMS], pos[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const uint32_t q = (uint32_t)(w * ITEMS * 64 + j * 64 + l);
     k[j] = q < valid ? src[base + q] : 0xFFFFFFFFu;
-    dg[j] = (k[j] >> shift) & dmask;
+    v[j] = q < valid ? vsrc[base + q] : 0u;
   }
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t dj = digit(k[j]);
     uint64_t mm = ~0ull;
 #pragma unroll
     for (int b = 0; b < BITS; ++b) {
-      const bool bit = (dg[j] >> b) & 1u;
+      const bool bit = (dj >> b) & 1u;
       const uint64_t bb = wave_ballot(bit);
       mm &= bit ? bb : ~bb;
     }
     const uint32_t rnk = mask_rank(mm);
     const uint32_t c = (uint32_t)__popcll(mm);
-    const uint32_t pre = s_whist[w][dg[j]];
-    if (rnk == 0) s_whist[w][dg[j]] = pre + c;
+    const uint32_t pre = s_whist[w][dj];
+    if (rnk == 0) s_whist[w][dj] = pre + c;
     pos[j] = pre + rnk;
   }
   __syncthreads();
@@ -370,35 +379,24 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
 
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    pos[j] = s_tdigit[dg[j]] + s_whist[w][dg[j]] + pos[j];
+    const uint32_t dj = digit(k[j]);
+    pos[j] = s_tdigit[dj] + s_whist[w][dj] + pos[j];
     s_buf[pos[j]] = k[j];
+    s_vbuf[pos[j]] = v[j];
   }
   __syncthreads();
-  uint32_t gaddr[ITEMS];
+  // Runs of equal digits are consecutive in the tile order, so the writes coalesce per run.
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const uint32_t i = (uint32_t)(j * THREADS + tid);
-    gaddr[j] = 0xFFFFFFFFu;
     if (i < valid) {
       const uint32_t key = s_buf[i];
-      const uint32_t g = s_gbase[(key >> shift) & dmask] + i;
+      const uint32_t g = s_gbase[digit(key)] + i;
       if (g >= d.primOffset && g < segEnd) {  // guards the write; always true when consistent
         dst[g] = key;
-        gaddr[j] = g;
+        vdst[g] = s_vbuf[i];
       }
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const uint32_t q = (uint32_t)(w * ITEMS * 64 + j * 64 + l);
-    s_buf[pos[j]] = q < valid ? vsrc[base + q] : 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const uint32_t i = (uint32_t)(j * THREADS + tid);
-    if (i < valid && gaddr[j] != 0xFFFFFFFFu) vdst[gaddr[j]] = s_buf[i];
   }
 }
 
@@ -412,90 +410,64 @@ __global__ void __launch_bounds__(256) k_copy_back(BuildArgs a, uint32_t prims) 
 }
 
 // ------------------------------------------------------------------------------------------------
-// Karras 2012 split search (gprt_lbvh.hlsl:64-103, :287-338). One thread per inner node.
+// Hierarchy and boxes in one bottom-up pass (agglomerative LBVH construction, Apetrei 2014).
+//
+// delta(i) = clz64(key_i ^ key_{i+1}) over adjacent sorted keys, -1 past either end of the mesh
+// (the delta of gprt_lbvh.hlsl:45-59 restricted to neighbours). The node covering leaves [l, r]
+// has as parent the node split at r when delta(r) > delta(l-1) (it is then the left child), else
+// the node split at l-1 (right child); the two never tie, since the keys are distinct. This is
+// the binary radix tree of the Karras split search (gprt_lbvh.hlsl:64-103, :287-338), built
+// without the searches, and the Karras index of a node follows from its side: r for a left
+// child, l for a right child, 0 for the root. Both children meet at their parent's split
+// position: the first to arrive leaves its record there and stops, the second merges both boxes,
+// writes the finished 64-byte node at its Karras index and climbs on (the atomic refit of
+// gprt_lbvh.hlsl:400-428, with every node combined exactly once).
+//
+// Three levels, so that no arrival counter needs ordering across XCDs:
+//  1. k_build_sub: one 1024-thread workgroup per 1024-key sub-tile, one leaf per thread. The
+//     sub-tile's deltas, leaf boxes, finished node boxes and the meeting slots of its interior
+//     splits live in LDS, ordered by LDS-only fences. A child whose parent is split on the
+//     sub-tile's edge leaves its record in `pending` and becomes an edge of its 65536-key region;
+//     a meeting slot still half-filled at the end (the sibling spans sub-tiles) is copied to
+//     `pending` with its split's counter set to 1.
+//  2. k_build_region: one workgroup per region climbs the region's edges through global counters
+//     ordered at workgroup scope (the workgroup's waves share one CU); climbs that leave the
+//     region become edges of the mesh.
+//  3. k_build_top: one workgroup per mesh finishes the nodes spanning regions.
+// Kernel boundaries order the levels.
 // ------------------------------------------------------------------------------------------------
-// One node per thread over 1024-key sub-tiles; the sub-tile's keys and 1024 keys on each side
-// are staged in LDS as 64-bit keys, so nearly every probe of the three searches is an LDS read
-// (a probe outside the staged window reads the key arrays).
-constexpr int kKarrasSub = 1024;
-constexpr int kKarrasHalo = 1024;
+constexpr int kBuildSub = 1024;
+constexpr uint32_t kBuildRegion = 65536;  // keys per level-2 region (a multiple of every tile size)
+constexpr int kBuildTop = 512;
+constexpr uint32_t kNoEdge = 0xFFFFFFFFu;
 
-__global__ void __launch_bounds__(kKarrasSub) k_karras(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
-  __shared__ uint64_t s_key[kKarrasSub + 2 * kKarrasHalo];
-  const uint32_t t = tileBase + blockIdx.x / subPerTile, sub = blockIdx.x % subPerTile;
-  const uint32_t m = a.tileMesh[t];
-  const MeshDesc d = a.desc[m];
-  const TileRange r = tile_range(a, d, t, m);
-  const uint32_t lo = r.lo + sub * kKarrasSub;
-  if (lo >= r.hi) return;  // uniform over the workgroup
-  const uint32_t hi = min(lo + (uint32_t)kKarrasSub, r.hi);
-  const int64_t N = d.ntri;
-  const uint32_t *__restrict__ codes = a.codes + d.primOffset;
-  const uint32_t *__restrict__ ids = a.ids + d.primOffset;
-  int4 *__restrict__ topo = a.topo + d.innerOffset;
-  uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
-  uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
-  const int64_t w0 = (int64_t)lo - kKarrasHalo;  // first staged position
-  for (int q = threadIdx.x; q < kKarrasSub + 2 * kKarrasHalo; q += kKarrasSub) {
-    const int64_t j = w0 + q;
-    s_key[q] = (j >= 0 && j < N) ? (((uint64_t)codes[j] << 32) | ids[j]) : 0ull;
-  }
-  __syncthreads();
-  const int64_t wEnd = min((int64_t)hi + kKarrasHalo, N);
-  auto key = [&](int64_t j) -> uint64_t {
-    return (j >= w0 && j < wEnd) ? s_key[j - w0] : (((uint64_t)codes[j] << 32) | ids[j]);
-  };
-  auto delta = [&](uint64_t ki, int64_t j) -> int {
-    if (j < 0 || j >= N) return -1;
-    return __clzll(ki ^ key(j));
-  };
-  const int64_t i = (int64_t)lo + threadIdx.x;
-  if (i >= (int64_t)hi || i >= N - 1) return;
-  const uint64_t ki = s_key[i - w0];
-  const int dir = delta(ki, i + 1) >= delta(ki, i - 1) ? 1 : -1;
-  const int dmin = delta(ki, i - dir);
-  int64_t lmax = 2;
-  while (delta(ki, i + lmax * dir) > dmin) lmax *= 2;
-  int64_t len = 0;
-  for (int64_t s = lmax >> 1; s >= 1; s >>= 1)
-    if (delta(ki, i + (len + s) * dir) > dmin) len += s;
-  const int64_t j = i + len * dir;
-  const int dnode = delta(ki, j);
-  int64_t s = 0, step = len;
-  do {
-    step = (step + 1) >> 1;
-    if (delta(ki, i + (s + step) * dir) > dnode) s += step;
-  } while (step > 1);
-  const int64_t split = i + s * dir + (dir < 0 ? -1 : 0);
-  const int64_t first = dir > 0 ? i : j;
-  const int64_t last = dir > 0 ? j : i;
-  if (split < first || split >= last) return;  // only possible with repeated keys
-  int c0, c1;
-  if (split == first) {
-    c0 = ~(int)split;
-    parentLeaf[split] = ((uint32_t)i << 1) | 0u;
-  } else {
-    c0 = (int)split;
-    parentInner[split] = ((uint32_t)i << 1) | 0u;
-  }
-  if (split + 1 == last) {
-    c1 = ~(int)(split + 1);
-    parentLeaf[split + 1] = ((uint32_t)i << 1) | 1u;
-  } else {
-    c1 = (int)(split + 1);
-    parentInner[split + 1] = ((uint32_t)i << 1) | 1u;
-  }
-  // Compact topology (coalesced 16-byte writes); the refit writes each full node once. z keeps
-  // the other end of the node's leaf range [min(i,j), max(i,j)].
-  topo[i] = make_int4(c0, c1, (int)j, 0);
-  if (i == 0) parentInner[0] = 0xFFFFFFFFu;
+__device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
+__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+
+// delta(i) of the mesh's sorted keys; i = 0xFFFFFFFF stands for -1.
+__device__ __forceinline__ int key_delta(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
+                                         uint32_t i, uint32_t n) {
+  if (i >= n - 1) return -1;
+  const uint64_t x = ((uint64_t)codes[i] << 32) | ids[i];
+  const uint64_t y = ((uint64_t)codes[i + 1] << 32) | ids[i + 1];
+  return __clzll(x ^ y);
 }
 
-__device__ __forceinline__ void store_child_box(BVHNode *nodes, uint32_t node, uint32_t side, const float b[6]) {
-  float2 *p = (float2 *)((float *)&nodes[node] + side * 6);
-  p[0] = make_float2(b[0], b[1]);
-  p[1] = make_float2(b[2], b[3]);
-  p[2] = make_float2(b[4], b[5]);
+__device__ __forceinline__ void put_pending(Pending *p, const float b[6], int ref, uint32_t end) {
+  p->a = make_float4(b[0], b[1], b[2], b[3]);
+  p->b = make_float4(b[4], b[5], __int_as_float(ref), __uint_as_float(end));
+}
+
+__device__ __forceinline__ void get_pending(const Pending *p, float b[6], int &ref, uint32_t &end) {
+  const float4 x = p->a, y = p->b;
+  b[0] = x.x;
+  b[1] = x.y;
+  b[2] = x.z;
+  b[3] = x.w;
+  b[4] = y.x;
+  b[5] = y.y;
+  ref = __float_as_int(y.z);
+  end = __float_as_uint(y.w);
 }
 
 __device__ __forceinline__ void load_child_box(const BVHNode *nodes, uint32_t node, uint32_t side, float b[6]) {
@@ -518,56 +490,44 @@ __device__ __forceinline__ void merge_box(float box[6], const float sib[6]) {
   box[5] = fmax_ord(box[5], sib[5]);
 }
 
-// ------------------------------------------------------------------------------------------------
-// Refit, phase 1 (one 1024-thread workgroup per 1024-key sub-tile, one leaf per thread): leaf
-// records and every inner node whose leaf range lies inside the sub-tile. The climb runs
-// entirely in LDS: the topology of the sub-tile's candidate nodes is loaded once (coalesced), a
-// finished node's box and each leaf box stay in LDS, arrival counters are LDS atomics ordered by
-// LDS-only fences (no wait on global stores), and the finished nodes' child boxes are written in
-// one coalesced pass at the end. A climb that reaches a node whose range leaves the sub-tile
-// writes its box into that node's child slot and records the edge for phase 2.
-// ------------------------------------------------------------------------------------------------
-constexpr int kRefitSub = 1024;
-constexpr uint32_t kRefitRegion = 65536;  // keys per phase-2 region (a multiple of every tile size)
+// Writes a finished node whole: child boxes, child references, the other end of its leaf range.
+__device__ __forceinline__ void write_node(BVHNode *node, const float b0[6], const float b1[6], int c0, int c1,
+                                           uint32_t other) {
+  float4 *dst = (float4 *)node;
+  dst[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
+  dst[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
+  dst[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
+  dst[3] = make_float4(__int_as_float(c0), __int_as_float(c1), __uint_as_float(other), 0.0f);
+}
 
-__device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
-__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
-
-__global__ void __launch_bounds__(kRefitSub) k_refit_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
-  __shared__ float s_box[kRefitSub][6];   // box of a finished local inner node (index - lo)
-  __shared__ float s_leaf[kRefitSub][6];  // leaf boxes (position - lo)
-  __shared__ uint32_t s_cnt[kRefitSub];
-  __shared__ uint32_t s_par[kRefitSub];
-  __shared__ int s_c0[kRefitSub], s_c1[kRefitSub], s_other[kRefitSub];
+__global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
+  __shared__ int s_delta[kBuildSub + 1];    // delta(lo - 1 + q)
+  __shared__ float s_leaf[kBuildSub][6];    // leaf boxes (position - lo)
+  __shared__ float s_box[kBuildSub][6];     // finished inner node boxes (index - lo)
+  __shared__ uint32_t s_meet[kBuildSub];    // sides arrived at split lo + q: bit 0 left, bit 1 right
+  __shared__ int s_ref[kBuildSub][2];       // the arrived child per side (inner index or ~leaf)
+  __shared__ uint32_t s_end[kBuildSub][2];  // its far leaf (l of a left child, r of a right child)
   const uint32_t t = tileBase + blockIdx.x / subPerTile, sub = blockIdx.x % subPerTile;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const TileRange r = tile_range(a, d, t, m);
-  const uint32_t lo = r.lo + sub * kRefitSub;
+  const uint32_t lo = r.lo + sub * kBuildSub;
   if (lo >= r.hi) return;  // uniform over the workgroup
-  const uint32_t hi = min(lo + (uint32_t)kRefitSub, r.hi);
+  const uint32_t hi = min(lo + (uint32_t)kBuildSub, r.hi);
   const uint32_t n = d.ntri;
+  const uint32_t *__restrict__ codes = a.codes + d.primOffset;
+  const uint32_t *__restrict__ ids = a.ids + d.primOffset;
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
-  const uint32_t *__restrict__ parentLeaf = a.parentLeaf + d.primOffset;
-  const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
+  Pending *__restrict__ pending = a.pending + 2 * (size_t)d.innerOffset;
   float *__restrict__ header = a.headers + (size_t)m * 16;
   const uint32_t tid = threadIdx.x;
-  const uint32_t i = lo + tid;  // candidate inner node (index inside the sub-tile)
-  const bool cand = i < hi && i + 1 < n;
-  s_cnt[tid] = 0u;
-  int4 dd = make_int4(0, 0, 0, 0);
-  if (cand) {
-    dd = a.topo[d.innerOffset + i];
-    s_c0[tid] = dd.x;
-    s_c1[tid] = dd.y;
-    s_other[tid] = dd.z;
-    s_par[tid] = parentInner[i];
-  }
-  __syncthreads();
-  const uint32_t k = i;  // leaf position
-  uint32_t edge = 0xFFFFFFFFu;  // link into a node spanning sub-tiles, for phase 2
+  const uint32_t k = lo + tid;  // leaf position
+  if (k < hi) s_delta[tid + 1] = key_delta(codes, ids, k, n);
+  if (tid == 0) s_delta[0] = key_delta(codes, ids, lo - 1u, n);
+  s_meet[tid] = 0u;
+  float box[6];
   if (k < hi) {
-    const uint32_t prim = a.ids[d.primOffset + k];
+    const uint32_t prim = ids[k];
     float3 p, q, s;
     load_triangle(d, prim, p, q, s);
     TriRecord rec;
@@ -575,156 +535,189 @@ __global__ void __launch_bounds__(kRefitSub) k_refit_sub(BuildArgs a, uint32_t t
     rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
     rec.v2 = make_float4(s.x, s.y, s.z, 0.0f);
     a.tris[d.primOffset + k] = rec;
-    float box[6];
     box[0] = fmin_ord(p.x, fmin_ord(q.x, s.x));
     box[1] = fmin_ord(p.y, fmin_ord(q.y, s.y));
     box[2] = fmin_ord(p.z, fmin_ord(q.z, s.z));
     box[3] = fmax_ord(p.x, fmax_ord(q.x, s.x));
     box[4] = fmax_ord(p.y, fmax_ord(q.y, s.y));
     box[5] = fmax_ord(p.z, fmax_ord(q.z, s.z));
-    if (n == 1) {
-      for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-    } else {
-      for (int c = 0; c < 6; ++c) s_leaf[tid][c] = box[c];
-      uint32_t link = parentLeaf[k];
-      // A radix tree over distinct 64-bit keys is at most 65 levels deep; the bound and the
-      // index checks only guard against a malformed topology.
-      for (int level = 0; level < 72; ++level) {
-        const uint32_t node = link >> 1, side = link & 1u;
-        if (node >= n - 1) break;
-        bool local = node >= lo && node < hi;
-        if (local) {
-          const uint32_t other = (uint32_t)s_other[node - lo];
-          local = other >= lo && other < hi;
-        }
-        if (!local) {
-          // The node's range leaves this sub-tile: phase 2 finishes it.
-          store_child_box(nodes, node, side, box);
-          edge = link;
-          break;
-        }
-        lds_release();
-        const uint32_t old =
-            __hip_atomic_fetch_add(&s_cnt[node - lo], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        lds_acquire();
-        if (old != 1u) break;  // first arrival: the sibling's thread finishes this node
-        // Both children lie inside the node's range, hence inside the sub-tile.
-        const int sib = side ? s_c0[node - lo] : s_c1[node - lo];
-        const float *sb = sib < 0 ? s_leaf[(uint32_t)(~sib) - lo] : s_box[(uint32_t)sib - lo];
-        float sbox[6];
-        for (int c = 0; c < 6; ++c) sbox[c] = sb[c];
-        merge_box(box, sbox);
-        for (int c = 0; c < 6; ++c) s_box[node - lo][c] = box[c];
-        link = s_par[node - lo];
-        if (link == 0xFFFFFFFFu) {
-          for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-          break;
-        }
+    for (int c = 0; c < 6; ++c) s_leaf[tid][c] = box[c];
+  }
+  __syncthreads();
+  uint32_t edge = kNoEdge;
+  if (k < hi) {
+    uint32_t l = k, rr = k;
+    int ref = ~(int)k;
+    // A radix tree over distinct 64-bit keys is at most 64 levels deep; the bound only guards
+    // against malformed keys.
+    for (int level = 0; level < 72; ++level) {
+      const int dl = s_delta[l - lo], dr = s_delta[rr - lo + 1];
+      if (dl < 0 && dr < 0) {  // [l, rr] = [0, n - 1]: the root (or the only leaf)
+        for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
+        break;
       }
+      const bool leftChild = dr > dl;
+      const uint32_t split = leftChild ? rr : l - 1u;
+      const uint32_t side = leftChild ? 0u : 1u;
+      const uint32_t end = leftChild ? l : rr;
+      if (split < lo || split + 1u >= hi) {
+        // The parent's leaf range leaves the sub-tile: the region level finishes it.
+        put_pending(&pending[2 * (size_t)split + side], box, ref, end);
+        edge = (split << 1) | side;
+        break;
+      }
+      const uint32_t g = split - lo;
+      s_ref[g][side] = ref;
+      s_end[g][side] = end;
+      lds_release();
+      const uint32_t old =
+          __hip_atomic_fetch_or(&s_meet[g], 1u << side, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      lds_acquire();
+      if (old == 0u) break;  // first arrival: the sibling's thread finishes the parent
+      // Both children lie inside the parent's range, hence inside the sub-tile.
+      const int sref = s_ref[g][side ^ 1u];
+      const uint32_t send = s_end[g][side ^ 1u];
+      const float *sb = sref < 0 ? s_leaf[(uint32_t)(~sref) - lo] : s_box[(uint32_t)sref - lo];
+      float sbox[6];
+      for (int c = 0; c < 6; ++c) sbox[c] = sb[c];
+      const uint32_t pl = leftChild ? l : send, pr = leftChild ? send : rr;
+      const int pdl = s_delta[pl - lo], pdr = s_delta[pr - lo + 1];
+      const uint32_t P = (pdl < 0 && pdr < 0) ? 0u : (pdr > pdl ? pr : pl);
+      if (leftChild)
+        write_node(&nodes[P], box, sbox, ref, sref, P == pl ? pr : pl);
+      else
+        write_node(&nodes[P], sbox, box, sref, ref, P == pl ? pr : pl);
+      merge_box(box, sbox);
+      for (int c = 0; c < 6; ++c) s_box[P - lo][c] = box[c];
+      l = pl;
+      rr = pr;
+      ref = (int)P;
     }
   }
   // Append the edges of this wave to the list of its 65536-key region: one atomic per wave.
   {
-    const uint64_t mask = __ballot(edge != 0xFFFFFFFFu);
+    const uint64_t mask = __ballot(edge != kNoEdge);
     if (mask) {
       const int lane = (int)(threadIdx.x & 63u);
       const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      const uint32_t region = lo / kRefitRegion;
-      const uint32_t regionLo = region * kRefitRegion;
+      const uint32_t regionLo = (lo / kBuildRegion) * kBuildRegion;
       uint32_t *cnt = &a.regionCount[d.tileOffset + regionLo / a.tileKeys];
       uint32_t base = 0;
       if (lane == __ffsll((long long)mask) - 1) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
       base = __shfl(base, __ffsll((long long)mask) - 1, 64);
-      // A region's list holds at most one edge per key of the region (each edge roots a disjoint
-      // subtree of the region's leaves); the guard keeps writes inside the mesh's entries.
-      if (edge != 0xFFFFFFFFu && base + before < min(kRefitRegion, n - regionLo))
+      // A region's list holds at most one edge per key of the region (each edge's child roots a
+      // disjoint subtree of the region's leaves); the guard keeps writes inside the mesh's entries.
+      if (edge != kNoEdge && base + before < min(kBuildRegion, n - regionLo))
         a.entries[d.primOffset + regionLo + base + before] = edge;
     }
   }
   __syncthreads();
-  // Every candidate node is written whole (64 bytes): finished local nodes with both child boxes;
-  // nodes spanning sub-tiles get their topology here and their child boxes from the climbs.
-  if (cand) {
-    const uint32_t other = (uint32_t)dd.z;
-    float4 *dst = (float4 *)&nodes[i];
-    if (other >= lo && other < hi) {
-      const int c0 = dd.x, c1 = dd.y;
-      const float *b0 = c0 < 0 ? s_leaf[(uint32_t)(~c0) - lo] : s_box[(uint32_t)c0 - lo];
-      const float *b1 = c1 < 0 ? s_leaf[(uint32_t)(~c1) - lo] : s_box[(uint32_t)c1 - lo];
-      dst[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
-      dst[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
-      dst[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
+  // Interior splits [lo, hi - 2] with one child arrived: the other child spans sub-tiles and
+  // arrives at a later level, which then finds the record here and the counter at 1.
+  if (tid + 1u < hi - lo) {
+    const uint32_t meet = s_meet[tid];
+    if (meet == 1u || meet == 2u) {
+      const uint32_t side = meet >> 1;
+      const int ref = s_ref[tid][side];
+      const float *b = ref < 0 ? s_leaf[(uint32_t)(~ref) - lo] : s_box[(uint32_t)ref - lo];
+      float cb[6];
+      for (int c = 0; c < 6; ++c) cb[c] = b[c];
+      put_pending(&pending[2 * (size_t)(lo + tid) + side], cb, ref, s_end[tid][side]);
+      a.counters[d.innerOffset + lo + tid] = 1u;
     }
-    dst[3] = make_float4(__int_as_float(dd.x), __int_as_float(dd.y), __int_as_float(dd.z), 0.0f);
   }
 }
 
-// ------------------------------------------------------------------------------------------------
-// Refit, phases 2 and 3: the nodes whose range spans several sub-tiles. Phase 2 runs one
-// workgroup per 65536-key region and finishes the nodes whose range lies inside the region;
-// phase 3 runs one workgroup per mesh for the few nodes that span regions. Each edge climbs
-// through arrival counters ordered at workgroup scope (the workgroup's waves share one CU, so this
-// needs no L2 maintenance across XCDs); the second arrival merges both child boxes and climbs on.
-// Kernel boundaries order the phases.
-// ------------------------------------------------------------------------------------------------
-constexpr int kRefitTop = 512;
-
-// Climbs from `link` while the nodes' leaf ranges lie in [regionLo, regionHi); returns the link
-// into the first node outside (0xFFFFFFFF when the climb ended).
-__device__ __forceinline__ uint32_t refit_climb(const BuildArgs &a, const MeshDesc &d, uint32_t m, uint32_t link,
-                                                uint32_t regionLo, uint32_t regionHi) {
+// Climbs from the child record waiting at `edge` (split << 1 | side) while the parents' splits
+// are interior to [rangeLo, rangeHi); returns the edge that leaves the range (kNoEdge when the
+// climb ended at a first arrival or at the root).
+__device__ __forceinline__ uint32_t build_climb(const BuildArgs &a, const MeshDesc &d, uint32_t m, uint32_t edge,
+                                                uint32_t rangeLo, uint32_t rangeHi) {
   const uint32_t n = d.ntri;
+  const uint32_t *__restrict__ codes = a.codes + d.primOffset;
+  const uint32_t *__restrict__ ids = a.ids + d.primOffset;
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
-  const uint32_t *__restrict__ parentInner = a.parentInner + d.innerOffset;
+  Pending *__restrict__ pending = a.pending + 2 * (size_t)d.innerOffset;
   uint32_t *__restrict__ counters = a.counters + d.innerOffset;
-  const int4 *__restrict__ topo = a.topo + d.innerOffset;
   for (int level = 0; level < 72; ++level) {
-    const uint32_t node = link >> 1;
-    if (node >= n - 1) return 0xFFFFFFFFu;
-    const uint32_t other = (uint32_t)topo[node].z;
-    if (min(node, other) < regionLo || max(node, other) >= regionHi) return link;
-    const uint32_t old = __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old != 1u) return 0xFFFFFFFFu;
-    float box[6], sib[6];
-    load_child_box(nodes, node, 0, box);
-    load_child_box(nodes, node, 1, sib);
-    merge_box(box, sib);
-    link = parentInner[node];
-    if (link == 0xFFFFFFFFu) {
+    const uint32_t split = edge >> 1, side = edge & 1u;
+    if (split >= n - 1) return kNoEdge;  // only a malformed edge
+    if (split < rangeLo || split + 1u >= rangeHi) return edge;
+    const uint32_t old = __hip_atomic_fetch_add(&counters[split], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old != 1u) return kNoEdge;  // first arrival
+    float b0[6], b1[6];
+    int c0, c1;
+    uint32_t pl, pr;
+    get_pending(&pending[2 * (size_t)split], b0, c0, pl);
+    get_pending(&pending[2 * (size_t)split + 1], b1, c1, pr);
+    const int dl = key_delta(codes, ids, pl - 1u, n), dr = key_delta(codes, ids, pr, n);
+    const bool root = dl < 0 && dr < 0;
+    const uint32_t P = root ? 0u : (dr > dl ? pr : pl);
+    write_node(&nodes[P], b0, b1, c0, c1, P == pl ? pr : pl);
+    merge_box(b0, b1);
+    if (root) {
       float *__restrict__ header = a.headers + (size_t)m * 16;
-      for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-      return 0xFFFFFFFFu;
+      for (int c = 0; c < 6; ++c) header[8 + c] = b0[c];
+      return kNoEdge;
     }
-    store_child_box(nodes, link >> 1, link & 1u, box);
+    const bool leftChild = dr > dl;
+    const uint32_t up = leftChild ? pr : pl - 1u;
+    const uint32_t upSide = leftChild ? 0u : 1u;
+    put_pending(&pending[2 * (size_t)up + upSide], b0, (int)P, leftChild ? pl : pr);
+    edge = (up << 1) | upSide;
   }
-  return 0xFFFFFFFFu;
+  return kNoEdge;
 }
 
-__global__ void __launch_bounds__(kRefitTop) k_refit_region(BuildArgs a, uint32_t tileBase) {
+__global__ void __launch_bounds__(kBuildTop) k_build_region(BuildArgs a, uint32_t tileBase) {
   const uint32_t t = tileBase + blockIdx.x;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const uint32_t lt = t - d.tileOffset;
-  const uint32_t perRegion = kRefitRegion / a.tileKeys;
+  const uint32_t perRegion = kBuildRegion / a.tileKeys;
   if (lt % perRegion != 0) return;  // one workgroup per region: its first tile
   const uint32_t regionLo = lt * a.tileKeys;
-  const uint32_t regionHi = min(regionLo + kRefitRegion, d.ntri);
+  const uint32_t regionHi = min(regionLo + kBuildRegion, d.ntri);
   const uint32_t count = min(a.regionCount[t], regionHi - regionLo);
-  for (uint32_t e = threadIdx.x; e < count; e += kRefitTop) {
-    const uint32_t up = refit_climb(a, d, m, a.entries[d.primOffset + regionLo + e], regionLo, regionHi);
-    if (up != 0xFFFFFFFFu) {
+  for (uint32_t e = threadIdx.x; e < count; e += kBuildTop) {
+    const uint32_t up = build_climb(a, d, m, a.entries[d.primOffset + regionLo + e], regionLo, regionHi);
+    if (up != kNoEdge) {
       const uint32_t slot = atomicAdd(&a.edgeCount[m], 1u);
       if (slot < d.ntri) a.entriesTop[d.primOffset + slot] = up;
     }
   }
 }
 
-__global__ void __launch_bounds__(kRefitTop) k_refit_top(BuildArgs a, uint32_t meshBase) {
+__global__ void __launch_bounds__(kBuildTop) k_build_top(BuildArgs a, uint32_t meshBase) {
   const uint32_t m = meshBase + blockIdx.x;
   const MeshDesc d = a.desc[m];
   const uint32_t count = min(a.edgeCount[m], d.ntri);
-  for (uint32_t e = threadIdx.x; e < count; e += kRefitTop)
-    refit_climb(a, d, m, a.entriesTop[d.primOffset + e], 0u, d.ntri);
+  for (uint32_t e = threadIdx.x; e < count; e += kBuildTop)
+    build_climb(a, d, m, a.entriesTop[d.primOffset + e], 0u, d.ntri);
+}
+
+// Parent links (inner index << 1 | side; 0xFFFFFFFF at the root) of one mesh, derived from the
+// finished nodes when a consumer asks for them (the reference-layout export, proposal classes).
+__global__ void __launch_bounds__(256) k_parents(const BVHNode *__restrict__ nodes, uint32_t n,
+                                                 uint32_t *__restrict__ parentLeaf,
+                                                 uint32_t *__restrict__ parentInner) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n <= 1) {
+    if (i == 0 && n == 1) parentLeaf[0] = 0xFFFFFFFFu;
+    return;
+  }
+  if (i >= n - 1) return;
+  if (i == 0) parentInner[0] = 0xFFFFFFFFu;
+  const int4 c = nodes[i].d;
+  const int ch[2] = {c.x, c.y};
+  for (uint32_t side = 0; side < 2; ++side) {
+    const int x = ch[side];
+    if (x < 0) {
+      if ((uint32_t)(~x) < n) parentLeaf[~x] = (i << 1) | side;
+    } else if ((uint32_t)x < n - 1) {
+      parentInner[x] = (i << 1) | side;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -804,7 +797,7 @@ BuildArgs args_of(BVHStore *st) {
   a.status = st->status;
   a.tickets = st->tickets;
   a.entries = st->entries;
-  a.topo = (int4 *)st->topo;
+  a.pending = (Pending *)st->pending;
   a.entriesTop = st->entriesTop;
   a.regionCount = st->regionCount;
   a.edgeCount = st->edgeCount;
@@ -815,11 +808,11 @@ BuildArgs args_of(BVHStore *st) {
 
 void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a, uint32_t tileBase, uint32_t tiles,
                   uint32_t meshBase, uint32_t meshes) {
-  const uint32_t subPerTile = (uint32_t)st->cfg.tileKeys / kRefitSub;
-  launch(ctx, s, "lbvh_refit_local", k_refit_sub, dim3(tiles * subPerTile), dim3(kRefitSub), 0, a, tileBase,
+  const uint32_t subPerTile = (uint32_t)st->cfg.tileKeys / kBuildSub;
+  launch(ctx, s, "lbvh_build_local", k_build_sub, dim3(tiles * subPerTile), dim3(kBuildSub), 0, a, tileBase,
          subPerTile);
-  launch(ctx, s, "lbvh_refit_region", k_refit_region, dim3(tiles), dim3(kRefitTop), 0, a, tileBase);
-  launch(ctx, s, "lbvh_refit_top", k_refit_top, dim3(meshes), dim3(kRefitTop), 0, a, meshBase);
+  launch(ctx, s, "lbvh_build_region", k_build_region, dim3(tiles), dim3(kBuildTop), 0, a, tileBase);
+  launch(ctx, s, "lbvh_build_top", k_build_top, dim3(meshes), dim3(kBuildTop), 0, a, meshBase);
 }
 
 // Extra dynamic LDS that caps a sort pass at `wavesPerCU` resident waves per CU (160 KB of LDS
@@ -828,7 +821,7 @@ template <int BITS, int THREADS, int ITEMS>
 size_t sort_lds_pad(int wavesPerCU) {
   if (wavesPerCU <= 0) return 0;
   constexpr int R = 1 << BITS, W = THREADS / 64;
-  const size_t natural = (size_t)(THREADS * ITEMS + W * R + 2 * R + W + 1) * 4;
+  const size_t natural = (size_t)(2 * THREADS * ITEMS + W * R + 2 * R + W + 1) * 4;
   const int wgs = std::max(1, wavesPerCU / W);
   const size_t target = std::min<size_t>((160u * 1024u) / wgs, 64u * 1024u);
   return target > natural ? target - natural : 0;
@@ -878,8 +871,6 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
   }
   if (st->passes & 1)
     launch(ctx, s, "lbvh_copy_back", k_copy_back, dim3(grid_for(st->prims, 256, 8192)), dim3(256), 0, a, st->prims);
-  launch(ctx, s, "lbvh_karras", k_karras, dim3(st->tiles * ((uint32_t)st->cfg.tileKeys / kKarrasSub)),
-         dim3(kKarrasSub), 0, a, 0u, (uint32_t)st->cfg.tileKeys / kKarrasSub);
   launch_refit(ctx, s, st, a, 0u, st->tiles, 0u, st->meshes);
 }
 
@@ -936,8 +927,8 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   off += al((size_t)prims * 4);
   const size_t oEntTop = off;
   off += al((size_t)prims * 4);
-  const size_t oTopo = off;
-  off += al((size_t)std::max<uint32_t>(I, 1) * 16);
+  const size_t oPend = off;
+  off += al((size_t)std::max<uint32_t>(I, 1) * 2 * sizeof(Pending));
   const size_t oZero = off;
   const size_t cntWords = std::max<uint32_t>(I, 1);
   const size_t histWords = (size_t)M * passes * R;
@@ -971,7 +962,7 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->parentInner = (uint32_t *)(b + oPI);
   st->headers = (float *)(b + oHdr);
   st->entries = (uint32_t *)(b + oEnt);
-  st->topo = (int4 *)(b + oTopo);
+  st->pending = b + oPend;
   st->entriesTop = (uint32_t *)(b + oEntTop);
   st->zeroBase = (uint32_t *)(b + oZero);
   st->zeroWords = zeroWords;
@@ -1131,10 +1122,18 @@ bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   return true;
 }
 
+void lbvh_parents(Context *ctx, hipStream_t s, Accel *acc) {
+  BVHStore *st = acc->store;
+  const MeshDesc &d = st->desc[acc->mesh];
+  launch(ctx, s, "lbvh_parents", k_parents, dim3(grid_for(d.ntri, 256, 1u << 22)), dim3(256), 0,
+         (const BVHNode *)acc->nodes, d.ntri, st->parentLeaf + d.primOffset, st->parentInner + d.innerOffset);
+}
+
 bool lbvh_export(Context *ctx, hipStream_t s, Accel *acc, int32_t *nodes, float *aabbs, uint64_t *keys) {
   BVHStore *st = acc->store;
   const MeshDesc &d = st->desc[acc->mesh];
   const uint32_t n = d.ntri;
+  lbvh_parents(ctx, s, acc);
   launch(ctx, s, "lbvh_export", k_export, dim3(grid_for(2 * (size_t)n - 1, 256, 1u << 22)), dim3(256), 0,
          (const BVHNode *)acc->nodes, (const uint32_t *)(st->codes + d.primOffset),
          (const uint32_t *)(st->ids + d.primOffset), (const uint32_t *)(st->parentLeaf + d.primOffset),

@@ -832,6 +832,7 @@ bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth)
   fill_u32(ctx, s, p->leafClass, 0xFFFFFFFFu, n);
   launch(ctx, s, "mlp_treelet_cut", k_treelet_cut, dim3(1), dim3(1024), 0, (const BVHNode *)a->nodes, n, depth,
          p->classRoot, p->innerClass, p->leafClass, p->numClassesDev);
+  lbvh_parents(ctx, s, a);
   launch(ctx, s, "mlp_leaf_class", k_leaf_class, dim3(blocks_for(n, 256)), dim3(256), 0,
          (const uint32_t *)(st->parentLeaf + d.primOffset), (const uint32_t *)(st->parentInner + d.innerOffset),
          (const int32_t *)p->innerClass, p->leafClass, (const uint32_t *)(st->ids + d.primOffset), p->primClass, n);
