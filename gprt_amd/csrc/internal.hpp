@@ -141,7 +141,7 @@ struct BVHStore {
   void *pending = nullptr;      // [inner][2] x 32 B: child records waiting at a split (box, ref, far end)
   uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into splits on region edges
   uint32_t *edgeCount = nullptr;    // [meshes] (in the zeroed range)
-  uint32_t *regionCount = nullptr;  // [tiles] (in the zeroed range)
+  uint32_t *subCount = nullptr;     // [tiles * tileKeys / 1024]: edges of each 1024-key sub-tile
   // Zeroed at the start of every build (one contiguous range):
   uint32_t *zeroBase = nullptr;
   size_t zeroWords = 0;

@@ -56,12 +56,12 @@ struct BuildArgs {
   uint32_t *hist, *status, *tickets;
   Pending *pending;      // [inner][2]: child records waiting at a split, per side
   // Build edges (split << 1 | side of a child whose parent a level cannot finish), at the mesh's
-  // primOffset: entries: from sub-tiles into splits on sub-tile edges, one list per 65536-key
-  // region (count at regionCount[first tile of the region]); entriesTop: from regions into
+  // primOffset: entries: from sub-tiles into splits on sub-tile edges, each sub-tile's list at its
+  // own key positions (count at subCount[tile * subPerTile + sub]); entriesTop: from regions into
   // splits on region edges, one list per mesh (count at edgeCount[mesh]). counters: arrivals
   // per split.
   uint32_t *entries, *entriesTop;
-  uint32_t *regionCount;
+  uint32_t *subCount;
   uint32_t *edgeCount;
   int passes;
   int digitBits;
@@ -424,12 +424,15 @@ __global__ void __launch_bounds__(256) k_copy_back(BuildArgs a, uint32_t prims) 
 // gprt_lbvh.hlsl:400-428, with every node combined exactly once).
 //
 // Three levels, so that no arrival counter needs ordering across XCDs:
-//  1. k_build_sub: one 1024-thread workgroup per 1024-key sub-tile, one leaf per thread. The
-//     sub-tile's deltas, leaf boxes, finished node boxes and the meeting slots of its interior
-//     splits live in LDS, ordered by LDS-only fences. A child whose parent is split on the
-//     sub-tile's edge leaves its record in `pending` and becomes an edge of its 65536-key region;
-//     a meeting slot still half-filled at the end (the sibling spans sub-tiles) is copied to
-//     `pending` with its split's counter set to 1.
+//  1. k_build_sub: one 1024-thread workgroup per 1024-key sub-tile. The sub-tile's deltas, leaf
+//     boxes, finished node boxes and the meeting slots of its interior splits live in LDS. The
+//     climb runs in rounds separated by barriers: round 0 sends every leaf to its parent's split,
+//     and each later round merges the splits whose second child arrived in the previous round,
+//     one per thread from a queue, so the shrinking work stays packed in the first waves instead
+//     of leaving one active lane in every wave. A child whose parent is split on the sub-tile's
+//     edge leaves its record in `pending` and becomes an edge of its 65536-key region; a meeting
+//     slot still half-filled at the end (the sibling spans sub-tiles) is copied to `pending` with
+//     its split's counter set to 1.
 //  2. k_build_region: one workgroup per region climbs the region's edges through global counters
 //     ordered at workgroup scope (the workgroup's waves share one CU); climbs that leave the
 //     region become edges of the mesh.
@@ -443,6 +446,13 @@ constexpr uint32_t kNoEdge = 0xFFFFFFFFu;
 
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
 __device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+// A workgroup barrier that orders LDS only: unlike __syncthreads it does not wait for the
+// workgroup's outstanding global stores (the finished nodes), which no one reads in this kernel.
+__device__ __forceinline__ void lds_barrier() {
+  lds_release();
+  __builtin_amdgcn_s_barrier();
+  lds_acquire();
+}
 
 // delta(i) of the mesh's sorted keys; i = 0xFFFFFFFF stands for -1.
 __device__ __forceinline__ int key_delta(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
@@ -501,18 +511,25 @@ __device__ __forceinline__ void write_node(BVHNode *node, const float b0[6], con
 }
 
 __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
-  __shared__ int s_delta[kBuildSub + 1];    // delta(lo - 1 + q)
-  __shared__ float s_leaf[kBuildSub][6];    // leaf boxes (position - lo)
-  __shared__ float s_box[kBuildSub][6];     // finished inner node boxes (index - lo)
-  __shared__ uint32_t s_meet[kBuildSub];    // sides arrived at split lo + q: bit 0 left, bit 1 right
-  __shared__ int s_ref[kBuildSub][2];       // the arrived child per side (inner index or ~leaf)
-  __shared__ uint32_t s_end[kBuildSub][2];  // its far leaf (l of a left child, r of a right child)
+  __shared__ int8_t s_delta[kBuildSub + 4];        // delta(lo - 1 + q), in [-1, 63]
+  __shared__ uint16_t s_split[kBuildSub];          // split of finished local node lo + q (0xFFFF: none)
+  __shared__ float s_leaf[kBuildSub][6];           // leaf boxes (position - lo)
+  __shared__ float s_box[kBuildSub][6];            // finished inner node boxes (index - lo)
+  __shared__ uint32_t s_meet[kBuildSub];           // sides arrived at split lo + q: bit 0 left, bit 1 right
+  __shared__ int s_ref[kBuildSub][2];              // the arrived child per side (inner index or ~leaf)
+  __shared__ uint32_t s_end[kBuildSub][2];         // its far leaf (l of a left child, r of a right child)
+  __shared__ uint32_t s_ready[2][kBuildSub / 2];   // splits whose two children have arrived
+  __shared__ uint32_t s_nready[4];                 // [0..2] ready-list sizes by round, [3] edges
   const uint32_t t = tileBase + blockIdx.x / subPerTile, sub = blockIdx.x % subPerTile;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const TileRange r = tile_range(a, d, t, m);
   const uint32_t lo = r.lo + sub * kBuildSub;
-  if (lo >= r.hi) return;  // uniform over the workgroup
+  uint32_t *__restrict__ subCount = a.subCount + (size_t)t * subPerTile + sub;
+  if (lo >= r.hi) {  // uniform over the workgroup
+    if (threadIdx.x == 0) *subCount = 0u;
+    return;
+  }
   const uint32_t hi = min(lo + (uint32_t)kBuildSub, r.hi);
   const uint32_t n = d.ntri;
   const uint32_t *__restrict__ codes = a.codes + d.primOffset;
@@ -524,7 +541,9 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   const uint32_t k = lo + tid;  // leaf position
   if (k < hi) s_delta[tid + 1] = key_delta(codes, ids, k, n);
   if (tid == 0) s_delta[0] = key_delta(codes, ids, lo - 1u, n);
+  if (tid < 4) s_nready[tid] = 0u;
   s_meet[tid] = 0u;
+  s_split[tid] = 0xFFFFu;
   float box[6];
   if (k < hi) {
     const uint32_t prim = ids[k];
@@ -543,75 +562,108 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
     box[5] = fmax_ord(p.z, fmax_ord(q.z, s.z));
     for (int c = 0; c < 6; ++c) s_leaf[tid][c] = box[c];
   }
-  __syncthreads();
-  uint32_t edge = kNoEdge;
-  if (k < hi) {
-    uint32_t l = k, rr = k;
-    int ref = ~(int)k;
-    // A radix tree over distinct 64-bit keys is at most 64 levels deep; the bound only guards
-    // against malformed keys.
-    for (int level = 0; level < 72; ++level) {
-      const int dl = s_delta[l - lo], dr = s_delta[rr - lo + 1];
-      if (dl < 0 && dr < 0) {  // [l, rr] = [0, n - 1]: the root (or the only leaf)
-        for (int c = 0; c < 6; ++c) header[8 + c] = box[c];
-        break;
+  lds_barrier();
+
+  // The node [l, rr] (reference `ref`, box `b`, already in LDS) arrives at its parent's split: the
+  // root ends the climb, a split on the sub-tile's edge becomes an edge for the region level, and
+  // the second arrival at an interior split queues it for the next round.
+  auto arrive = [&](uint32_t l, uint32_t rr, int ref, const float b[6], uint32_t nextQ, uint32_t &edge) {
+    const int dl = s_delta[l - lo], dr = s_delta[rr - lo + 1];
+    if (dl < 0 && dr < 0) {  // [l, rr] = [0, n - 1]: the root (or the only leaf)
+      for (int c = 0; c < 6; ++c) header[8 + c] = b[c];
+      return;
+    }
+    const bool leftChild = dr > dl;
+    const uint32_t split = leftChild ? rr : l - 1u;
+    const uint32_t side = leftChild ? 0u : 1u;
+    const uint32_t end = leftChild ? l : rr;
+    if (split < lo || split + 1u >= hi) {
+      put_pending(&pending[2 * (size_t)split + side], b, ref, end);
+      edge = (split << 1) | side;
+      return;
+    }
+    const uint32_t g = split - lo;
+    s_ref[g][side] = ref;
+    s_end[g][side] = end;
+    const uint32_t old =
+        __hip_atomic_fetch_or(&s_meet[g], 1u << side, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old != 0u) {
+      const uint32_t slot =
+          __hip_atomic_fetch_add(&s_nready[nextQ % 3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      s_ready[nextQ & 1][slot] = g;
+    }
+  };
+  // Appends this wave's edges to the sub-tile's list (one LDS atomic per wave). The list lives at
+  // the sub-tile's own key positions: it holds at most one edge per key (each edge's child roots a
+  // disjoint subtree of the sub-tile's leaves); the guard keeps writes inside the sub-tile.
+  auto flush_edges = [&](uint32_t edge) {
+    const uint64_t mask = __ballot(edge != kNoEdge);
+    if (!mask) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int first = __ffsll((long long)mask) - 1;
+    const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == first)
+      base = __hip_atomic_fetch_add(&s_nready[3], (uint32_t)__popcll(mask), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+    base = __shfl(base, first, 64);
+    if (edge != kNoEdge && base + before < hi - lo) a.entries[d.primOffset + lo + base + before] = edge;
+  };
+
+  // Round 0: the leaves. Round i > 0 merges the splits queued in round i - 1, so the active climbs
+  // are packed into the first waves; the barrier between rounds orders every LDS write.
+  {
+    uint32_t edge = kNoEdge;
+    if (k < hi) arrive(k, k, ~(int)k, box, 1u, edge);
+    flush_edges(edge);
+  }
+  lds_barrier();
+  // A split is queued once, so rounds never exceed the sub-tile's keys; the bound only guards
+  // against malformed keys.
+  for (uint32_t round = 1; round <= kBuildSub; ++round) {
+    const uint32_t count = s_nready[round % 3];
+    if (count == 0u) break;  // uniform: read after the barrier
+    if (tid == 0) s_nready[(round + 2) % 3] = 0u;  // the list the next round fills
+    uint32_t edge = kNoEdge;
+    if (tid < count) {
+      const uint32_t g = s_ready[round & 1][tid];
+      const int c0 = s_ref[g][0], c1 = s_ref[g][1];
+      const uint32_t pl = s_end[g][0], pr = s_end[g][1];
+      const float *p0 = c0 < 0 ? s_leaf[(uint32_t)(~c0) - lo] : s_box[(uint32_t)c0 - lo];
+      const float *p1 = c1 < 0 ? s_leaf[(uint32_t)(~c1) - lo] : s_box[(uint32_t)c1 - lo];
+      float b0[6], b1[6];
+      for (int c = 0; c < 6; ++c) {
+        b0[c] = p0[c];
+        b1[c] = p1[c];
       }
-      const bool leftChild = dr > dl;
-      const uint32_t split = leftChild ? rr : l - 1u;
-      const uint32_t side = leftChild ? 0u : 1u;
-      const uint32_t end = leftChild ? l : rr;
-      if (split < lo || split + 1u >= hi) {
-        // The parent's leaf range leaves the sub-tile: the region level finishes it.
-        put_pending(&pending[2 * (size_t)split + side], box, ref, end);
-        edge = (split << 1) | side;
-        break;
-      }
-      const uint32_t g = split - lo;
-      s_ref[g][side] = ref;
-      s_end[g][side] = end;
-      lds_release();
-      const uint32_t old =
-          __hip_atomic_fetch_or(&s_meet[g], 1u << side, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      lds_acquire();
-      if (old == 0u) break;  // first arrival: the sibling's thread finishes the parent
-      // Both children lie inside the parent's range, hence inside the sub-tile.
-      const int sref = s_ref[g][side ^ 1u];
-      const uint32_t send = s_end[g][side ^ 1u];
-      const float *sb = sref < 0 ? s_leaf[(uint32_t)(~sref) - lo] : s_box[(uint32_t)sref - lo];
-      float sbox[6];
-      for (int c = 0; c < 6; ++c) sbox[c] = sb[c];
-      const uint32_t pl = leftChild ? l : send, pr = leftChild ? send : rr;
       const int pdl = s_delta[pl - lo], pdr = s_delta[pr - lo + 1];
       const uint32_t P = (pdl < 0 && pdr < 0) ? 0u : (pdr > pdl ? pr : pl);
-      if (leftChild)
-        write_node(&nodes[P], box, sbox, ref, sref, P == pl ? pr : pl);
-      else
-        write_node(&nodes[P], sbox, box, sref, ref, P == pl ? pr : pl);
-      merge_box(box, sbox);
-      for (int c = 0; c < 6; ++c) s_box[P - lo][c] = box[c];
-      l = pl;
-      rr = pr;
-      ref = (int)P;
+      s_split[P - lo] = (uint16_t)g;  // written whole in the final pass
+      merge_box(b0, b1);
+      for (int c = 0; c < 6; ++c) s_box[P - lo][c] = b0[c];
+      arrive(pl, pr, (int)P, b0, round + 1u, edge);
     }
+    flush_edges(edge);
+    lds_barrier();
   }
-  // Append the edges of this wave to the list of its 65536-key region: one atomic per wave.
+  if (tid == 0) *subCount = min(s_nready[3], hi - lo);
+  // Finished local nodes, written whole and coalesced (node lo + tid by thread tid).
   {
-    const uint64_t mask = __ballot(edge != kNoEdge);
-    if (mask) {
-      const int lane = (int)(threadIdx.x & 63u);
-      const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      const uint32_t regionLo = (lo / kBuildRegion) * kBuildRegion;
-      uint32_t *cnt = &a.regionCount[d.tileOffset + regionLo / a.tileKeys];
-      uint32_t base = 0;
-      if (lane == __ffsll((long long)mask) - 1) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
-      base = __shfl(base, __ffsll((long long)mask) - 1, 64);
-      // A region's list holds at most one edge per key of the region (each edge's child roots a
-      // disjoint subtree of the region's leaves); the guard keeps writes inside the mesh's entries.
-      if (edge != kNoEdge && base + before < min(kBuildRegion, n - regionLo))
-        a.entries[d.primOffset + regionLo + base + before] = edge;
+    const uint32_t g = s_split[tid];
+    if (g != 0xFFFFu) {
+      const int c0 = s_ref[g][0], c1 = s_ref[g][1];
+      const uint32_t pl = s_end[g][0], pr = s_end[g][1];
+      const float *p0 = c0 < 0 ? s_leaf[(uint32_t)(~c0) - lo] : s_box[(uint32_t)c0 - lo];
+      const float *p1 = c1 < 0 ? s_leaf[(uint32_t)(~c1) - lo] : s_box[(uint32_t)c1 - lo];
+      float b0[6], b1[6];
+      for (int c = 0; c < 6; ++c) {
+        b0[c] = p0[c];
+        b1[c] = p1[c];
+      }
+      const uint32_t P = lo + tid;
+      write_node(&nodes[P], b0, b1, c0, c1, P == pl ? pr : pl);
     }
   }
-  __syncthreads();
   // Interior splits [lo, hi - 2] with one child arrived: the other child spans sub-tiles and
   // arrives at a later level, which then finds the record here and the counter at 1.
   if (tid + 1u < hi - lo) {
@@ -678,9 +730,27 @@ __global__ void __launch_bounds__(kBuildTop) k_build_region(BuildArgs a, uint32_
   if (lt % perRegion != 0) return;  // one workgroup per region: its first tile
   const uint32_t regionLo = lt * a.tileKeys;
   const uint32_t regionHi = min(regionLo + kBuildRegion, d.ntri);
-  const uint32_t count = min(a.regionCount[t], regionHi - regionLo);
+  // The region's sub-tiles are the sub-tiles of its consecutive tiles: gather their edge counts.
+  constexpr uint32_t kSubs = kBuildRegion / kBuildSub;
+  __shared__ uint32_t s_pre[kSubs + 1];
+  const uint32_t subPerTile = a.tileKeys / kBuildSub;
+  const uint32_t nsub = (regionHi - regionLo + kBuildSub - 1) / kBuildSub;
+  if (threadIdx.x < 64) {
+    const uint32_t c = threadIdx.x < nsub ? min(a.subCount[(size_t)t * subPerTile + threadIdx.x], (uint32_t)kBuildSub) : 0u;
+    s_pre[threadIdx.x + 1] = wave_incl_sum(c);
+    if (threadIdx.x == 0) s_pre[0] = 0u;
+  }
+  __syncthreads();
+  const uint32_t count = s_pre[nsub];
   for (uint32_t e = threadIdx.x; e < count; e += kBuildTop) {
-    const uint32_t up = build_climb(a, d, m, a.entries[d.primOffset + regionLo + e], regionLo, regionHi);
+    uint32_t j = 0, jh = nsub;  // the sub-tile j with s_pre[j] <= e < s_pre[j + 1]
+    while (jh - j > 1) {
+      const uint32_t mid = (j + jh) >> 1;
+      if (s_pre[mid] <= e) j = mid;
+      else jh = mid;
+    }
+    const uint32_t entry = a.entries[d.primOffset + regionLo + j * kBuildSub + (e - s_pre[j])];
+    const uint32_t up = build_climb(a, d, m, entry, regionLo, regionHi);
     if (up != kNoEdge) {
       const uint32_t slot = atomicAdd(&a.edgeCount[m], 1u);
       if (slot < d.ntri) a.entriesTop[d.primOffset + slot] = up;
@@ -799,7 +869,7 @@ BuildArgs args_of(BVHStore *st) {
   a.entries = st->entries;
   a.pending = (Pending *)st->pending;
   a.entriesTop = st->entriesTop;
-  a.regionCount = st->regionCount;
+  a.subCount = st->subCount;
   a.edgeCount = st->edgeCount;
   a.passes = st->passes;
   a.digitBits = st->cfg.digitBits;
@@ -935,7 +1005,7 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   const size_t statWords = (size_t)passes * T * R;
   const size_t tickWords = 64;
   const size_t edgeWords = M;
-  const size_t regionWords = T;
+  const size_t regionWords = (size_t)T * ((size_t)cfg.tileKeys / kBuildSub);
   const size_t zeroWords = cntWords + histWords + statWords + tickWords + edgeWords + regionWords;
   off += al(zeroWords * 4);
   if (off > st->blockBytes) {
@@ -971,7 +1041,7 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->status = st->hist + histWords;
   st->tickets = st->status + statWords;
   st->edgeCount = st->tickets + tickWords;
-  st->regionCount = st->edgeCount + edgeWords;
+  st->subCount = st->edgeCount + edgeWords;
   st->desc = desc;
   st->tileMesh = tileMesh;
   return true;
@@ -1116,7 +1186,6 @@ bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   const MeshDesc &d = st->desc[acc->mesh];
   if (d.ntri > 1) fill_u32(ctx, s, st->counters + d.innerOffset, 0u, d.ntri - 1);
   fill_u32(ctx, s, st->edgeCount + acc->mesh, 0u, 1);
-  fill_u32(ctx, s, st->regionCount + d.tileOffset, 0u, d.ntiles);
   const BuildArgs a = args_of(st);
   launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u);
   return true;
