@@ -80,18 +80,25 @@ __device__ __forceinline__ TileRange tile_range(const BuildArgs &a, const MeshDe
   return r;
 }
 
+// Geometry pointers come from the mesh table; they are device (global) memory, and saying so lets
+// the loads be global rather than flat loads.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr(const char *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+
 // Vertex indices are clamped to the vertex count, so a malformed index buffer yields a wrong
 // box rather than an out-of-bounds read.
 __device__ __forceinline__ float3 load_vertex(const MeshDesc &d, uint32_t i) {
   i = i < d.nverts ? i : d.nverts - 1;
-  const float *p = (const float *)(d.verts + (size_t)i * d.vstride);
+  const auto *p = global_ptr<float>(d.verts + (size_t)i * d.vstride);
   return make_float3(p[0], p[1], p[2]);
 }
 
 // primIDs are clamped to the mesh as well, so no gather can leave the caller's buffers.
 __device__ __forceinline__ void load_triangle(const MeshDesc &d, uint32_t prim, float3 &a, float3 &b, float3 &c) {
   prim = prim < d.ntri ? prim : d.ntri - 1;
-  const uint32_t *p = (const uint32_t *)(d.idx + (size_t)prim * d.istride);
+  const auto *p = global_ptr<uint32_t>(d.idx + (size_t)prim * d.istride);
   a = load_vertex(d, p[0]);
   b = load_vertex(d, p[1]);
   c = load_vertex(d, p[2]);
@@ -539,16 +546,29 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   float *__restrict__ header = a.headers + (size_t)m * 16;
   const uint32_t tid = threadIdx.x;
   const uint32_t k = lo + tid;  // leaf position
-  if (k < hi) s_delta[tid + 1] = key_delta(codes, ids, k, n);
-  if (tid == 0) s_delta[0] = key_delta(codes, ids, lo - 1u, n);
+  // The leaf's key pair (its delta) and primID are loaded in one round trip, then its triangle.
+  uint32_t prim = 0, ck = 0, cn = 0, in = 0, cp = 0, ip = 0;
+  if (k < hi) {
+    prim = ids[k];
+    ck = codes[k];
+    if (k + 1 < n) {
+      cn = codes[k + 1];
+      in = ids[k + 1];
+    }
+  }
+  if (tid == 0 && lo > 0) {
+    cp = codes[lo - 1];
+    ip = ids[lo - 1];
+  }
   if (tid < 4) s_nready[tid] = 0u;
   s_meet[tid] = 0u;
   s_split[tid] = 0xFFFFu;
   float box[6];
   if (k < hi) {
-    const uint32_t prim = ids[k];
     float3 p, q, s;
     load_triangle(d, prim, p, q, s);
+    s_delta[tid + 1] = k + 1 < n ? __clzll((((uint64_t)ck << 32) | prim) ^ (((uint64_t)cn << 32) | in)) : -1;
+    if (tid == 0) s_delta[0] = lo > 0 ? __clzll((((uint64_t)cp << 32) | ip) ^ (((uint64_t)ck << 32) | prim)) : -1;
     TriRecord rec;
     rec.v0 = make_float4(p.x, p.y, p.z, __uint_as_float(prim));
     rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
