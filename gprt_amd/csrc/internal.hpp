@@ -142,10 +142,12 @@ struct BVHStore {
   uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into splits on region edges
   uint32_t *edgeCount = nullptr;    // [meshes] (in the zeroed range)
   uint32_t *subCount = nullptr;     // [tiles * tileKeys / 1024]: edges of each 1024-key sub-tile
+  // Child arrivals per split: zeroed when the arrays are laid out; every build leaves them zero.
+  uint32_t *counters = nullptr;  // [inner]
+  size_t cntWords = 0;
   // Zeroed at the start of every build (one contiguous range):
   uint32_t *zeroBase = nullptr;
   size_t zeroWords = 0;
-  uint32_t *counters = nullptr;  // [inner] child arrivals per split
   uint32_t *hist = nullptr;      // [meshes][passes][R]
   uint32_t *status = nullptr;    // [passes][tiles][R]
   uint32_t *tickets = nullptr;   // [passes]

@@ -120,7 +120,7 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Init: zero counters / histograms / look-back state; box accumulators to min/max identities.
+// Init: zero histograms / look-back state / edge counts; box accumulators to min/max identities.
 // ------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_init(uint32_t *__restrict__ zero, size_t words, float *__restrict__ headers,
                                               uint32_t meshes) {
@@ -717,6 +717,9 @@ __device__ __forceinline__ uint32_t build_climb(const BuildArgs &a, const MeshDe
     if (split < rangeLo || split + 1u >= rangeHi) return edge;
     const uint32_t old = __hip_atomic_fetch_add(&counters[split], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old != 1u) return kNoEdge;  // first arrival
+    // Both children are in: no one touches this counter again in this build, so it goes back to
+    // zero here and the next build needs no clearing pass.
+    counters[split] = 0u;
     float b0[6], b1[6];
     int c0, c1;
     uint32_t pl, pr;
@@ -1019,14 +1022,16 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   off += al((size_t)prims * 4);
   const size_t oPend = off;
   off += al((size_t)std::max<uint32_t>(I, 1) * 2 * sizeof(Pending));
-  const size_t oZero = off;
   const size_t cntWords = std::max<uint32_t>(I, 1);
+  const size_t oCnt = off;  // zero after every build (each build returns them to zero)
+  off += al(cntWords * 4);
+  const size_t oZero = off;
   const size_t histWords = (size_t)M * passes * R;
   const size_t statWords = (size_t)passes * T * R;
   const size_t tickWords = 64;
   const size_t edgeWords = M;
   const size_t regionWords = (size_t)T * ((size_t)cfg.tileKeys / kBuildSub);
-  const size_t zeroWords = cntWords + histWords + statWords + tickWords + edgeWords + regionWords;
+  const size_t zeroWords = histWords + statWords + tickWords + edgeWords + regionWords;
   off += al(zeroWords * 4);
   if (off > st->blockBytes) {
     free_block(st);
@@ -1056,8 +1061,9 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->entriesTop = (uint32_t *)(b + oEntTop);
   st->zeroBase = (uint32_t *)(b + oZero);
   st->zeroWords = zeroWords;
-  st->counters = st->zeroBase;
-  st->hist = st->counters + cntWords;
+  st->counters = (uint32_t *)(b + oCnt);
+  st->cntWords = cntWords;
+  st->hist = st->zeroBase;
   st->status = st->hist + histWords;
   st->tickets = st->status + statWords;
   st->edgeCount = st->tickets + tickWords;
@@ -1149,6 +1155,7 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
     }
     st->graphValid = false;
     if (!store_layout(st, desc, tileMesh, cfg, (uint32_t)prims)) return false;
+    fill_u32(ctx, s, st->counters, 0u, st->cntWords);
     if (!hip_ok(hipMemcpyAsync(st->dDesc, st->desc.data(), desc.size() * sizeof(MeshDesc), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync(mesh table)"))
       return false;
@@ -1204,7 +1211,6 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
 bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   BVHStore *st = acc->store;
   const MeshDesc &d = st->desc[acc->mesh];
-  if (d.ntri > 1) fill_u32(ctx, s, st->counters + d.innerOffset, 0u, d.ntri - 1);
   fill_u32(ctx, s, st->edgeCount + acc->mesh, 0u, 1);
   const BuildArgs a = args_of(st);
   launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u);
