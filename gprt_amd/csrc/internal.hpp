@@ -135,6 +135,7 @@ struct BVHStore {
   uint32_t *codes = nullptr, *ids = nullptr, *codesTmp = nullptr, *idsTmp = nullptr;
   void *nodes = nullptr;  // BVHNode[inner]
   void *tris = nullptr;   // TriRecord[prims]
+  void *centroids = nullptr;  // float[prims][3]: centroids in primitive order (written by the bounds pass)
   uint32_t *parentLeaf = nullptr, *parentInner = nullptr;  // filled on demand by lbvh_parents
   float *headers = nullptr;  // [meshes][16]
   uint32_t *entries = nullptr;  // [prims] per-region lists of build edges into splits on sub-tile edges
