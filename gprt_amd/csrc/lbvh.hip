@@ -220,7 +220,6 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
     const float qz = fminf(fmaxf((cz - loz) / ez * 1024.0f, 0.0f), 1023.0f);
     const uint32_t code = morton30((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
     a.codes[d.primOffset + i] = code;
-    a.ids[d.primOffset + i] = i;
     if (multi) {
       for (int ps = 0; ps < a.passes; ++ps) {
         const int shift = ps * BITS;
@@ -302,14 +301,15 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
   const bool single = d.ntiles == 1;
 
   // Keys and payloads are loaded together up front, so the payload loads overlap the ranking and
-  // the look-back; digits are recomputed from the keys (fewer live registers).
+  // the look-back; digits are recomputed from the keys (fewer live registers). The first pass's
+  // input is in primitive order, so its payload (the primID) is the key's position.
   auto digit = [&](uint32_t x) { return (x >> shift) & dmask; };
   uint32_t k[ITEMS], v[ITEMS], pos[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const uint32_t q = (uint32_t)(w * ITEMS * 64 + j * 64 + l);
     k[j] = q < valid ? src[base + q] : 0xFFFFFFFFu;
-    v[j] = q < valid ? vsrc[base + q] : 0u;
+    v[j] = q < valid ? (pass == 0 ? lo + q : vsrc[base + q]) : 0u;
   }
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
