@@ -104,7 +104,12 @@ __global__ void __launch_bounds__(256) k_encode(const GPRTRay *__restrict__ rays
 // Generic bf16 GEMM on MFMA 16x16x32 with FP32 accumulation:
 //   C[m][n] = alpha * sum_k A(m,k) B(k,n) (+ bias[n]) (* (mask(m,n) > 0)) (relu) (+= C)
 // A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]: any transposition through strides.
-// Workgroup tile 64x64 (4 waves of 32x32 = 2x2 MFMA tiles), K step 32 staged in LDS.
+// Workgroup tile 64x64 (4 waves of 32x32 = 2x2 MFMA tiles), K step 64 staged in LDS. Each operand
+// is read 16 bytes at a time along whichever of its strides is 1 (K-contiguous rows are copied,
+// M/N-contiguous columns are transposed into LDS); other layouts, unaligned operands and tile
+// edges use element loads. Split K (split > 1): workgroup z covers the K range
+// [z*kPer, (z+1)*kPer) and writes its partial sums to ws[z][M][N]; k_gemm_reduce adds the splits
+// in order and applies the epilogue, so the result does not depend on scheduling.
 // ------------------------------------------------------------------------------------------------
 struct GemmArgs {
   const uint16_t *A;
@@ -121,38 +126,111 @@ struct GemmArgs {
   int relu;
   int accumulate;
   float alpha;
+  int aMode, bMode;  // 0: element loads; 1: K-contiguous 16-byte loads; 2: M/N-contiguous 16-byte loads
+  int split, kPer;
+  float *ws;
 };
 
+constexpr int kGK = 64;    // K step
+constexpr int kGPad = 72;  // LDS row length in elements (16-byte aligned rows)
+
+// S[row][k] = P(r0 + row, k0 + k) for a 64 x 64 block, where P(r, k) = P[r*srow + k*sk], zero
+// outside [0, rows) x [k0, kEnd).
+__device__ __forceinline__ void gemm_stage(const uint16_t *__restrict__ P, long long srow, long long sk, int mode,
+                                           int rows, int r0, int k0, int kEnd, uint16_t (*S)[kGPad], int tid) {
+  if (mode == 1) {
+#pragma unroll
+    for (int c = tid; c < 512; c += 256) {
+      const int row = c >> 3, kc = (c & 7) * 8;
+      const int r = r0 + row, k = k0 + kc;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (r < rows && k + 8 <= kEnd) {
+        v = *(const uint4 *)(P + (long long)r * srow + k);
+      } else if (r < rows) {
+        uint16_t e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = k + j < kEnd ? P[(long long)r * srow + k + j] : (uint16_t)0;
+        v = make_uint4(e[0] | (uint32_t)e[1] << 16, e[2] | (uint32_t)e[3] << 16, e[4] | (uint32_t)e[5] << 16,
+                       e[6] | (uint32_t)e[7] << 16);
+      }
+      *(uint4 *)&S[row][kc] = v;
+    }
+  } else if (mode == 2) {
+#pragma unroll
+    for (int c = tid; c < 512; c += 256) {
+      const int kk = c >> 3, rc = (c & 7) * 8;
+      const int r = r0 + rc, k = k0 + kk;
+      uint16_t e[8];
+      if (k < kEnd && r + 8 <= rows) {
+        const uint4 v = *(const uint4 *)(P + r + (long long)k * sk);
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          e[2 * j] = (uint16_t)(w4[j] & 0xFFFFu);
+          e[2 * j + 1] = (uint16_t)(w4[j] >> 16);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = (k < kEnd && r + j < rows) ? P[r + j + (long long)k * sk] : (uint16_t)0;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) S[rc + j][kk] = e[j];
+    }
+  } else {
+    const int row = tid >> 2, kq = (tid & 3) * 16;
+    const int r = r0 + row;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int k = k0 + kq + j;
+      S[row][kq + j] = (r < rows && k < kEnd) ? P[(long long)r * srow + (long long)k * sk] : (uint16_t)0;
+    }
+  }
+}
+
+__device__ __forceinline__ void gemm_epilogue(const GemmArgs &g, int m, int n, float sum) {
+  float v = g.alpha * sum;
+  if (g.bias) v += g.bias[n];
+  if (g.mask && !(bf2f(g.mask[(long long)m * g.smm + (long long)n * g.smn]) > 0.f)) v = 0.f;
+  if (g.relu) v = fmaxf(v, 0.f);
+  const long long o = (long long)m * g.scm + (long long)n * g.scn;
+  if (g.Cf) {
+    if (g.accumulate)
+      g.Cf[o] += v;
+    else
+      g.Cf[o] = v;
+  }
+  if (g.Cb) g.Cb[o] = f2bf(v);
+}
+
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][kGPad];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][kGPad];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int kBeg = blockIdx.z * g.kPer, kEnd = min(g.K, kBeg + g.kPer);
   f32x4_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  const int r = tid >> 2, kk = (tid & 3) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += 32) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int m = m0 + r, k = k0 + kk + j, n = n0 + r;
-      As[r][kk + j] = (m < g.M && k < g.K) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : (uint16_t)0;
-      Bs[r][kk + j] = (n < g.N && k < g.K) ? g.B[(long long)k * g.sbk + (long long)n * g.sbn] : (uint16_t)0;
-    }
+  for (int k0 = kBeg; k0 < kEnd; k0 += kGK) {
+    gemm_stage(g.A, g.sam, g.sak, g.aMode, g.M, m0, k0, kEnd, As, tid);
+    gemm_stage(g.B, g.sbn, g.sbk, g.bMode, g.N, n0, k0, kEnd, Bs, tid);
     __syncthreads();
-    Frag a[2], b[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      a[i].u = *(const uint4 *)&As[wm + i * 16 + (l & 15)][8 * (l >> 4)];
-      b[i].u = *(const uint4 *)&Bs[wn + i * 16 + (l & 15)][8 * (l >> 4)];
+    for (int ks = 0; ks < kGK; ks += 32) {
+      Frag a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i].u = *(const uint4 *)&As[wm + i * 16 + (l & 15)][ks + 8 * (l >> 4)];
+        b[i].u = *(const uint4 *)&Bs[wn + i * 16 + (l & 15)][ks + 8 * (l >> 4)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
     }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
     __syncthreads();
   }
 #pragma unroll
@@ -164,19 +242,21 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         const int m = m0 + wm + i * 16 + 4 * (l >> 4) + q;
         const int n = n0 + wn + j * 16 + (l & 15);
         if (m >= g.M || n >= g.N) continue;
-        float v = g.alpha * acc[i][j][q];
-        if (g.bias) v += g.bias[n];
-        if (g.mask && !(bf2f(g.mask[(long long)m * g.smm + (long long)n * g.smn]) > 0.f)) v = 0.f;
-        if (g.relu) v = fmaxf(v, 0.f);
-        const long long o = (long long)m * g.scm + (long long)n * g.scn;
-        if (g.Cf) {
-          if (g.accumulate)
-            g.Cf[o] += v;
-          else
-            g.Cf[o] = v;
-        }
-        if (g.Cb) g.Cb[o] = f2bf(v);
+        if (g.split > 1)
+          g.ws[((long long)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][q];
+        else
+          gemm_epilogue(g, m, n, acc[i][j][q]);
       }
+}
+
+// Sums the split-K partials in split order, then applies the epilogue.
+__global__ void __launch_bounds__(256) k_gemm_reduce(GemmArgs g) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long mn = (long long)g.M * g.N;
+  if (i >= mn) return;
+  float sum = 0.f;
+  for (int z = 0; z < g.split; ++z) sum += g.ws[z * mn + i];
+  gemm_epilogue(g, (int)(i / g.N), (int)(i % g.N), sum);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -560,9 +640,30 @@ __global__ void __launch_bounds__(256) k_labels_of_leaves(const int32_t *__restr
 
 unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-void gemm(Context *ctx, hipStream_t s, const GemmArgs &g, const char *name) {
-  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
+// Picks each operand's load mode and the K split, then launches (and reduces the splits).
+void gemm(Context *ctx, hipStream_t s, GemmArgs g, const char *name, const Proposal *p) {
+  auto vec_ok = [](const void *ptr, long long otherStride) {
+    return ((uintptr_t)ptr & 15u) == 0 && otherStride % 8 == 0;
+  };
+  g.aMode = (g.sak == 1 && vec_ok(g.A, g.sam)) ? 1 : (g.sam == 1 && vec_ok(g.A, g.sak)) ? 2 : 0;
+  g.bMode = (g.sbk == 1 && vec_ok(g.B, g.sbn)) ? 1 : (g.sbn == 1 && vec_ok(g.B, g.sbk)) ? 2 : 0;
+  const long long tiles = (long long)((g.N + 63) / 64) * ((g.M + 63) / 64);
+  // Few output tiles over a long K (weight gradients): split K so ~512 workgroups run, each over
+  // at least 256 of K, as far as the workspace holds the partials.
+  int split = 1;
+  if (tiles < 256 && g.K >= 512 && p->gemmWs) {
+    split = (int)std::min<long long>(std::max<long long>(512 / tiles, 1), g.K / 256);
+    while (split > 1 && (size_t)split * g.M * g.N * 4 > p->gemmWsBytes) --split;
+  }
+  g.kPer = ((g.K + split - 1) / split + kGK - 1) / kGK * kGK;
+  g.split = (g.K + g.kPer - 1) / g.kPer;
+  if (g.split < 1) g.split = 1;
+  g.ws = p->gemmWs;
+  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, g.split);
   launch(ctx, s, name, k_gemm, grid, dim3(256), 0, g);
+  if (g.split > 1)
+    launch(ctx, s, "mlp_gemm_reduce", k_gemm_reduce, dim3((unsigned)(((long long)g.M * g.N + 255) / 256)), dim3(256),
+           0, g);
 }
 
 GemmArgs gemm_args(const uint16_t *A, long long sam, long long sak, const uint16_t *B, long long sbk, long long sbn,
@@ -623,6 +724,9 @@ bool mlp_alloc(Context *ctx, Proposal *p, uint32_t maxBatch) {
   off += al(B * kMlpOut * 2);
   const size_t oL = off;
   off += al(B * 4);
+  const size_t oWs = off;  // split-K partial sums of the training GEMMs
+  const size_t wsBytes = (size_t)16 << 20;
+  off += al(wsBytes);
   if (!hip_ok(hipMalloc(&p->block, off), "hipMalloc(proposal)")) return false;
   p->blockBytes = off;
   char *b = (char *)p->block;
@@ -640,6 +744,8 @@ bool mlp_alloc(Context *ctx, Proposal *p, uint32_t maxBatch) {
   p->Z = (float *)(b + oZ);
   p->dZ = (uint16_t *)(b + oDZ);
   p->labels = (int32_t *)(b + oL);
+  p->gemmWs = (float *)(b + oWs);
+  p->gemmWsBytes = wsBytes;
   p->maxBatch = maxBatch;
   // Map from flat fp32 parameter index to its bf16 copy (weights only).
   std::vector<uint32_t> map(P, 0xFFFFFFFFu);
@@ -698,7 +804,7 @@ void mlp_forward_batch(Context *ctx, hipStream_t s, Proposal *p, const void *ray
     g.scn = 1;
     g.bias = p->w + mlp_bias_offset(layer);
     g.relu = 1;
-    gemm(ctx, s, g, "mlp_gemm_fwd");
+    gemm(ctx, s, g, "mlp_gemm_fwd", p);
     in = out;
     inStride = kMlpHid;
   }
@@ -707,7 +813,7 @@ void mlp_forward_batch(Context *ctx, hipStream_t s, Proposal *p, const void *ray
   g.scm = kMlpOut;
   g.scn = 1;
   g.bias = p->w + mlp_bias_offset(3);
-  gemm(ctx, s, g, "mlp_gemm_fwd");
+  gemm(ctx, s, g, "mlp_gemm_fwd", p);
 }
 
 // Gradients of the mean cross-entropy over the labelled rays of one batch (into p->g).
@@ -724,7 +830,7 @@ void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, ui
     g.Cf = p->g + mlp_weight_offset(3);
     g.scm = kMlpHid;
     g.scn = 1;
-    gemm(ctx, s, g, "mlp_gemm_bwd");
+    gemm(ctx, s, g, "mlp_gemm_bwd", p);
     launch(ctx, s, "mlp_colsum", k_colsum, dim3(blocks_for(kMlpOut, 256), blocks_for(n, 256)), dim3(256), 0,
            (const uint16_t *)p->dZ, (int)n, kMlpOut, kMlpOut, p->g + mlp_bias_offset(3));
     GemmArgs h = gemm_args(p->dZ, kMlpOut, 1, mlp_wbf(p, 3), kMlpHid, 1, (int)n, kMlpHid, kMlpOut);
@@ -734,7 +840,7 @@ void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, ui
     h.mask = H[2];
     h.smm = kMlpHid;
     h.smn = 1;
-    gemm(ctx, s, h, "mlp_gemm_bwd");
+    gemm(ctx, s, h, "mlp_gemm_bwd", p);
   }
   for (int layer = 2; layer >= 0; --layer) {
     const uint16_t *dA = p->dA + (size_t)layer * B * kMlpHid;
@@ -744,7 +850,7 @@ void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, ui
     g.Cf = p->g + mlp_weight_offset(layer);
     g.scm = K;
     g.scn = 1;
-    gemm(ctx, s, g, "mlp_gemm_bwd");
+    gemm(ctx, s, g, "mlp_gemm_bwd", p);
     launch(ctx, s, "mlp_colsum", k_colsum, dim3(blocks_for(kMlpHid, 256), blocks_for(n, 256)), dim3(256), 0, dA,
            (int)n, kMlpHid, kMlpHid, p->g + mlp_bias_offset(layer));
     if (layer > 0) {
@@ -755,7 +861,7 @@ void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, ui
       h.mask = H[layer - 1];
       h.smm = kMlpHid;
       h.smn = 1;
-      gemm(ctx, s, h, "mlp_gemm_bwd");
+      gemm(ctx, s, h, "mlp_gemm_bwd", p);
     }
   }
 }

@@ -40,6 +40,8 @@ struct Proposal {
   uint16_t *X = nullptr, *H = nullptr, *dA = nullptr, *dZ = nullptr;
   float *Z = nullptr;
   int32_t *labels = nullptr;
+  float *gemmWs = nullptr;  // split-K partial sums of the training GEMMs
+  size_t gemmWsBytes = 0;
   uint32_t step = 0;
   // Treelets of the accel the proposal serves.
   void *treeBlock = nullptr;
