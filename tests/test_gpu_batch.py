@@ -8,7 +8,8 @@ from gprt_amd import api, scenes
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [1, 2, 3, 64, 100, 777, 2048, 4095, 4096, 4097, 8191, 8193, 20000, 65536, 3, 1000]
+# 65536 fills one hierarchy region exactly; 150001 spans three (edges through the region and top levels)
+SIZES = [1, 2, 3, 64, 100, 777, 2048, 4095, 4096, 4097, 8191, 8193, 20000, 65536, 3, 1000, 150001]
 
 
 def _meshes(seed0):

@@ -64,7 +64,8 @@ def test_lbvh_shared_vertices_and_flat_axis(ctx, orc):
 
 
 def test_lbvh_update_refit(ctx, orc):
-    verts, idx = scenes.blob_scene(50_000, seed=6)
+    # 200,000 triangles: the refit also runs the region and top hierarchy levels
+    verts, idx = scenes.blob_scene(200_000, seed=6)
     acc = _build(ctx, verts, idx, api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
     keys0, nodes0, _ = acc.export()
     moved = (verts + scenes.rand01(7, 0, verts.size).reshape(verts.shape).astype(np.float32) * np.float32(2e-3)
