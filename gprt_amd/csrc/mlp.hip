@@ -266,6 +266,7 @@ __global__ void __launch_bounds__(256) k_gemm_reduce(GemmArgs g) {
 __global__ void __launch_bounds__(256) k_softmax_ce(const float *__restrict__ Z, const int32_t *__restrict__ labels,
                                                     uint16_t *__restrict__ dZ, float scale, float *__restrict__ loss,
                                                     uint32_t *__restrict__ correct) {
+  static_assert(kMlpOut == 4096, "one row is 16 logits per thread");
   const uint32_t row = blockIdx.x;
   const int tid = threadIdx.x;
   const float *z = Z + (size_t)row * kMlpOut;
@@ -273,12 +274,23 @@ __global__ void __launch_bounds__(256) k_softmax_ce(const float *__restrict__ Z,
   const int y = labels[row];
   __shared__ float s_red[4];
   __shared__ int s_idx[4];
+  // The row is read once: thread tid holds logits j*1024 + 4*tid + q (increasing with j, q).
+  float v[16];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float4 f = *(const float4 *)(z + j * 1024 + 4 * tid);
+    v[4 * j] = f.x;
+    v[4 * j + 1] = f.y;
+    v[4 * j + 2] = f.z;
+    v[4 * j + 3] = f.w;
+  }
   float mx = -INFINITY;
   int amx = 0;
-  for (int c = tid; c < kMlpOut; c += 256) {
-    if (z[c] > mx) {
-      mx = z[c];
-      amx = c;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (v[i] > mx) {
+      mx = v[i];
+      amx = (i >> 2) * 1024 + 4 * tid + (i & 3);
     }
   }
 #pragma unroll
@@ -303,18 +315,28 @@ __global__ void __launch_bounds__(256) k_softmax_ce(const float *__restrict__ Z,
       amx = s_idx[w];
     }
   __syncthreads();
-  float sum = 0.f;
-  for (int c = tid; c < kMlpOut; c += 256) sum += expf(z[c] - mx);
+  float e[16], sum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    e[i] = expf(v[i] - mx);
+    sum += e[i];
+  }
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 64);
   if ((tid & 63) == 0) s_red[tid >> 6] = sum;
   __syncthreads();
   sum = s_red[0] + s_red[1] + s_red[2] + s_red[3];
   const float inv = 1.0f / sum;
-  for (int c = tid; c < kMlpOut; c += 256) {
-    float g = 0.f;
-    if (y >= 0) g = (expf(z[c] - mx) * inv - (c == y ? 1.f : 0.f)) * scale;
-    dz[c] = f2bf(g);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint16_t o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = j * 1024 + 4 * tid + q;
+      const float g = y >= 0 ? (e[4 * j + q] * inv - (c == y ? 1.f : 0.f)) * scale : 0.f;
+      o[q] = f2bf(g);
+    }
+    *(uint2 *)(dz + j * 1024 + 4 * tid) = make_uint2(o[0] | (uint32_t)o[1] << 16, o[2] | (uint32_t)o[3] << 16);
   }
   if (tid == 0 && y >= 0) {
     atomicAdd(loss, (logf(sum) + mx - z[y]) * scale);
@@ -322,16 +344,50 @@ __global__ void __launch_bounds__(256) k_softmax_ce(const float *__restrict__ Z,
   }
 }
 
-// Column sums (bias gradients): out[n] += sum_m A[m*lda + n] (bf16 input). Blocks cover 256
-// columns x 256 rows; partial sums meet in FP32 atomics (out is zeroed beforehand).
-__global__ void __launch_bounds__(256) k_colsum(const uint16_t *__restrict__ A, int M, int N, int lda,
-                                                float *__restrict__ out) {
+// Column sums (bias gradients), deterministic in two passes: k_colsum_part sums a chunk of rows
+// for 64 columns per workgroup (8 columns per thread with 16-byte loads when `vec`; 32 row lanes
+// reduced in LDS in a fixed order) into part[chunk][n]; k_colsum_fin adds the chunks in order.
+__global__ void __launch_bounds__(256) k_colsum_part(const uint16_t *__restrict__ A, int M, int N, int lda,
+                                                     int rowsPerChunk, int vec, float *__restrict__ part) {
+  __shared__ float s[32][65];
+  const int cg = threadIdx.x & 7, rl = threadIdx.x >> 3;
+  const int n0 = blockIdx.x * 64 + cg * 8;
+  const int mBeg = blockIdx.y * rowsPerChunk, mEnd = min(M, mBeg + rowsPerChunk);
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int m = mBeg + rl; m < mEnd; m += 32) {
+    const uint16_t *row = A + (size_t)m * lda;
+    if (vec && n0 + 8 <= N) {
+      const uint4 u = *(const uint4 *)(row + n0);
+      const uint32_t w4[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[2 * j] += bf2f((uint16_t)(w4[j] & 0xFFFFu));
+        acc[2 * j + 1] += bf2f((uint16_t)(w4[j] >> 16));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (n0 + j < N) acc[j] += bf2f(row[n0 + j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[rl][cg * 8 + j] = acc[j];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    float t = 0.f;
+    for (int r = 0; r < 32; ++r) t += s[r][threadIdx.x];
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n < N) part[(size_t)blockIdx.y * N + n] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_colsum_fin(const float *__restrict__ part, int chunks, int N,
+                                                    float *__restrict__ out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
-  const int m0 = blockIdx.y * 256, m1 = min(M, m0 + 256);
-  float s = 0.f;
-  for (int m = m0; m < m1; ++m) s += bf2f(A[(size_t)m * lda + n]);
-  atomicAdd(&out[n], s);
+  float t = 0.f;
+  for (int c = 0; c < chunks; ++c) t += part[(size_t)c * N + n];
+  out[n] = t;
 }
 
 // Adam (bias-corrected) on the fp32 parameters; refreshes the bf16 weight copies.
@@ -640,6 +696,18 @@ __global__ void __launch_bounds__(256) k_labels_of_leaves(const int32_t *__restr
 
 unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// out[n] = sum over the M rows of A (bf16, row stride lda), through the GEMM workspace.
+void colsum(Context *ctx, hipStream_t s, const uint16_t *A, int M, int N, int lda, float *out, const Proposal *p) {
+  int rows = 256;
+  while ((size_t)((M + rows - 1) / rows) * N * 4 > p->gemmWsBytes) rows *= 2;
+  const int chunks = std::max(1, (M + rows - 1) / rows);
+  const int vec = ((uintptr_t)A & 15u) == 0 && lda % 8 == 0;
+  launch(ctx, s, "mlp_colsum", k_colsum_part, dim3((N + 63) / 64, chunks), dim3(256), 0, A, M, N, lda, rows, vec,
+         p->gemmWs);
+  launch(ctx, s, "mlp_colsum", k_colsum_fin, dim3(blocks_for(N, 256)), dim3(256), 0, (const float *)p->gemmWs, chunks,
+         N, out);
+}
+
 // Picks each operand's load mode and the K split, then launches (and reduces the splits).
 void gemm(Context *ctx, hipStream_t s, GemmArgs g, const char *name, const Proposal *p) {
   auto vec_ok = [](const void *ptr, long long otherStride) {
@@ -819,7 +887,7 @@ void mlp_forward_batch(Context *ctx, hipStream_t s, Proposal *p, const void *ray
 // Gradients of the mean cross-entropy over the labelled rays of one batch (into p->g).
 void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, uint32_t labelled) {
   const float scale = labelled ? 1.0f / (float)labelled : 0.f;
-  fill_u32(ctx, s, p->g, 0u, kMlpParams);  // bias gradients accumulate; weight gradients overwrite
+  fill_u32(ctx, s, p->g, 0u, kMlpParams);  // each gradient entry is then written exactly once
   launch(ctx, s, "mlp_softmax_ce", k_softmax_ce, dim3(n), dim3(256), 0, (const float *)p->Z,
          (const int32_t *)p->labels, p->dZ, scale, p->stats, (uint32_t *)(p->stats + 1));
   const size_t B = p->maxBatch;
@@ -831,8 +899,7 @@ void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, ui
     g.scm = kMlpHid;
     g.scn = 1;
     gemm(ctx, s, g, "mlp_gemm_bwd", p);
-    launch(ctx, s, "mlp_colsum", k_colsum, dim3(blocks_for(kMlpOut, 256), blocks_for(n, 256)), dim3(256), 0,
-           (const uint16_t *)p->dZ, (int)n, kMlpOut, kMlpOut, p->g + mlp_bias_offset(3));
+    colsum(ctx, s, p->dZ, (int)n, kMlpOut, kMlpOut, p->g + mlp_bias_offset(3), p);
     GemmArgs h = gemm_args(p->dZ, kMlpOut, 1, mlp_wbf(p, 3), kMlpHid, 1, (int)n, kMlpHid, kMlpOut);
     h.Cb = p->dA + 2 * B * kMlpHid;
     h.scm = kMlpHid;
@@ -851,8 +918,7 @@ void mlp_backward_batch(Context *ctx, hipStream_t s, Proposal *p, uint32_t n, ui
     g.scm = K;
     g.scn = 1;
     gemm(ctx, s, g, "mlp_gemm_bwd", p);
-    launch(ctx, s, "mlp_colsum", k_colsum, dim3(blocks_for(kMlpHid, 256), blocks_for(n, 256)), dim3(256), 0, dA,
-           (int)n, kMlpHid, kMlpHid, p->g + mlp_bias_offset(layer));
+    colsum(ctx, s, dA, (int)n, kMlpHid, kMlpHid, p->g + mlp_bias_offset(layer), p);
     if (layer > 0) {
       GemmArgs h = gemm_args(dA, kMlpHid, 1, mlp_wbf(p, layer), kMlpInPad, 1, (int)n, kMlpHid, kMlpHid);
       h.Cb = p->dA + (size_t)(layer - 1) * B * kMlpHid;
