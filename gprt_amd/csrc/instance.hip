@@ -288,10 +288,23 @@ __device__ __forceinline__ RayPre object_ray(const TlasInst &ti, const GPRTRay &
 
 // kCheck: the instance table may be stale (a triangle accel was destroyed or moved since this
 // instance accel was built), so every use of an instance first checks its descriptor.
+// After the top-level build: each top-level leaf record (in key order) also carries its instance
+// (or -1) and triangle position in the unused w slots of v1 and v2, so a leaf test reads them with
+// the record instead of through one more dependent load.
+__global__ void __launch_bounds__(256) k_tlas_tag(TriRecord *__restrict__ topTris, uint32_t leaves,
+                                                  const int2 *__restrict__ subroots) {
+  const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= leaves) return;
+  const uint32_t k = __float_as_uint(topTris[pos].v0.w);
+  const int2 sr = k < leaves ? subroots[k] : make_int2(-1, 0);
+  topTris[pos].v1.w = __int_as_float(sr.x);
+  topTris[pos].v2.w = __int_as_float(sr.y);
+}
+
 template <bool kCheck>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace_tlas(const BVHNode *__restrict__ topNodes, const TriRecord *__restrict__ topTris, uint32_t leaves,
-                 const int2 *__restrict__ subroots, const TlasInst *__restrict__ insts,
+                 const TlasInst *__restrict__ insts,
                  const BlasDesc *__restrict__ pool, const GPRTRay *__restrict__ rays, GPRTHit *__restrict__ hits,
                  uint32_t count, uint32_t *__restrict__ work, uint32_t *__restrict__ spill,
                  const int32_t *__restrict__ proposal, const int2 *__restrict__ classNodes, uint32_t numClasses,
@@ -313,10 +326,10 @@ __global__ void __launch_bounds__(kTraceBlock)
       HitState h;
       hit_init(h, ray.tmax);
       uint32_t visits = 0;
-      // A top-level leaf: one triangle of one instance, intersected in the instance's space.
-      auto cluster = [&](int k) __attribute__((always_inline)) {
+      // A top-level leaf k: triangle sr.y of instance sr.x (-1: disabled), intersected in the
+      // instance's space.
+      auto cluster = [&](int k, int2 sr) __attribute__((always_inline)) {
         if ((uint32_t)k >= leaves) return;
-        const int2 sr = subroots[k];
         if (sr.x < 0) return;
         const TlasInst &ti = insts[sr.x];
         if (kCheck && !blas_live(pool, ti)) return;
@@ -349,7 +362,9 @@ __global__ void __launch_bounds__(kTraceBlock)
       visits += walk<kMaxStack>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip,
                                 [&](int pos, int) __attribute__((always_inline)) {
                                   if ((uint32_t)pos >= leaves) return;
-                                  cluster((int)__float_as_uint(topTris[pos].v0.w));
+                                  const TriRecord *tr = topTris + pos;
+                                  cluster((int)__float_as_uint(tr->v0.w),
+                                          make_int2(__float_as_int(tr->v1.w), __float_as_int(tr->v2.w)));
                                 });
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
@@ -700,6 +715,8 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
          (const TlasInst *)t->insts, N, t->classDepth, t->classNodes);
   Accel *top = &t->top;
   if (!lbvh_build_batch(ctx, s, &top, 1, true)) return false;
+  launch(ctx, s, "tlas_tag", k_tlas_tag, dim3((L + 255) / 256), dim3(256), 0, (TriRecord *)top->tris, L,
+         (const int2 *)t->subroots);
   a->nodes = top->nodes;
   a->tris = top->tris;
   a->header = top->header;
@@ -722,12 +739,12 @@ bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *h
   const char *name = proposal ? "trace_closest_proposal_instances" : "trace_closest_instances";
   if (checked)
     launch(ctx, s, name, k_trace_tlas<true>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
-           (const TriRecord *)t->top.tris, t->leaves, (const int2 *)t->subroots, (const TlasInst *)t->insts,
+           (const TriRecord *)t->top.tris, t->leaves, (const TlasInst *)t->insts,
            (const BlasDesc *)ctx->blasPool, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
            (const int2 *)t->classNodes, nc, instOut, leafOut, visits);
   else
     launch(ctx, s, name, k_trace_tlas<false>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
-           (const TriRecord *)t->top.tris, t->leaves, (const int2 *)t->subroots, (const TlasInst *)t->insts,
+           (const TriRecord *)t->top.tris, t->leaves, (const TlasInst *)t->insts,
            (const BlasDesc *)ctx->blasPool, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
            (const int2 *)t->classNodes, nc, instOut, leafOut, visits);
   return true;
