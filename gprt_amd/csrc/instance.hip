@@ -288,17 +288,31 @@ __device__ __forceinline__ RayPre object_ray(const TlasInst &ti, const GPRTRay &
 
 // kCheck: the instance table may be stale (a triangle accel was destroyed or moved since this
 // instance accel was built), so every use of an instance first checks its descriptor.
-// After the top-level build: each top-level leaf record (in key order) also carries its instance
-// (or -1) and triangle position in the unused w slots of v1 and v2, so a leaf test reads them with
-// the record instead of through one more dependent load.
+// After the top-level build, each top-level leaf record (in key order) is replaced by a copy of
+// its triangle record in the instance's own space: {v0, primID} {v1, instance | kLeafIdentity, or
+// -1 when disabled} {v2, top-level leaf index}. Only the parent nodes' boxes steer the top-level
+// walk, so a leaf test then reads one record instead of three dependent loads (leaf -> instance ->
+// triangle). An instance accel thus traces the triangles its instances held when it was built.
+constexpr uint32_t kLeafIdentity = 0x40000000u;
+constexpr uint32_t kLeafInstMask = 0x3FFFFFFFu;
+
 __global__ void __launch_bounds__(256) k_tlas_tag(TriRecord *__restrict__ topTris, uint32_t leaves,
-                                                  const int2 *__restrict__ subroots) {
+                                                  const int2 *__restrict__ subroots,
+                                                  const TlasInst *__restrict__ insts) {
   const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= leaves) return;
-  const uint32_t k = __float_as_uint(topTris[pos].v0.w);
+  TriRecord r = topTris[pos];
+  const uint32_t k = __float_as_uint(r.v0.w);
   const int2 sr = k < leaves ? subroots[k] : make_int2(-1, 0);
-  topTris[pos].v1.w = __int_as_float(sr.x);
-  topTris[pos].v2.w = __int_as_float(sr.y);
+  int code = -1;
+  if (sr.x >= 0) {
+    const TlasInst &ti = insts[sr.x];
+    r = ti.tris[sr.y];  // enabled leaves satisfy sr.y < ti.n
+    code = (int)((uint32_t)sr.x | ((ti.flags & kInstIdentity) ? kLeafIdentity : 0u));
+  }
+  r.v1.w = __int_as_float(code);
+  r.v2.w = __uint_as_float(k);
+  topTris[pos] = r;
 }
 
 template <bool kCheck>
@@ -326,20 +340,6 @@ __global__ void __launch_bounds__(kTraceBlock)
       HitState h;
       hit_init(h, ray.tmax);
       uint32_t visits = 0;
-      // A top-level leaf k: triangle sr.y of instance sr.x (-1: disabled), intersected in the
-      // instance's space.
-      auto cluster = [&](int k, int2 sr) __attribute__((always_inline)) {
-        if ((uint32_t)k >= leaves) return;
-        if (sr.x < 0) return;
-        const TlasInst &ti = insts[sr.x];
-        if (kCheck && !blas_live(pool, ti)) return;
-        if (ti.flags & kInstIdentity) {
-          intersect_leaf(rw, ti.tris, sr.y, ti.n, (uint32_t)sr.x, k, h);
-        } else {
-          const RayPre ro = object_ray(ti, ray);
-          intersect_leaf(ro, ti.tris, sr.y, ti.n, (uint32_t)sr.x, k, h);
-        }
-      };
       // With a proposal: the proposed class (a subtree of one instance's own tree) first, whose hit
       // tightens t; then the top-level walk, which finds that hit again and records its leaf.
       if (proposal) {
@@ -361,10 +361,21 @@ __global__ void __launch_bounds__(kTraceBlock)
       }
       visits += walk<kMaxStack>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip,
                                 [&](int pos, int) __attribute__((always_inline)) {
+                                  // A top-level leaf: one triangle of one instance (a copy of its
+                                  // record), intersected in the instance's space.
                                   if ((uint32_t)pos >= leaves) return;
-                                  const TriRecord *tr = topTris + pos;
-                                  cluster((int)__float_as_uint(tr->v0.w),
-                                          make_int2(__float_as_int(tr->v1.w), __float_as_int(tr->v2.w)));
+                                  const TriRecord tr = topTris[pos];
+                                  const int code = __float_as_int(tr.v1.w);
+                                  if (code < 0) return;
+                                  const uint32_t inst = (uint32_t)code & kLeafInstMask;
+                                  const int k = (int)__float_as_uint(tr.v2.w);
+                                  if (kCheck && !blas_live(pool, insts[inst])) return;
+                                  if ((uint32_t)code & kLeafIdentity) {
+                                    intersect_record(rw, tr, inst, k, h);
+                                  } else {
+                                    const RayPre ro = object_ray(insts[inst], ray);
+                                    intersect_record(ro, tr, inst, k, h);
+                                  }
                                 });
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
@@ -716,7 +727,7 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
   Accel *top = &t->top;
   if (!lbvh_build_batch(ctx, s, &top, 1, true)) return false;
   launch(ctx, s, "tlas_tag", k_tlas_tag, dim3((L + 255) / 256), dim3(256), 0, (TriRecord *)top->tris, L,
-         (const int2 *)t->subroots);
+         (const int2 *)t->subroots, (const TlasInst *)t->insts);
   a->nodes = top->nodes;
   a->tris = top->tris;
   a->header = top->header;

@@ -48,10 +48,9 @@ struct LaneStack {
   uint32_t *spill;  // per-lane global entries [kLdsStack, maxStack)
 };
 
-__device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord *__restrict__ tris, int leaf,
-                                               uint32_t nprims, uint32_t inst, int tag, HitState &h) {
-  if ((uint32_t)leaf >= nprims) return;
-  const TriRecord tr = tris[leaf];
+// Tests one triangle record ({v0, primID} {v1} {v2}) of instance `inst`; `tag` is recorded with a hit.
+__device__ __forceinline__ void intersect_record(const RayPre &r, const TriRecord &tr, uint32_t inst, int tag,
+                                                 HitState &h) {
   float t, u, v;
   if (tri_hit(r, tr.v0, tr.v1, tr.v2, t, u, v)) {
     const int prim = (int)__float_as_uint(tr.v0.w);
@@ -66,6 +65,12 @@ __device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord 
       h.leaf = tag;  // the same hit, first found by a walk that did not know its top-level leaf
     }
   }
+}
+
+__device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord *__restrict__ tris, int leaf,
+                                               uint32_t nprims, uint32_t inst, int tag, HitState &h) {
+  if ((uint32_t)leaf >= nprims) return;
+  intersect_record(r, tris[leaf], inst, tag, h);
 }
 
 // Depth-first walk of the subtree rooted at `start` (an inner node index, or ~leaf position) of a
