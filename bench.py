@@ -52,12 +52,15 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 11929.0e6 + 69.8e6,
-    "mlp_infer_top1": None,  # re-measured after the inference split
-    "lbvh_sort_pass": 40.0e6 + 42.6e6,
-    "build": (48.0 + 48.0 + 8.8 + 4 * (9.5 + 10.2) + 27.0 + 24.3 + 272.6 + 113.2 + 3.8 + 1.1) * N_TRIS,
-    "batched_build_per_prim": 48.0 + 48.1 + 9.3 + 4 * (9.1 + 10.2) + 27.2 + 24.4 + 177.5 + 113.0 + 4.4 + 1.2,
+    "trace_closest": 11937.0e6 + 69.8e6,
+    "mlp_infer_top1": 1325.4e6 + 16.8e6,  # one launch of 4,194,304 rays
+    "lbvh_sort_pass": 35.9e6 + 42.8e6,
+    "build": (48.0 + 12.0 + 12.0 + 4.7 + 4 * (8.5 + 10.2) + 257.0 + 113.6 + 3.7 + 1.7 + 0.1) * N_TRIS,
+    "batched_build_per_prim": 48.0 + 12.3 + 12.1 + 4.8 + 4 * (8.1 + 10.5) + 156.7 + 112.7 + 5.5 + 1.9 + 0.2,
 }
+# Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
+# commands and per-kernel table in profiles/cache_hit_rates_scene.md).
+TRACE_CACHE_HIT = {"l1": 0.819, "l2": 0.704}
 
 
 def log(msg):
@@ -220,7 +223,8 @@ def scene_workload(args):
                       "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["trace_closest"],
                       "traffic_gbs": TRAFFIC_BYTES["trace_closest"] / (trace_avg * 1e-3) / 1e9,
                       "avg_launch_ms": trace_avg,
-                      "bytes_per_ray": trace_bytes, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray}
+                      "bytes_per_ray": trace_bytes, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray,
+                      "l1_hit_rate": TRACE_CACHE_HIT["l1"], "l2_hit_rate": TRACE_CACHE_HIT["l2"]}
         log(f"cpu: build {cpu['value']:.3f} Mprims/s, trace {cpu['trace_mrays_s']:.4f} Mrays/s; "
             f"{nodes_per_ray:.1f} nodes + {tris_per_ray:.1f} tris per ray")
 
