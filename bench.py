@@ -2,7 +2,7 @@
 
 Default workload (the large single scene): 4,194,304 triangles from 64 Gaussian blobs (FP32);
 8,388,608 rays = 2048x2048 pinhole primary rays + 4,194,304 cosine-weighted bounce rays from the
-primary hits. A step is one full LBVH build (bounds, Morton keys, key sort, Karras, refit)
+primary hits. A step is one full LBVH build (bounds, Morton keys, key sort, hierarchy + boxes)
 followed by a closest-hit trace of all rays, once plainly and once with the learned treelet
 proposal (top-1 treelet from the bf16 MFMA MLP, traversed first to tighten tmax, then the exact
 traversal). Every input is resident in HBM before timing starts. Before the timed steps the
