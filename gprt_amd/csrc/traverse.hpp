@@ -84,6 +84,7 @@ template <int MaxStack, class LeafFn>
 __device__ __forceinline__ uint32_t walk(const RayPre &r, const BVHNode *__restrict__ nodes, uint32_t nprims,
                                          int start, const HitState &h, const LaneStack &st, int sp0, int skip,
                                          LeafFn &&leaf) {
+  if (nprims == 0) return 0;  // an empty tree has no nodes
   if (start < 0) {
     leaf(~start, sp0);
     return 0;
@@ -137,16 +138,111 @@ __device__ __forceinline__ uint32_t walk(const RayPre &r, const BVHNode *__restr
   return visits;
 }
 
+// Depth-first walk like walk(), with leaf tests postponed and batched across the wave: a lane
+// queues the leaf children it meets (up to kLeafQueue, in registers) and keeps walking; the wave
+// runs its leaf tests when at least half of its active lanes have leaves queued, or when a lane can
+// go no further (no node left, or no room for another node's two leaves). The triangle test is
+// long, and run per node it executes whenever any one lane of the wave needs it; batched, it
+// executes for many lanes at once. Queued leaves are tested later, so box culling sees a
+// bound that is less tight in between (a few more node fetches); the closest-hit rule makes the
+// result independent of the order of the tests. Returns the number of inner nodes fetched.
+constexpr int kLeafQueue = 4;
+
+template <int MaxStack, class LeafFn>
+__device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode *__restrict__ nodes, uint32_t nprims,
+                                                 int start, const HitState &h, const LaneStack &st, int sp0, int skip,
+                                                 LeafFn &&leaf) {
+  constexpr int kDone = INT_MIN;
+  if (nprims == 0) return 0;  // an empty tree has no nodes
+  int q0 = -1, q1 = -1, q2 = -1, q3 = -1, nq = 0;  // queued leaf positions (q0 is the next tested)
+  auto enqueue = [&](int lf) __attribute__((always_inline)) {
+    q3 = q2;
+    q2 = q1;
+    q1 = q0;
+    q0 = lf;
+    ++nq;
+  };
+  int node = start;
+  if (start < 0) {
+    enqueue(~start);
+    node = kDone;
+  }
+  int sp = sp0;
+  uint32_t visits = 0;
+  // Every iteration either fetches a node (each inner node at most once per walk) or tests all
+  // queued leaves, so the bound only guards against a malformed tree.
+  const uint32_t limit = 4u * nprims + 8u;
+  for (uint32_t it = 0; it < limit; ++it) {
+    if (node != kDone && nq <= kLeafQueue - 2) {
+      if ((uint32_t)node >= nprims - 1) {
+        node = kDone;  // only a malformed tree
+      } else {
+        const BVHNode nd = nodes[node];
+        ++visits;
+        float tn0, tn1;
+        bool h0 = box_hit(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, h.t, tn0);
+        bool h1 = box_hit(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, h.t, tn1);
+        const int c0 = nd.d.x, c1 = nd.d.y;
+        h0 = h0 && c0 != skip;
+        h1 = h1 && c1 != skip;
+        if (h0 && c0 < 0) {
+          enqueue(~c0);
+          h0 = false;
+        }
+        if (h1 && c1 < 0) {
+          enqueue(~c1);
+          h1 = false;
+        }
+        if (h0 && h1) {
+          const bool firstNear = tn0 <= tn1;
+          const int farC = firstNear ? c1 : c0;
+          if (sp < kLdsStack)
+            st.lds[sp][st.tid] = (uint32_t)farC;
+          else if (sp < MaxStack)
+            st.spill[sp - kLdsStack] = (uint32_t)farC;
+          ++sp;
+          node = firstNear ? c0 : c1;
+        } else if (h0) {
+          node = c0;
+        } else if (h1) {
+          node = c1;
+        } else if (sp == sp0 || sp > MaxStack) {
+          node = kDone;
+        } else {
+          --sp;
+          node = (int)(sp < kLdsStack ? st.lds[sp][st.tid] : st.spill[sp - kLdsStack]);
+        }
+      }
+    }
+    const bool stuck = nq > 0 && (node == kDone || nq > kLeafQueue - 2);
+    const uint64_t active = __ballot(1);
+    const uint64_t waiting = __ballot(nq > 0);
+    if (__ballot(stuck) || 2 * __popcll(waiting) >= __popcll(active)) {
+      // One call site for the triangle test: lanes drain their queues together.
+      while (nq > 0) {
+        const int lf = q0;
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        --nq;
+        leaf(lf, sp);
+      }
+    }
+    if (node == kDone && nq == 0) break;
+  }
+  return visits;
+}
+
 // Triangle-accel walk: leaves are triangles of instance `inst`; `tag` is recorded with a hit.
 template <int MaxStack>
 __device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__restrict__ nodes,
                                              const TriRecord *__restrict__ tris, uint32_t nprims, int start,
                                              HitState &h, const LaneStack &st, int sp0, uint32_t inst, int tag,
                                              int skip = kNoSkip) {
-  return walk<MaxStack>(r, nodes, nprims, start, h, st, sp0, skip,
-                        [&](int leaf, int) __attribute__((always_inline)) {
-                          intersect_leaf(r, tris, leaf, nprims, inst, tag, h);
-                        });
+  return walk_batched<MaxStack>(r, nodes, nprims, start, h, st, sp0, skip,
+                                [&](int leaf, int) __attribute__((always_inline)) {
+                                  intersect_leaf(r, tris, leaf, nprims, inst, tag, h);
+                                });
 }
 
 }  // namespace gprt
