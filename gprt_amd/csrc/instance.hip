@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(kTraceBlock)
           }
         }
       }
-      visits += walk<kMaxStack>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip,
+      visits += walk_batched<kMaxStack>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip,
                                 [&](int pos, int) __attribute__((always_inline)) {
                                   // A top-level leaf: one triangle of one instance (a copy of its
                                   // record), intersected in the instance's space.
