@@ -140,6 +140,7 @@ struct BVHStore {
   float *headers = nullptr;  // [meshes][16]
   uint32_t *entries = nullptr;  // [prims] per-region lists of build edges into splits on sub-tile edges
   void *pending = nullptr;      // [inner][2] x 32 B: child records waiting at a split (box, ref, far end)
+  void *pendDelta = nullptr;    // [inner][2] int8: the waiting child's outer key delta
   uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into splits on region edges
   uint32_t *edgeCount = nullptr;    // [meshes] (in the zeroed range)
   uint32_t *subCount = nullptr;     // [tiles * tileKeys / 1024]: edges of each 1024-key sub-tile

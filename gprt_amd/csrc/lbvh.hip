@@ -56,6 +56,8 @@ struct BuildArgs {
   float *headers;
   uint32_t *hist, *status, *tickets;
   Pending *pending;      // [inner][2]: child records waiting at a split, per side
+  int8_t *pendDelta;     // [inner][2]: each waiting child's outer delta (delta(l-1) of a left
+                         // child, delta(r) of a right child): the merged node's two deltas
   // Build edges (split << 1 | side of a child whose parent a level cannot finish), at the mesh's
   // primOffset: entries: from sub-tiles into splits on sub-tile edges, each sub-tile's list at its
   // own key positions (count at subCount[tile * subPerTile + sub]); entriesTop: from regions into
@@ -471,18 +473,11 @@ __device__ __forceinline__ void lds_barrier() {
   lds_acquire();
 }
 
-// delta(i) of the mesh's sorted keys; i = 0xFFFFFFFF stands for -1.
-__device__ __forceinline__ int key_delta(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ ids,
-                                         uint32_t i, uint32_t n) {
-  if (i >= n - 1) return -1;
-  const uint64_t x = ((uint64_t)codes[i] << 32) | ids[i];
-  const uint64_t y = ((uint64_t)codes[i + 1] << 32) | ids[i + 1];
-  return __clzll(x ^ y);
-}
-
-__device__ __forceinline__ void put_pending(Pending *p, const float b[6], int ref, uint32_t end) {
+__device__ __forceinline__ void put_pending(Pending *p, int8_t *pd, const float b[6], int ref, uint32_t end,
+                                            int outerDelta) {
   p->a = make_float4(b[0], b[1], b[2], b[3]);
   p->b = make_float4(b[4], b[5], __int_as_float(ref), __uint_as_float(end));
+  *pd = (int8_t)outerDelta;
 }
 
 __device__ __forceinline__ void get_pending(const Pending *p, float b[6], int &ref, uint32_t &end) {
@@ -553,6 +548,7 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   const uint32_t *__restrict__ ids = a.ids + d.primOffset;
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
   Pending *__restrict__ pending = a.pending + 2 * (size_t)d.innerOffset;
+  int8_t *__restrict__ pendDelta = a.pendDelta + 2 * (size_t)d.innerOffset;
   float *__restrict__ header = a.headers + (size_t)m * 16;
   const uint32_t tid = threadIdx.x;
   const uint32_t k = lo + tid;  // leaf position
@@ -608,7 +604,8 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
     const uint32_t side = leftChild ? 0u : 1u;
     const uint32_t end = leftChild ? l : rr;
     if (split < lo || split + 1u >= hi) {
-      put_pending(&pending[2 * (size_t)split + side], b, ref, end);
+      put_pending(&pending[2 * (size_t)split + side], &pendDelta[2 * (size_t)split + side], b, ref, end,
+                  leftChild ? dl : dr);
       edge = (split << 1) | side;
       return;
     }
@@ -704,7 +701,11 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
       const float *b = ref < 0 ? s_leaf[(uint32_t)(~ref) - lo] : s_box[(uint32_t)ref - lo];
       float cb[6];
       for (int c = 0; c < 6; ++c) cb[c] = b[c];
-      put_pending(&pending[2 * (size_t)(lo + tid) + side], cb, ref, s_end[tid][side]);
+      // The child's outer delta: delta(l-1) of a left child, delta(r) of a right one.
+      const uint32_t end = s_end[tid][side];
+      const int outer = side == 0 ? s_delta[end - lo] : s_delta[end - lo + 1];
+      put_pending(&pending[2 * (size_t)(lo + tid) + side], &pendDelta[2 * (size_t)(lo + tid) + side], cb, ref, end,
+                  outer);
       a.counters[d.innerOffset + lo + tid] = 1u;
     }
   }
@@ -716,10 +717,9 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
 __device__ __forceinline__ uint32_t build_climb(const BuildArgs &a, const MeshDesc &d, uint32_t m, uint32_t edge,
                                                 uint32_t rangeLo, uint32_t rangeHi) {
   const uint32_t n = d.ntri;
-  const uint32_t *__restrict__ codes = a.codes + d.primOffset;
-  const uint32_t *__restrict__ ids = a.ids + d.primOffset;
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
   Pending *__restrict__ pending = a.pending + 2 * (size_t)d.innerOffset;
+  int8_t *__restrict__ pendDelta = a.pendDelta + 2 * (size_t)d.innerOffset;
   uint32_t *__restrict__ counters = a.counters + d.innerOffset;
   for (int level = 0; level < 72; ++level) {
     const uint32_t split = edge >> 1, side = edge & 1u;
@@ -735,7 +735,7 @@ __device__ __forceinline__ uint32_t build_climb(const BuildArgs &a, const MeshDe
     uint32_t pl, pr;
     get_pending(&pending[2 * (size_t)split], b0, c0, pl);
     get_pending(&pending[2 * (size_t)split + 1], b1, c1, pr);
-    const int dl = key_delta(codes, ids, pl - 1u, n), dr = key_delta(codes, ids, pr, n);
+    const int dl = pendDelta[2 * (size_t)split], dr = pendDelta[2 * (size_t)split + 1];
     const bool root = dl < 0 && dr < 0;
     const uint32_t P = root ? 0u : (dr > dl ? pr : pl);
     write_node(&nodes[P], b0, b1, c0, c1, P == pl ? pr : pl);
@@ -748,7 +748,8 @@ __device__ __forceinline__ uint32_t build_climb(const BuildArgs &a, const MeshDe
     const bool leftChild = dr > dl;
     const uint32_t up = leftChild ? pr : pl - 1u;
     const uint32_t upSide = leftChild ? 0u : 1u;
-    put_pending(&pending[2 * (size_t)up + upSide], b0, (int)P, leftChild ? pl : pr);
+    put_pending(&pending[2 * (size_t)up + upSide], &pendDelta[2 * (size_t)up + upSide], b0, (int)P,
+                leftChild ? pl : pr, leftChild ? dl : dr);
     edge = (up << 1) | upSide;
   }
   return kNoEdge;
@@ -902,6 +903,7 @@ BuildArgs args_of(BVHStore *st) {
   a.tickets = st->tickets;
   a.entries = st->entries;
   a.pending = (Pending *)st->pending;
+  a.pendDelta = (int8_t *)st->pendDelta;
   a.entriesTop = st->entriesTop;
   a.subCount = st->subCount;
   a.edgeCount = st->edgeCount;
@@ -1035,6 +1037,8 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   off += al((size_t)prims * 4);
   const size_t oPend = off;
   off += al((size_t)std::max<uint32_t>(I, 1) * 2 * sizeof(Pending));
+  const size_t oPendD = off;
+  off += al((size_t)std::max<uint32_t>(I, 1) * 2);
   const size_t cntWords = std::max<uint32_t>(I, 1);
   const size_t oCnt = off;  // zero after every build (each build returns them to zero)
   off += al(cntWords * 4);
@@ -1072,6 +1076,7 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->headers = (float *)(b + oHdr);
   st->entries = (uint32_t *)(b + oEnt);
   st->pending = b + oPend;
+  st->pendDelta = b + oPendD;
   st->entriesTop = (uint32_t *)(b + oEntTop);
   st->zeroBase = (uint32_t *)(b + oZero);
   st->zeroWords = zeroWords;
