@@ -308,7 +308,7 @@ def batched_workload(args):
         lib.gprtAccelBuildBatch(ctx.h, ev, len(accs), params)
 
     per_mesh_ms = timed(per_mesh, args.steps) if not args.no_per_mesh else None
-    lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
+    lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)  # the default configuration
     batch_ms = timed(batch, args.steps)
     if args.no_autotune:
         tune, best, tuned_ms = {"best": None, "configs": []}, -1, batch_ms

@@ -184,7 +184,7 @@ GPRT_API void gprtBufferSortKV(GPRTContext context, GPRTBuffer keys, GPRTBuffer 
 /* Radix sort knobs used by every sort and accel build on this context (the autotuner's search
  * space): digitBits in {4, 6, 8}, tileKeys (keys per workgroup tile) in {2048, 4096, 8192},
  * wavesPerCU (0 = no cap) caps the resident waves of the accel builds' sort passes through LDS
- * padding. Results do not depend on them. */
+ * padding. Results do not depend on them. Default: 8, 8192, 0. */
 GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_t tileKeys, uint32_t wavesPerCU);
 GPRT_API void gprtGetSortConfig(GPRTContext context, uint32_t *digitBits, uint32_t *tileKeys, uint32_t *wavesPerCU);
 

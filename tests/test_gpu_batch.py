@@ -41,7 +41,7 @@ def test_batch_equals_oracle_per_mesh(ctx, orc, bits, tile):
         for a in accs:
             a.destroy()
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
 
 
 def test_batch_graph_replays_with_moving_vertices(ctx, orc):

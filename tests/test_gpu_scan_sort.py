@@ -168,7 +168,7 @@ def test_sort_stable_all_configs(ctx, orc, bits, tile):
         np.testing.assert_array_equal(k, rk)
         np.testing.assert_array_equal(v, rv)
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 4096, 0)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
 
 
 @pytest.mark.parametrize("kdt,vdt", [(np.uint32, None), (np.uint64, None), (np.uint64, np.uint32),
