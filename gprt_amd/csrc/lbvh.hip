@@ -146,14 +146,16 @@ __device__ __forceinline__ float3 centroid(float3 p, float3 q, float3 s) {
 // triangles' centroids are stored on the way, so the key kernel reads 12 bytes per triangle
 // instead of gathering the triangle again.
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_bounds(BuildArgs a, uint32_t tileBase) {
+constexpr int kBoundsThreads = 1024;  // one workgroup per tile (its six atomics per mesh stay few)
+
+__global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t tileBase) {
   const uint32_t t = tileBase + blockIdx.x;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const TileRange r = tile_range(a, d, t, m);
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kThreads) {
+  for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kBoundsThreads) {
     float3 p, q, s;
     load_triangle(d, i, p, q, s);
     const float3 c = centroid(p, q, s);
@@ -168,7 +170,7 @@ __global__ void __launch_bounds__(kThreads) k_bounds(BuildArgs a, uint32_t tileB
     mx[1] = fmax_ord(mx[1], fmax_ord(p.y, fmax_ord(q.y, s.y)));
     mx[2] = fmax_ord(mx[2], fmax_ord(p.z, fmax_ord(q.z, s.z)));
   }
-  __shared__ float sb[kThreads / 64][6];
+  __shared__ float sb[kBoundsThreads / 64][6];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(kThreads) k_bounds(BuildArgs a, uint32_t tileB
   if (threadIdx.x < 6) {
     const int c = threadIdx.x;
     float v = sb[0][c];
-    for (int ww = 1; ww < kThreads / 64; ++ww) v = c < 3 ? fmin_ord(v, sb[ww][c]) : fmax_ord(v, sb[ww][c]);
+    for (int ww = 1; ww < kBoundsThreads / 64; ++ww) v = c < 3 ? fmin_ord(v, sb[ww][c]) : fmax_ord(v, sb[ww][c]);
     uint32_t *h = (uint32_t *)a.headers + (size_t)m * 16;
     if (c < 3)
       atomicMin(&h[c], ford(v));
@@ -956,7 +958,7 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
   const int R = 1 << st->cfg.digitBits;
   launch(ctx, s, "lbvh_init", k_init, dim3(grid_for(std::max<size_t>(st->zeroWords, st->meshes * 16ull), 256, 4096)),
          dim3(256), 0, st->zeroBase, st->zeroWords, st->headers, st->meshes);
-  launch(ctx, s, "lbvh_bounds", k_bounds, dim3(st->tiles), dim3(kThreads), 0, a, 0u);
+  launch(ctx, s, "lbvh_bounds", k_bounds, dim3(st->tiles), dim3(kBoundsThreads), 0, a, 0u);
   const size_t histLds = (size_t)st->passes * R * 4;
   switch (st->cfg.digitBits) {
     case 4:
