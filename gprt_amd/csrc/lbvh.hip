@@ -1,5 +1,5 @@
-// Triangle LBVH build for gfx950: scene bounds, Morton keys, key sort, Karras split search and
-// bottom-up refit, following the stage order of GPRT's development LBVH
+// Triangle LBVH build for gfx950: scene bounds, Morton keys, key sort, then the Karras radix tree
+// and its boxes in one bottom-up pass, following the stage order of GPRT's development LBVH
 // (gprt/development/gprt_lbvh.h:170-195; kernels gprt/development/gprt_lbvh.hlsl:215-428).
 //
 // Every kernel is launched over a tile table that covers one or many meshes: tile t belongs to

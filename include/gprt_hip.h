@@ -135,8 +135,9 @@ GPRT_API void gprtBufferSortPayload(GPRTContext context, GPRTBuffer keys, GPRTBu
 /* ---------------------------------------------------------------------------------------------
  * Triangle geometry and accel (gprt/gprt_host.h:406, :476, :498, :1308, :1421, :1506, :1529,
  * :1553, :1557, :1580). The accel is a binary LBVH: 30-bit Morton keys of triangle centroids
- * normalised to the scene box, packed as morton<<32|primID, sorted, Karras split search, and a
- * bottom-up refit with per-node arrival counters. Build and update are enqueued on the stream.
+ * normalised to the scene box, packed as morton<<32|primID, sorted; the Karras radix tree (same
+ * topology and node numbering) is built bottom-up together with its boxes, every node merged once
+ * by the second of its children to arrive. Build and update are enqueued on the stream.
  * ------------------------------------------------------------------------------------------- */
 GPRT_API GPRTGeomType gprtGeomTypeCreate(GPRTContext context, GPRTGeomKind kind, size_t recordSize);
 GPRT_API void gprtGeomTypeDestroy(GPRTGeomType geomType);
@@ -189,8 +190,8 @@ GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_
 GPRT_API void gprtGetSortConfig(GPRTContext context, uint32_t *digitBits, uint32_t *tileKeys, uint32_t *wavesPerCU);
 
 /* Batched build: every accel in `accels` (each over its own triangle geometry) is built by one
- * launch sequence over a shared tile table — bounds, keys, segmented radix sort, Karras and refit
- * of all meshes together — captured into a hipGraph on the first call and replayed by later calls
+ * launch sequence over a shared tile table — bounds, keys, segmented radix sort, hierarchy and
+ * boxes of all meshes together — captured into a hipGraph on the first call and replayed by later calls
  * with the same accels and geometry shapes (vertex positions may change between calls). The
  * accels share one device store until they are rebuilt alone or destroyed. Each accel's result
  * is identical to gprtAccelBuild on it. The reference builds every accel separately with a
