@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libgprt_hip.so")
 GPRT_ERROR_RAISE = 0
 GPRT_ERROR_RECORD = 1
 
+GPRT_AABBS = 1
 GPRT_TRIANGLES = 2
 GPRT_BUILD_MODE_FAST_BUILD_NO_UPDATE = 1
 GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE = 2
@@ -96,6 +97,8 @@ _SIGS = {
     "gprtAccelBuildBatchAutotune": (C.c_uint32, [_P, C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_char_p,
                                                  C.c_size_t]),
     "gprtGeomTypeCreate": (_P, [_P, C.c_int, C.c_size_t]),
+    "gprtAABBsSetPositions": (None, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gprtAABBAccelCreate": (_P, [_P, _P, C.c_uint]),
     "gprtGeomTypeDestroy": (None, [_P]),
     "gprtGeomCreate": (_P, [_P, _P]),
     "gprtGeomDestroy": (None, [_P]),
@@ -303,6 +306,31 @@ class TriangleAccel:
             if self.owns_buffers:
                 self.vbuf.destroy()
                 self.ibuf.destroy()
+            self.h = None
+
+
+class AABBAccel(TriangleAccel):
+    """gprtGeomTypeCreate(GPRT_AABBS) + gprtGeomCreate + gprtAABBsSetPositions + gprtAABBAccelCreate
+    over boxes {minX, minY, minZ, maxX, maxY, maxZ} (float32 [n, 6])."""
+
+    def __init__(self, ctx: Context, boxes: np.ndarray):
+        self.ctx, self.lib = ctx, ctx.lib
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+        self.nt = self.nv = len(boxes)
+        self.vbuf = ctx.device_buffer(boxes)
+        self.ibuf = None
+        self.owns_buffers = True
+        self.gtype = self.lib.gprtGeomTypeCreate(ctx.h, GPRT_AABBS, 0)
+        self.geom = self.lib.gprtGeomCreate(ctx.h, self.gtype)
+        self.lib.gprtAABBsSetPositions(self.geom, self.vbuf.h, self.nt, 24, 0)
+        self.h = self.lib.gprtAABBAccelCreate(ctx.h, self.geom, 0)
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtAccelDestroy(self.h)
+            self.lib.gprtGeomDestroy(self.geom)
+            self.lib.gprtGeomTypeDestroy(self.gtype)
+            self.vbuf.destroy()
             self.h = None
 
 

@@ -19,8 +19,9 @@ GPRT_API GPRTGeomType gprtGeomTypeCreate(GPRTContext _ctx, GPRTGeomKind kind, si
     fail("gprtGeomTypeCreate: null context");
     return nullptr;
   }
-  if (kind != GPRT_TRIANGLES) {
-    fail("gprtGeomTypeCreate: only GPRT_TRIANGLES geometry has a software accel in this build (kind %d)", (int)kind);
+  if (kind != GPRT_TRIANGLES && kind != GPRT_AABBS) {
+    fail("gprtGeomTypeCreate: only GPRT_TRIANGLES and GPRT_AABBS geometry have a software accel in this build "
+         "(kind %d)", (int)kind);
     return nullptr;
   }
   GeomType *t = new GeomType{ctx, kind, recordSize};
@@ -88,6 +89,58 @@ GPRT_API void gprtTrianglesSetIndices(GPRTGeom _g, GPRTBuffer _i, uint32_t count
   g->ioffset = offset;
 }
 
+// Box primitives (gprt/gprt_host.h:614): count boxes {minX, minY, minZ, maxX, maxY, maxZ} at
+// offset + i * stride. They take the vertex slots of the geometry; the primitive count is count.
+GPRT_API void gprtAABBsSetPositions(GPRTGeom _g, GPRTBuffer _p, uint32_t count, uint32_t stride, uint32_t offset) {
+  Geom *g = (Geom *)_g;
+  Buffer *pb = (Buffer *)_p;
+  if (!g || !pb) {
+    fail("gprtAABBsSetPositions: null argument");
+    return;
+  }
+  if (g->type->kind != GPRT_AABBS) {
+    fail("gprtAABBsSetPositions: geometry is not GPRT_AABBS");
+    return;
+  }
+  if (stride < 24 || (stride & 3) || (offset & 3)) {
+    fail("gprtAABBsSetPositions: stride %u / offset %u must be 4-byte aligned with stride >= 24", stride, offset);
+    return;
+  }
+  if (count && (size_t)offset + (size_t)(count - 1) * stride + 24 > pb->size) {
+    fail("gprtAABBsSetPositions: %u boxes (stride %u, offset %u) exceed the buffer (%zu B)", count, stride, offset,
+         pb->size);
+    return;
+  }
+  g->vbuf = pb;
+  g->vcount = count;
+  g->vstride = stride;
+  g->voffset = offset;
+  g->icount = count;
+}
+
+// An accel over box primitives (gprt/gprt_host.h:1282). It is built by the same LBVH builder; its
+// closest hit is the box surface (boxprim_hit), standing in for the user intersection program
+// that the reference's AABB geometry runs. It has no descriptor slot, so instance accels can not
+// reference it.
+GPRT_API GPRTAccel gprtAABBAccelCreate(GPRTContext _ctx, GPRTGeom _g, unsigned int flags) {
+  Context *ctx = (Context *)_ctx;
+  Geom *g = (Geom *)_g;
+  if (!ctx || !g) {
+    fail("gprtAABBAccelCreate: null argument");
+    return nullptr;
+  }
+  if (g->type->kind != GPRT_AABBS) {
+    fail("gprtAABBAccelCreate: geometry is not GPRT_AABBS");
+    return nullptr;
+  }
+  Accel *a = new Accel();
+  a->ctx = ctx;
+  a->kind = AccelKind::AABBs;
+  a->geom = g;
+  a->flags = flags;
+  return (GPRTAccel)a;
+}
+
 GPRT_API GPRTAccel gprtTriangleAccelCreate(GPRTContext _ctx, GPRTGeom _g, unsigned int flags) {
   Context *ctx = (Context *)_ctx;
   Geom *g = (Geom *)_g;
@@ -124,6 +177,17 @@ GPRT_API void gprtAccelDestroy(GPRTAccel _a) {
 }
 
 static bool validate_geom(Geom *g, const char *fn) {
+  if (g->type->kind == GPRT_AABBS) {
+    if (!g->vbuf || g->icount == 0) {
+      fail("%s: box geometry has no boxes (gprtAABBsSetPositions)", fn);
+      return false;
+    }
+    if (g->icount >= (1u << 30)) {
+      fail("%s: %u boxes exceeds the 2^30 limit", fn, g->icount);
+      return false;
+    }
+    return true;
+  }
   if (!g->vbuf || !g->ibuf) {
     fail("%s: triangle geometry needs vertices and indices", fn);
     return false;
@@ -181,8 +245,8 @@ GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t
       fail("gprtAccelBuildBatch: accel %u appears twice", i);
       return;
     }
-    if (accels[i]->kind != AccelKind::Triangles) {
-      fail("gprtAccelBuildBatch: accel %u is not a triangle accel", i);
+    if (accels[i]->kind == AccelKind::Instances) {
+      fail("gprtAccelBuildBatch: accel %u is an instance accel", i);
       return;
     }
     if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatch")) return;
@@ -215,7 +279,7 @@ GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
   }
   if (!validate_geom(a->geom, "gprtAccelUpdate")) return;
   if (a->geom->icount != a->n) {
-    fail("gprtAccelUpdate: triangle count changed (%u -> %u); rebuild instead", a->n, a->geom->icount);
+    fail("gprtAccelUpdate: primitive count changed (%u -> %u); rebuild instead", a->n, a->geom->icount);
     return;
   }
   lbvh_refit(ctx, ctx->stream, a);
@@ -234,7 +298,7 @@ GPRT_API size_t gprtAccelGetSize(GPRTAccel _a, int) {
 GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel _a, int) {
   Accel *a = (Accel *)_a;
   if (!a) return nullptr;
-  if (a->kind == AccelKind::Instances) return a->nodes;
+  if (a->kind != AccelKind::Triangles || a->slot == 0xFFFFFFFFu) return a->nodes;
   return a->ctx->blasPool ? (const void *)(a->ctx->blasPool + a->slot) : nullptr;
 }
 
@@ -283,8 +347,8 @@ GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _no
     fail("gprtAccelExportLBVH: accel has not been built");
     return;
   }
-  if (a->kind != AccelKind::Triangles) {
-    fail("gprtAccelExportLBVH: expects a triangle accel");
+  if (a->kind == AccelKind::Instances) {
+    fail("gprtAccelExportLBVH: expects a triangle or box accel");
     return;
   }
   const size_t n = a->n;
@@ -312,8 +376,8 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
   std::vector<Accel *> accels(count);
   for (uint32_t i = 0; i < count; ++i) {
     accels[i] = (Accel *)_accels[i];
-    if (!accels[i] || accels[i]->kind != AccelKind::Triangles) {
-      fail("gprtAccelBuildBatchAutotune: accel %u is null or not a triangle accel", i);
+    if (!accels[i] || accels[i]->kind == AccelKind::Instances) {
+      fail("gprtAccelBuildBatchAutotune: accel %u is null or an instance accel", i);
       return 0;
     }
     if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatchAutotune")) return 0;

@@ -131,6 +131,27 @@ __device__ __forceinline__ bool box_hit(const RayPre &r, float lox, float loy, f
   return lo <= hi;
 }
 
+// Box primitive (AABB geometry) as a closed surface: the entry distance when the ray enters the
+// box after tmin, else the exit distance when it starts inside; exact FP32 slab test in a fixed
+// operation order, with the total-order min/max (so the sign of a zero is deterministic).
+__device__ __forceinline__ bool boxprim_hit(const RayPre &r, const float4 &lo, const float4 &hi, float &t) {
+  const float t0x = (lo.x - r.ox) * r.ix, t1x = (hi.x - r.ox) * r.ix;
+  const float t0y = (lo.y - r.oy) * r.iy, t1y = (hi.y - r.oy) * r.iy;
+  const float t0z = (lo.z - r.oz) * r.iz, t1z = (hi.z - r.oz) * r.iz;
+  const float tn = fmax_ord(fmax_ord(fmin_ord(t0x, t1x), fmin_ord(t0y, t1y)), fmin_ord(t0z, t1z));
+  const float tf = fmin_ord(fmin_ord(fmax_ord(t0x, t1x), fmax_ord(t0y, t1y)), fmax_ord(t0z, t1z));
+  if (!(tn <= tf)) return false;
+  if (tn > r.tmin) {
+    t = tn;
+    return true;
+  }
+  if (tf > r.tmin) {
+    t = tf;
+    return true;
+  }
+  return false;
+}
+
 // Watertight ray/triangle intersection (Woop, Benthin, Wald, "Watertight Ray/Triangle
 // Intersection", JCGT 2(1), 2013), FP32 with a fixed operation order and an FP64 re-evaluation
 // of the 2D edge functions when one of them is exactly zero. Returns true and (t, u, v) when the

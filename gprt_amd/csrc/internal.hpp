@@ -117,6 +117,7 @@ struct MeshDesc {
   uint32_t innerOffset;  // first inner node of this mesh (= primOffset - mesh index)
   uint32_t tileOffset;   // first tile of this mesh
   uint32_t ntiles;       // ceil(ntri / tileKeys)
+  uint32_t boxes;        // 1: the primitives are boxes {lo.xyz, hi.xyz} at verts + i * vstride (idx unused)
 };
 
 // Device storage for the LBVHs of one or more meshes built together (one accel = one mesh).
@@ -175,7 +176,9 @@ struct BlasDesc {
 constexpr uint32_t kBlasMagic = 0x53414C42u;
 constexpr uint32_t kBlasPoolSlots = 65536;
 
-enum class AccelKind { Triangles, Instances };
+// Triangles and AABBs are primitive accels (built by the LBVH builder); Instances is the two-level
+// accel over triangle accels.
+enum class AccelKind { Triangles, Instances, AABBs };
 struct Tlas;
 
 struct Accel {

@@ -107,6 +107,20 @@ __device__ __forceinline__ void load_triangle(const MeshDesc &d, uint32_t prim, 
   c = load_vertex(d, p[2]);
 }
 
+// Primitive `prim` as three points whose bounding box is the primitive's box: a triangle's
+// vertices, or a box primitive's two corners {lo.xyz, hi.xyz} (the first repeated).
+__device__ __forceinline__ void load_prim(const MeshDesc &d, uint32_t prim, float3 &a, float3 &b, float3 &c) {
+  if (d.boxes) {
+    prim = prim < d.ntri ? prim : d.ntri - 1;
+    const auto *p = global_ptr<float>(d.verts + (size_t)prim * d.vstride);
+    a = make_float3(p[0], p[1], p[2]);
+    b = make_float3(p[3], p[4], p[5]);
+    c = a;
+  } else {
+    load_triangle(d, prim, a, b, c);
+  }
+}
+
 __device__ __forceinline__ float unord(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
@@ -136,9 +150,13 @@ __global__ void __launch_bounds__(256) k_init(uint32_t *__restrict__ zero, size_
   for (; i < words; i += stride) zero[i] = 0u;
 }
 
-// Triangle centroid, rounded exactly as the key computation of gprt_lbvh.hlsl:139-146 (no contraction).
+// Triangle centroid, rounded exactly as the key computation of gprt_lbvh.hlsl:139-146 (no
+// contraction); a box primitive's centre is (lo + hi) * 0.5.
 __device__ __forceinline__ float3 centroid(float3 p, float3 q, float3 s) {
   return make_float3((p.x + q.x + s.x) / 3.0f, (p.y + q.y + s.y) / 3.0f, (p.z + q.z + s.z) / 3.0f);
+}
+__device__ __forceinline__ float3 prim_centroid(const MeshDesc &d, float3 p, float3 q, float3 s) {
+  return d.boxes ? make_float3((p.x + q.x) * 0.5f, (p.y + q.y) * 0.5f, (p.z + q.z) * 0.5f) : centroid(p, q, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -157,8 +175,8 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kBoundsThreads) {
     float3 p, q, s;
-    load_triangle(d, i, p, q, s);
-    const float3 c = centroid(p, q, s);
+    load_prim(d, i, p, q, s);
+    const float3 c = prim_centroid(d, p, q, s);
     float *cd = a.centroids + 3 * ((size_t)d.primOffset + i);
     cd[0] = c.x;
     cd[1] = c.y;
@@ -574,20 +592,28 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   float box[6];
   if (k < hi) {
     float3 p, q, s;
-    load_triangle(d, prim, p, q, s);
+    load_prim(d, prim, p, q, s);
     s_delta[tid + 1] = k + 1 < n ? __clzll((((uint64_t)ck << 32) | prim) ^ (((uint64_t)cn << 32) | in)) : -1;
     if (tid == 0) s_delta[0] = lo > 0 ? __clzll((((uint64_t)cp << 32) | ip) ^ (((uint64_t)ck << 32) | prim)) : -1;
-    TriRecord rec;
-    rec.v0 = make_float4(p.x, p.y, p.z, __uint_as_float(prim));
-    rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
-    rec.v2 = make_float4(s.x, s.y, s.z, 0.0f);
-    a.tris[d.primOffset + k] = rec;
     box[0] = fmin_ord(p.x, fmin_ord(q.x, s.x));
     box[1] = fmin_ord(p.y, fmin_ord(q.y, s.y));
     box[2] = fmin_ord(p.z, fmin_ord(q.z, s.z));
     box[3] = fmax_ord(p.x, fmax_ord(q.x, s.x));
     box[4] = fmax_ord(p.y, fmax_ord(q.y, s.y));
     box[5] = fmax_ord(p.z, fmax_ord(q.z, s.z));
+    // Leaf record: a triangle {v0, primID} {v1} {v2}, or a box primitive's ordered corners
+    // {lo, primID} {hi} {0}.
+    TriRecord rec;
+    if (d.boxes) {
+      rec.v0 = make_float4(box[0], box[1], box[2], __uint_as_float(prim));
+      rec.v1 = make_float4(box[3], box[4], box[5], 0.0f);
+      rec.v2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      rec.v0 = make_float4(p.x, p.y, p.z, __uint_as_float(prim));
+      rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
+      rec.v2 = make_float4(s.x, s.y, s.z, 0.0f);
+    }
+    a.tris[d.primOffset + k] = rec;
     for (int c = 0; c < 6; ++c) s_leaf[tid][c] = box[c];
   }
   lds_barrier();
@@ -1134,11 +1160,15 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
     MeshDesc &d = desc[m];
     std::memset(&d, 0, sizeof(d));
     d.verts = (const char *)g->vbuf->d + g->voffset;
-    d.idx = (const char *)g->ibuf->d + g->ioffset;
     d.vstride = g->vstride;
     d.nverts = g->vcount;
-    d.istride = g->istride;
     d.ntri = g->icount;
+    if (accels[m]->kind == AccelKind::AABBs) {
+      d.boxes = 1u;  // {lo.xyz, hi.xyz} per primitive at verts + i * vstride
+    } else {
+      d.idx = (const char *)g->ibuf->d + g->ioffset;
+      d.istride = g->istride;
+    }
     d.primOffset = (uint32_t)prims;
     d.innerOffset = (uint32_t)(prims - m);
     d.tileOffset = (uint32_t)tiles;

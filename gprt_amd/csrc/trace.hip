@@ -25,6 +25,7 @@ namespace gprt {
 
 namespace {
 
+template <bool kBoxes>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const BVHNode *__restrict__ nodes, const TriRecord *__restrict__ tris, uint32_t nprims,
             const GPRTRay *__restrict__ rays, GPRTHit *__restrict__ hits, uint32_t count,
@@ -55,12 +56,12 @@ __global__ void __launch_bounds__(kTraceBlock)
           const int32_t root = classRoot[c];
           // A treelet rooted at the tree's root (cut depth 0) would only repeat the full pass.
           if (root != 0) {
-            visits += traverse<kMaxStack>(r, nodes, tris, nprims, root, h, st, 0, 0u, -1);
+            visits += traverse<kMaxStack, kBoxes>(r, nodes, tris, nprims, root, h, st, 0, 0u, -1);
             skip = root;  // searched completely: the full pass does not enter it again
           }
         }
       }
-      visits += traverse<kMaxStack>(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, st, 0, 0u, -1, skip);
+      visits += traverse<kMaxStack, kBoxes>(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, st, 0, 0u, -1, skip);
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
       o.primID = h.prim;
@@ -97,9 +98,15 @@ bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, v
   if (!work) return false;
   const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
   fill_u32(ctx, s, work, 0u, 1);
-  launch(ctx, s, proposal ? "trace_closest_proposal" : "trace_closest", k_trace, dim3(blocks), dim3(kTraceBlock), 0,
-         (const BVHNode *)a->nodes, (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count,
-         work, work + 64, proposal, classRoot, numClasses, visits);
+  const char *name = proposal ? "trace_closest_proposal" : "trace_closest";
+  if (a->kind == AccelKind::AABBs)
+    launch(ctx, s, name, k_trace<true>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
+           (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
+           classRoot, numClasses, visits);
+  else
+    launch(ctx, s, name, k_trace<false>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
+           (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
+           classRoot, numClasses, visits);
   return true;
 }
 

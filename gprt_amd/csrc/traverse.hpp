@@ -67,10 +67,34 @@ __device__ __forceinline__ void intersect_record(const RayPre &r, const TriRecor
   }
 }
 
+// Tests one box-primitive record ({lo, primID} {hi} {0}); box hits report u = v = 0.
+__device__ __forceinline__ void intersect_box_record(const RayPre &r, const TriRecord &tr, uint32_t inst, int tag,
+                                                     HitState &h) {
+  float t;
+  if (boxprim_hit(r, tr.v0, tr.v1, t)) {
+    const int prim = (int)__float_as_uint(tr.v0.w);
+    if (hit_closer(t, inst, prim, h)) {
+      h.t = t;
+      h.prim = prim;
+      h.u = 0.f;
+      h.v = 0.f;
+      h.inst = inst;
+      h.leaf = tag;
+    } else if (t == h.t && inst == h.inst && prim == h.prim && h.leaf < 0) {
+      h.leaf = tag;
+    }
+  }
+}
+
+// kBoxes: the leaves are box primitives rather than triangles.
+template <bool kBoxes = false>
 __device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord *__restrict__ tris, int leaf,
                                                uint32_t nprims, uint32_t inst, int tag, HitState &h) {
   if ((uint32_t)leaf >= nprims) return;
-  intersect_record(r, tris[leaf], inst, tag, h);
+  if (kBoxes)
+    intersect_box_record(r, tris[leaf], inst, tag, h);
+  else
+    intersect_record(r, tris[leaf], inst, tag, h);
 }
 
 // Depth-first walk of the subtree rooted at `start` (an inner node index, or ~leaf position) of a
@@ -233,15 +257,16 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
   return visits;
 }
 
-// Triangle-accel walk: leaves are triangles of instance `inst`; `tag` is recorded with a hit.
-template <int MaxStack>
+// Primitive-accel walk: leaves are triangles (or, with kBoxes, box primitives) of instance `inst`;
+// `tag` is recorded with a hit.
+template <int MaxStack, bool kBoxes = false>
 __device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__restrict__ nodes,
                                              const TriRecord *__restrict__ tris, uint32_t nprims, int start,
                                              HitState &h, const LaneStack &st, int sp0, uint32_t inst, int tag,
                                              int skip = kNoSkip) {
   return walk_batched<MaxStack>(r, nodes, nprims, start, h, st, sp0, skip,
                                 [&](int leaf, int) __attribute__((always_inline)) {
-                                  intersect_leaf(r, tris, leaf, nprims, inst, tag, h);
+                                  intersect_leaf<kBoxes>(r, tris, leaf, nprims, inst, tag, h);
                                 });
 }
 

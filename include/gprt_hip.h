@@ -148,6 +148,17 @@ GPRT_API void gprtTrianglesSetVertices(GPRTGeom triangles, GPRTBuffer vertices, 
 GPRT_API void gprtTrianglesSetIndices(GPRTGeom triangles, GPRTBuffer indices, uint32_t count,
                                       uint32_t stride GPRT_IF_CPP(= 12), uint32_t offset GPRT_IF_CPP(= 0));
 GPRT_API GPRTAccel gprtTriangleAccelCreate(GPRTContext context, GPRTGeom geom, unsigned int flags GPRT_IF_CPP(= 0));
+
+/* Box primitives (gprt/gprt_host.h:614, :1282): `count` boxes {minX, minY, minZ, maxX, maxY, maxZ}
+ * at offset + i * stride (stride >= 24, default 2 * sizeof(float3)) of a GPRT_AABBS geometry, and
+ * an accel over them built by the same LBVH builder (keys from the box centres (lo + hi) * 0.5).
+ * The reference runs a user intersection program on the boxes; here the closest hit is the box
+ * surface: the entry distance when the ray enters the box after tmin, else the exit distance when
+ * it starts inside (u = v = 0). Box accels can be built alone or in batches, updated, traced,
+ * exported and given a proposal; instance accels reference triangle accels only. */
+GPRT_API void gprtAABBsSetPositions(GPRTGeom aabbs, GPRTBuffer positions, uint32_t count,
+                                    uint32_t stride GPRT_IF_CPP(= 24), uint32_t offset GPRT_IF_CPP(= 0));
+GPRT_API GPRTAccel gprtAABBAccelCreate(GPRTContext context, GPRTGeom geom, unsigned int flags GPRT_IF_CPP(= 0));
 GPRT_API void gprtAccelDestroy(GPRTAccel accel);
 GPRT_API void gprtAccelBuild(GPRTContext context, GPRTAccel accel,
                              GPRTBuildParams buildParams GPRT_IF_CPP(= GPRTBuildParams()));

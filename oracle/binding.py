@@ -50,6 +50,12 @@ def lib() -> C.CDLL:
             "orc_trace_bruteforce": (None, [P, sz, P, P, sz, P, C.c_int]),
             "orc_trace_bvh": (None, [P, sz, P, P, P, P, sz, P, P, C.c_int]),
             "orc_max_threads": (C.c_int, []),
+            "orc_box_bounds": (None, [P, sz, P]),
+            "orc_box_keys": (None, [P, sz, P, P]),
+            "orc_box_refit": (None, [P, P, sz, P]),
+            "orc_lbvh_build_boxes": (None, [P, sz, P, P, P]),
+            "orc_box_hit": (C.c_int, [P, P, P]),
+            "orc_trace_boxes_bruteforce": (None, [P, sz, P, sz, P, C.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -143,6 +149,61 @@ def lbvh_build(verts, idx):
     aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
     lib().orc_lbvh_build(_p(verts), _p(idx), n, _p(keys), _p(nodes), _p(aabbs))
     return keys, nodes, aabbs
+
+
+def _boxes(boxes):
+    return np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+
+
+def box_bounds(boxes):
+    b = _boxes(boxes)
+    out = np.zeros(6, dtype=np.float32)
+    lib().orc_box_bounds(_p(b), len(b), _p(out))
+    return out
+
+
+def box_keys(boxes, bounds):
+    b = _boxes(boxes)
+    bounds = np.ascontiguousarray(bounds, dtype=np.float32)
+    keys = np.zeros(len(b), dtype=np.uint64)
+    lib().orc_box_keys(_p(b), len(b), _p(bounds), _p(keys))
+    return keys
+
+
+def box_refit(boxes, nodes):
+    b = _boxes(boxes)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    aabbs = np.zeros((len(nodes), 6), dtype=np.float32)
+    lib().orc_box_refit(_p(b), _p(nodes), len(b), _p(aabbs))
+    return aabbs
+
+
+def lbvh_build_boxes(boxes):
+    """LBVH over box primitives {lo.xyz, hi.xyz}: (sorted keys, int4 nodes, node boxes)."""
+    b = _boxes(boxes)
+    n = len(b)
+    keys = np.zeros(n, dtype=np.uint64)
+    nodes = np.zeros((2 * n - 1, 4), dtype=np.int32)
+    aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
+    lib().orc_lbvh_build_boxes(_p(b), n, _p(keys), _p(nodes), _p(aabbs))
+    return keys, nodes, aabbs
+
+
+def box_hit(ray8, box6):
+    """(hit, t) of the box surface test for one ray {ox, oy, oz, tmin, dx, dy, dz, tmax}."""
+    r = np.ascontiguousarray(ray8, dtype=np.float32)
+    b = np.ascontiguousarray(box6, dtype=np.float32)
+    t = C.c_float()
+    hit = lib().orc_box_hit(_p(r), _p(b), C.byref(t))
+    return bool(hit), t.value
+
+
+def trace_boxes_bruteforce(rays, boxes, threads: int = 0):
+    r = _rays_f32(rays)
+    b = _boxes(boxes)
+    hits = np.zeros(len(r), dtype=HIT_DTYPE)
+    lib().orc_trace_boxes_bruteforce(_p(r), len(r), _p(b), len(b), _p(hits), threads or os.cpu_count() or 1)
+    return hits
 
 
 def intersect(ray8, v0, v1, v2):

@@ -20,6 +20,11 @@
 //   trace     closest hit with the open interval tmin < t (gprt/gprt_fallbacks.slang:108,139);
 //             triangle test = watertight ray/triangle intersection (Woop, Benthin, Wald, JCGT
 //             2013), restated here independently of the device code.
+//   boxes     AABB geometry (gprt/gprt_host.h:614, :1282; built by the driver at
+//             gprt/gprt.cpp:3961-4003): the LBVH stages above with a box's corners as its leaf box
+//             and its centre (lo + hi) * 0.5 as its key point; the closest hit is the box surface
+//             (entry distance, or the exit distance from inside), the build's stand-in for a user
+//             intersection program.
 //
 // Pinning (see tests/test_oracle.py): Morton known-answer values, closed forms of the t03
 // patterns, brute-force sort/scan on small inputs, an independent top-down radix-tree builder
@@ -276,8 +281,10 @@ static void tri_box(const float *verts, const uint32_t *idx, uint32_t prim, floa
 }
 
 // Refit as gprt_lbvh.hlsl:400-428: every leaf writes its box and folds it into all ancestors.
-// aabbs: float[6 * (2n-1)].
-void orc_refit(const float *verts, const uint32_t *idx, const int32_t *nodes, size_t n_, float *aabbs) {
+// aabbs: float[6 * (2n-1)]; leafBox(prim, box) gives a primitive's box.
+extern "C++" {
+template <class LeafBox>
+static void refit_with(const int32_t *nodes, size_t n_, float *aabbs, LeafBox leafBox) {
   const long n = (long)n_;
   for (long x = 0; x < 2 * n - 1; ++x) {
     for (int a = 0; a < 3; ++a) {
@@ -288,7 +295,7 @@ void orc_refit(const float *verts, const uint32_t *idx, const int32_t *nodes, si
   for (long i = 0; i < n; ++i) {
     const long leaf = n - 1 + i;
     float b[6];
-    tri_box(verts, idx, (uint32_t)nodes[4 * leaf + 3], b);
+    leafBox((uint32_t)nodes[4 * leaf + 3], b);
     std::memcpy(&aabbs[6 * leaf], b, sizeof(b));
     for (long p = nodes[4 * leaf + 2]; p >= 0; p = nodes[4 * p + 2]) {
       for (int a = 0; a < 3; ++a) {
@@ -297,6 +304,12 @@ void orc_refit(const float *verts, const uint32_t *idx, const int32_t *nodes, si
       }
     }
   }
+}
+
+}  // extern "C++"
+
+void orc_refit(const float *verts, const uint32_t *idx, const int32_t *nodes, size_t n_, float *aabbs) {
+  refit_with(nodes, n_, aabbs, [&](uint32_t prim, float *b) { tri_box(verts, idx, prim, b); });
 }
 
 // Independent box check: each node's box = min/max over the triangles of its leaf range, with
@@ -514,6 +527,125 @@ void orc_trace_bvh(const float *rays, size_t nrays, const float *verts, const ui
   if (counters) {
     counters[0] = visited;
     counters[1] = tested;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Box primitives. boxes: float[6 * n] {lo.xyz, hi.xyz} as given (not necessarily ordered).
+// ------------------------------------------------------------------------------------------------
+static void box_of(const float *boxes, uint32_t prim, float *b) {
+  const float *p = boxes + 6 * (size_t)prim;
+  for (int a = 0; a < 3; ++a) {
+    b[a] = omin(p[a], p[3 + a]);
+    b[3 + a] = omax(p[a], p[3 + a]);
+  }
+}
+
+// Scene box over both corners of every box.
+void orc_box_bounds(const float *boxes, size_t n, float *out) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = 0; i < n; ++i) {
+    const float *p = boxes + 6 * i;
+    for (int c = 0; c < 2; ++c)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = omin(lo[a], p[3 * c + a]);
+        hi[a] = omax(hi[a], p[3 * c + a]);
+      }
+  }
+  for (int a = 0; a < 3; ++a) {
+    out[a] = lo[a];
+    out[3 + a] = hi[a];
+  }
+}
+
+// keys[i] = morton30(centre_i) << 32 | i, centre = (lo + hi) * 0.5, quantised as orc_morton_keys.
+void orc_box_keys(const float *boxes, size_t n, const float *bounds, uint64_t *keys) {
+  for (size_t i = 0; i < n; ++i) {
+    const float *p = boxes + 6 * i;
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+      const float c = (p[a] + p[3 + a]) * 0.5f;
+      float pt = (c - bounds[a]) / (bounds[3 + a] - bounds[a]);
+      pt = std::fmin(std::fmax(pt * 1024.f, 0.f), 1023.f);
+      q[a] = (uint32_t)pt;
+    }
+    keys[i] = ((uint64_t)orc_morton3(q[0], q[1], q[2]) << 32) | (uint64_t)i;
+  }
+}
+
+void orc_box_refit(const float *boxes, const int32_t *nodes, size_t n, float *aabbs) {
+  refit_with(nodes, n, aabbs, [&](uint32_t prim, float *b) { box_of(boxes, prim, b); });
+}
+
+void orc_lbvh_build_boxes(const float *boxes, size_t n, uint64_t *keysOut, int32_t *nodes, float *aabbs) {
+  float bounds[6];
+  orc_box_bounds(boxes, n, bounds);
+  orc_box_keys(boxes, n, bounds, keysOut);
+  orc_stable_sort(keysOut, nullptr, n, 64, 0);
+  orc_karras(keysOut, n, nodes);
+  orc_box_refit(boxes, nodes, n, aabbs);
+}
+
+// Box surface hit: slab distances with the total-order min/max; the entry distance when the ray
+// enters the box after tmin, else the exit distance when it starts inside. inv = 1 / d with
+// |d| clamped to 1e-30 (the traversal's reciprocal).
+static int hit_box(const Pre &r, const float *inv, const float *b, float *t) {
+  float t0[3], t1[3];
+  for (int a = 0; a < 3; ++a) {
+    t0[a] = (b[a] - r.o[a]) * inv[a];
+    t1[a] = (b[3 + a] - r.o[a]) * inv[a];
+  }
+  const float tn = omax(omax(omin(t0[0], t1[0]), omin(t0[1], t1[1])), omin(t0[2], t1[2]));
+  const float tf = omin(omin(omax(t0[0], t1[0]), omax(t0[1], t1[1])), omax(t0[2], t1[2]));
+  if (!(tn <= tf)) return 0;
+  if (tn > r.tmin) {
+    *t = tn;
+    return 1;
+  }
+  if (tf > r.tmin) {
+    *t = tf;
+    return 1;
+  }
+  return 0;
+}
+
+int orc_box_hit(const float *ray, const float *box, float *t) {
+  const Pre r = pre_of(ray);
+  float inv[3], b[6];
+  for (int a = 0; a < 3; ++a) {
+    const float d = std::fabs(r.d[a]) < 1e-30f ? std::copysign(1e-30f, r.d[a]) : r.d[a];
+    inv[a] = 1.0f / d;
+    b[a] = omin(box[a], box[3 + a]);
+    b[3 + a] = omax(box[a], box[3 + a]);
+  }
+  return hit_box(r, inv, b, t);
+}
+
+// hits as orc_trace_bruteforce (u = v = 0): every box tested, the smallest (t, primID) kept.
+void orc_trace_boxes_bruteforce(const float *rays, size_t nrays, const float *boxes, size_t n, float *hits,
+                                int threads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+  for (long i = 0; i < (long)nrays; ++i) {
+    const Pre r = pre_of(rays + 8 * i);
+    float inv[3];
+    for (int a = 0; a < 3; ++a) {
+      const float d = std::fabs(r.d[a]) < 1e-30f ? std::copysign(1e-30f, r.d[a]) : r.d[a];
+      inv[a] = 1.0f / d;
+    }
+    float tbest = r.tmax;
+    int32_t best = -1;
+    for (size_t p = 0; p < n; ++p) {
+      float b[6], t;
+      box_of(boxes, (uint32_t)p, b);
+      if (hit_box(r, inv, b, &t) && (t < tbest || (t == tbest && (uint32_t)p < (uint32_t)best))) {
+        tbest = t;
+        best = (int32_t)p;
+      }
+    }
+    hits[4 * i + 0] = tbest;
+    std::memcpy(&hits[4 * i + 1], &best, 4);
+    hits[4 * i + 2] = 0.f;
+    hits[4 * i + 3] = 0.f;
   }
 }
 
