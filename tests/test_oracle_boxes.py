@@ -67,7 +67,12 @@ def test_box_lbvh_tree_and_boxes(orc, n):
     assert np.all(np.diff(keys.astype(np.float64)) > 0) or n == 1
     if n > 1:
         np.testing.assert_array_equal(nodes, orc.topdown(keys))
-    # every node's box = union of the ordered leaf boxes in its leaf range
+    _check_unions(b, nodes, aabbs)
+
+
+def _check_unions(b, nodes, aabbs):
+    """Every node's box = the union of the ordered leaf boxes in its leaf range."""
+    n = len(b)
     lo = np.minimum(b[:, :3], b[:, 3:])
     hi = np.maximum(b[:, :3], b[:, 3:])
     leaf_prim = nodes[n - 1:, 3]
@@ -101,3 +106,13 @@ def test_box_bruteforce_is_the_minimum_of_single_tests(orc):
                 best_t, best = np.float32(t), p
         assert hits["primID"][i] == best
         assert hits["t"][i] == best_t
+
+
+def test_box_refit_keeps_topology_and_unions(orc):
+    b = random_boxes(2000, 5)
+    _, nodes, aabbs = orc.lbvh_build_boxes(b)
+    np.testing.assert_array_equal(orc.box_refit(b, nodes).view(np.uint32), aabbs.view(np.uint32))
+    rng = np.random.default_rng(6)
+    moved = (b + np.repeat(rng.random((len(b), 3), dtype=np.float32) * np.float32(0.2), 2, axis=0)
+             .reshape(-1, 6)).astype(np.float32)
+    _check_unions(moved, nodes, orc.box_refit(moved, nodes))
