@@ -134,6 +134,11 @@ _SIGS = {
     "gprtAccelTraceClosestHitDetail": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P]),
     "gprtProposalTrainLabels": (C.c_float, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float,
                                             C.POINTER(C.c_float)]),
+    "gprtModuleCreate": (_P, [_P, _P, C.c_size_t]),
+    "gprtModuleDestroy": (None, [_P]),
+    "gprtComputeCreate": (_P, [_P, _P, C.c_char_p]),
+    "gprtComputeDestroy": (None, [_P]),
+    "gprtComputeLaunchArgs": (C.c_uint64, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _P, C.c_size_t]),
     "gprtComputeSynchronize": (C.c_uint64, [_P]),
     "gprtDeviceSynchronize": (C.c_uint64, [_P]),
     "gprtBeginProfile": (None, [_P]),
@@ -445,4 +450,65 @@ class Proposal:
     def destroy(self):
         if self.h:
             self.lib.gprtProposalDestroy(self.ctx.h, self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------------------------------------
+# User compute programs
+# ------------------------------------------------------------------------------------------------
+PUSH_CONSTANTS_LIMIT = 256
+
+
+def pack_args(*args) -> bytes:
+    """Packs ctypes values as a kernel's parameter list: each at its natural alignment (the HIP
+    kernel-argument layout, and the layout gprtPackArgs of the C++ header writes)."""
+    out = bytearray()
+    for a in args:
+        al = C.alignment(a)
+        out += b"\0" * (-len(out) % al)
+        out += bytes(a)
+    return bytes(out)
+
+
+class Module:
+    """gprtModuleCreate over a gfx950 code object file (hipcc --genco)."""
+
+    def __init__(self, ctx: Context, path_or_bytes):
+        self.ctx, self.lib = ctx, ctx.lib
+        if isinstance(path_or_bytes, (bytes, bytearray)):
+            self.code = bytes(path_or_bytes)
+        else:
+            with open(path_or_bytes, "rb") as f:
+                self.code = f.read()
+        self.h = self.lib.gprtModuleCreate(ctx.h, self.code, len(self.code))
+        if not self.h:
+            raise RuntimeError("gprtModuleCreate failed: %s" % last_error())
+
+    def compute(self, name: str) -> "Compute":
+        return Compute(self, name)
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtModuleDestroy(self.h)
+            self.h = None
+
+
+class Compute:
+    """gprtComputeCreate / gprtComputeLaunch: one kernel of a module."""
+
+    def __init__(self, module: Module, name: str):
+        self.ctx, self.lib = module.ctx, module.lib
+        self.h = self.lib.gprtComputeCreate(self.ctx.h, module.h, name.encode())
+        if not self.h:
+            raise RuntimeError("gprtComputeCreate failed: %s" % last_error())
+
+    def launch(self, groups, threads, *args):
+        g = (C.c_uint32 * 3)(*(list(groups) + [1] * (3 - len(groups))))
+        t = (C.c_uint32 * 3)(*(list(threads) + [1] * (3 - len(threads))))
+        blob = pack_args(*args)
+        return self.lib.gprtComputeLaunchArgs(self.h, g, t, blob, len(blob))
+
+    def destroy(self):
+        if self.h:
+            self.lib.gprtComputeDestroy(self.h)
             self.h = None

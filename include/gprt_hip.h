@@ -167,6 +167,91 @@ GPRT_API size_t gprtAccelGetSize(GPRTAccel accel, int deviceID GPRT_IF_CPP(= 0))
 GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel accel, int deviceID GPRT_IF_CPP(= 0));
 
 /* ---------------------------------------------------------------------------------------------
+ * User compute programs (modules gprt/gprt_host.h:403-404, compute handles :1007 / :1025,
+ * launches :2537-2586; limits gprt/gprt_shared.h:60-67).
+ *
+ * A module is device code for gfx950: a code object or an offload bundle holding one, as
+ * `hipcc --genco --offload-arch=gfx950` writes it (the reference takes SPIR-V). A compute handle
+ * names one of its kernels by symbol (declare it `extern "C" __global__`). A launch runs
+ * numGroups workgroups of groupSize threads on the context's stream, ordered after every earlier
+ * call on the context (sorts, builds, traces, copies) and before every later one, so a program
+ * can fill the buffers a build or trace reads. Its arguments are the kernel's parameters, in
+ * order, each at its natural alignment (the HIP kernel-argument layout; for a single record
+ * argument this is the reference's push-constant block byte for byte), at most
+ * GPRT_PUSH_CONSTANTS_LIMIT bytes in all. Each group count must be at most
+ * GPRT_WORKGROUP_LIMIT and a group at most GPRT_THREADGROUP_LIMIT threads, as the reference's
+ * launch asserts (gprt/gprt_host.h:2549-2554); a group count of 0 launches nothing. Launches
+ * return 0 (the reference returns its timeline counter). Destroying a module waits for the
+ * context's stream; compute handles of a destroyed module report an error when launched.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct GPRTModule_T *GPRTModule;
+typedef struct GPRTCompute_T *GPRTCompute;
+#define GPRT_PUSH_CONSTANTS_LIMIT 256 /* gprt/gprt_shared.h:60 */
+#define GPRT_WORKGROUP_LIMIT 65535    /* gprt/gprt_shared.h:66 */
+#define GPRT_THREADGROUP_LIMIT 1024   /* gprt/gprt_shared.h:67 */
+GPRT_API GPRTModule gprtModuleCreate(GPRTContext context, const void *code, size_t bytes); /* :403 */
+GPRT_API void gprtModuleDestroy(GPRTModule module);                                        /* :404 */
+GPRT_API GPRTCompute gprtComputeCreate(GPRTContext context, GPRTModule module, const char *entrypoint); /* :1007 */
+GPRT_API void gprtComputeDestroy(GPRTCompute compute);                                                /* :1025 */
+/* The untyped launch under the templates below (the reference's _gprtComputeLaunch,
+ * gprt/gprt_host.h:2539): `args` holds `bytes` bytes of packed arguments. */
+GPRT_API uint64_t gprtComputeLaunchArgs(GPRTCompute compute, const uint32_t numGroups[3], const uint32_t groupSize[3],
+                                        const void *args, size_t bytes);
+
+#ifdef __cplusplus
+/* C++ conveniences with the reference's spelling of the same calls. */
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+using GPRTProgram = std::vector<uint8_t>; /* gprt/gprt_host.h:136 */
+template <typename... T> struct GPRTComputeOf_T;
+template <typename... T> using GPRTComputeOf = GPRTComputeOf_T<T...> *; /* typed handle, as :131 */
+struct GPRTUint3 {
+  uint32_t x, y, z;
+};
+inline GPRTModule gprtModuleCreate(GPRTContext context, const GPRTProgram &code) {
+  return gprtModuleCreate(context, code.data(), code.size());
+}
+template <typename... Uniforms>
+GPRTComputeOf<Uniforms...> gprtComputeCreate(GPRTContext context, GPRTModule module, const std::string &entrypoint) {
+  return (GPRTComputeOf<Uniforms...>)gprtComputeCreate(context, module, entrypoint.c_str());
+}
+template <typename... Uniforms> void gprtComputeDestroy(GPRTComputeOf<Uniforms...> compute) {
+  gprtComputeDestroy((GPRTCompute)compute);
+}
+/* Bytes the arguments take when each starts at its natural alignment. */
+template <typename... Args> constexpr size_t gprtPackedSize() {
+  size_t off = 0;
+  ((off = (off + alignof(Args) - 1) / alignof(Args) * alignof(Args) + sizeof(Args)), ...);
+  return off;
+}
+/* Packs the arguments as a kernel's parameter list; returns the bytes written. */
+template <typename... Args> size_t gprtPackArgs(unsigned char *block, const Args &...args) {
+  static_assert((std::is_trivially_copyable<Args>::value && ...), "arguments must be trivially copyable");
+  size_t off = 0;
+  ((off = (off + alignof(Args) - 1) / alignof(Args) * alignof(Args), std::memcpy(block + off, &args, sizeof(Args)),
+    off += sizeof(Args)),
+   ...);
+  return off;
+}
+template <typename... Uniforms>
+uint64_t gprtComputeLaunch(GPRTCompute compute, GPRTUint3 numGroups, GPRTUint3 groupSize, Uniforms... uniforms) {
+  static_assert(gprtPackedSize<Uniforms...>() <= GPRT_PUSH_CONSTANTS_LIMIT,
+                "Total size of arguments exceeds GPRT_PUSH_CONSTANTS_LIMIT bytes");
+  alignas(16) unsigned char block[GPRT_PUSH_CONSTANTS_LIMIT] = {};
+  const size_t bytes = gprtPackArgs(block, uniforms...);
+  const uint32_t g[3] = {numGroups.x, numGroups.y, numGroups.z}, t[3] = {groupSize.x, groupSize.y, groupSize.z};
+  return gprtComputeLaunchArgs(compute, g, t, block, bytes);
+}
+template <typename... Uniforms>
+uint64_t gprtComputeLaunch(GPRTComputeOf<Uniforms...> compute, GPRTUint3 numGroups, GPRTUint3 groupSize,
+                           Uniforms... uniforms) {
+  return gprtComputeLaunch<Uniforms...>((GPRTCompute)compute, numGroups, groupSize, uniforms...);
+}
+#endif
+
+/* ---------------------------------------------------------------------------------------------
  * Synchronisation and profiling (gprt/gprt_host.h:2588-2595). Begin/EndProfile time the work
  * enqueued between the two calls on the context's stream, in milliseconds.
  * ------------------------------------------------------------------------------------------- */

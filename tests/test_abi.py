@@ -102,3 +102,50 @@ def test_instance_record_layout(tmp_path):
     assert rec["accelAddress"] == 0x1122334455667788
     c = api.GPRTInstance.from_buffer_copy(raw)
     assert (c.instanceCustomIndex, c.mask, c.geometryIndex, c.flags) == (0x123456, 0xAB, 7, 3)
+
+
+def test_compute_entry_points_validate(rec):
+    g = (C.c_uint32 * 3)(1, 1, 1)
+    assert not rec.gprtModuleCreate(None, None, 0)
+    assert "null argument" in api.last_error()
+    rec.gprtClearLastError()
+    assert not rec.gprtComputeCreate(None, None, b"x")
+    assert "null argument" in api.last_error()
+    rec.gprtClearLastError()
+    assert rec.gprtComputeLaunchArgs(None, g, g, None, 0) == 0
+    assert "null argument" in api.last_error()
+
+
+def test_cpp_argument_packing_matches_python(tmp_path):
+    """gprtPackArgs (C++ header) and api.pack_args lay arguments out identically: natural alignment,
+    in order (a uint32 before a pointer leaves 4 bytes of padding, as a HIP kernel's argument
+    block does); and an argument list over 256 bytes does not compile."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    src = tmp_path / "pack.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include "gprt_hip.h"\n'
+        "struct Rec { void *a; float b; };\n"
+        "int main() { alignas(16) unsigned char blk[256] = {};\n"
+        "  Rec r{(void *)0x1122334455667788ull, 2.5f};\n"
+        "  size_t n = gprtPackArgs(blk, (uint32_t)7, (void *)0x0102030405060708ull, (uint16_t)9, 1.5, (uint8_t)3, r);\n"
+        "  static_assert(gprtPackedSize<uint32_t, void *, uint16_t, double, uint8_t, Rec>() == 56, \"size\");\n"
+        "  fwrite(blk, 1, n, stdout); return 0; }\n")
+    exe = tmp_path / "pack"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["g++", "-std=c++17", "-I", inc, str(src), "-o", str(exe)], check=True)
+    raw = subprocess.run([str(exe)], check=True, capture_output=True).stdout
+
+    class Rec(C.Structure):
+        _fields_ = [("a", C.c_void_p), ("b", C.c_float)]
+
+    want = api.pack_args(C.c_uint32(7), C.c_void_p(0x0102030405060708), C.c_uint16(9), C.c_double(1.5),
+                         C.c_uint8(3), Rec(0x1122334455667788, 2.5))
+    assert len(raw) == 56 and raw == want
+    big = tmp_path / "big.cpp"
+    big.write_text('#include "gprt_hip.h"\nstruct B { char c[250]; };\n'
+                   "int main() { B b{}; return (int)gprtComputeLaunch((GPRTCompute)0, {1, 1, 1}, {1, 1, 1}, 1.0, b); }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, str(big)], capture_output=True, text=True)
+    assert r.returncode != 0 and "PUSH_CONSTANTS_LIMIT" in r.stderr
