@@ -22,9 +22,13 @@ The CPU baseline times the oracle (single thread) on a bounded sample and is rep
 
 Other workloads (one JSON line each): --workload batched (4,096 meshes, log-uniform 64..262,144
 triangles: per-mesh launch sequence vs one hipGraph-captured batch launch vs the autotuned
-configuration). bench.py --gpus N with N > 1 exits: this build measures one GPU.
+configuration); dynamic (64 meshes rebuilt per frame under an instance accel, plain vs learned
+traces); refit (refit-only updates of the blob scene); boxes (LBVH build and trace over 4,194,304
+AABB primitives); compute (a user program through gprtComputeLaunch); copy (HBM counter
+calibration). bench.py --gpus N with N > 1 exits: this build measures one GPU.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload scene|batched]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--workload scene|batched|dynamic|refit|boxes|compute|copy]
 """
 from __future__ import annotations
 
@@ -519,13 +523,278 @@ def copy_workload(args):
                       "ms_per_copy": ms, "steps": args.steps}))
 
 
+REFIT_BYTES_PER_PRIM = 104  # 48 B triangle read + 8 B parent link + 48 B node box write
+BOX_LBVH_BYTES_PER_PRIM = 296  # the 368 B/prim build with a 24-byte box in place of each 48-byte triangle read
+N_BOXES = 4_194_304
+N_SPHERES = 16_777_216
+BOUNDS_BYTES_PER_PRIM = 12 + 4 + 24  # centre + radius read, box written
+
+
+def _roof_hbm(kernel, bytes_per_launch, avg_ms, **extra):
+    ach = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
+            **extra}
+
+
+def refit_workload(args):
+    """Refit-only update (gprtAccelUpdate): the 4,194,304-triangle blob scene is built once with
+    GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE; a step moves every vertex (base + U(-1e-3, 1e-3), device
+    generated, seed 7000 + step) and updates the accel: node boxes recomputed over the kept
+    topology. The 2048^2 primary rays are traced after each update and after a full rebuild of the
+    same vertices, to show what the kept topology costs the trace."""
+    from gprt_amd import api, scenes
+
+    lib = api.load()
+    verts, idx = scenes.blob_scene(N_TRIS, seed=2)
+    prim = scenes.pinhole_rays(IMG, IMG)
+    ctx = api.Context(0)
+    base = ctx.device_buffer(verts)
+    acc = api.TriangleAccel(ctx, verts, idx)
+    acc.build(api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    rb = ctx.device_buffer(prim)
+    hb = ctx.device_buffer(nbytes=16 * len(prim))
+    floats = verts.size
+
+    def step(k):
+        lib.gprtSynthPerturb(ctx.h, base.h, acc.vbuf.h, floats, 7000 + k, 1e-3)
+        acc.update()
+        acc.trace(rb, hb, len(prim))
+
+    for k in range(args.warmup):
+        step(k)
+    ctx.sync()
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(args.warmup + k)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    K = args.steps
+    pk = kernel_report(lib, ctx)
+    refit_ms = ms_of(pk, "lbvh_build")[0] / K
+    trace_refit_ms = ms_of(pk, "trace_closest")[0] / K
+    # The same vertices rebuilt from scratch: build and trace times for comparison.
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    for _ in range(K):
+        acc.build(api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+        acc.trace(rb, hb, len(prim))
+    ctx.sync()
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    pk2 = kernel_report(lib, ctx)
+    build_ms = ms_of(pk2, "lbvh_")[0] / K
+    trace_rebuilt_ms = ms_of(pk2, "trace_closest")[0] / K
+    roof = _roof_hbm("lbvh_build_local + region + top (refit over the kept topology)", N_TRIS * REFIT_BYTES_PER_PRIM,
+                     refit_ms, bytes_per_prim=REFIT_BYTES_PER_PRIM)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as orc
+        moved = acc.vbuf.read(np.float32).reshape(-1, 3)
+        _, nodes, _ = acc.export()
+        t0 = time.perf_counter()
+        orc.refit(moved, idx, nodes)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": N_TRIS / cpu_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
+               "sample": f"oracle refit of the whole {N_TRIS}-triangle scene over the same topology, single thread"}
+    out = {
+        "metric": "refit-only update Mprims/s (gprtAccelUpdate), 1 GPU", "value": N_TRIS / (refit_ms * 1e3),
+        "unit": "Mprims/s", "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded blob scene, device-generated per-step vertex motion)",
+        "config": {"workload": "blob scene 4,194,304 tris, every vertex moved by U(+-1e-3) per step; step = "
+                               "perturb + refit + trace of 2048^2 primary rays", "triangles": N_TRIS,
+                   "rays": len(prim)},
+        "components": {"refit_ms": refit_ms, "rebuild_ms": build_ms, "refit_vs_rebuild": build_ms / refit_ms,
+                       "trace_ms_after_refit": trace_refit_ms, "trace_ms_after_rebuild": trace_rebuilt_ms},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": pk,
+    }
+    print(json.dumps(out))
+    for b in (base, rb, hb):
+        b.destroy()
+    acc.destroy()
+    ctx.close()
+
+
+def _box_scene(n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3), dtype=np.float32)
+    h = (rng.random((n, 3), dtype=np.float32) * np.float32(0.002) + np.float32(5e-4)).astype(np.float32)
+    return np.concatenate([c - h, c + h], 1).astype(np.float32)
+
+
+def _box_rays(n, seed):
+    from gprt_amd import scenes
+    rng = np.random.default_rng(seed)
+    o = rng.random((n, 3), dtype=np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    r = np.zeros(n, dtype=scenes.RAY_DTYPE)
+    r["ox"], r["oy"], r["oz"] = o[:, 0], o[:, 1], o[:, 2]
+    r["dx"], r["dy"], r["dz"] = d[:, 0], d[:, 1], d[:, 2]
+    r["tmin"], r["tmax"] = 0.0, 1e30
+    return r
+
+
+def boxes_workload(args):
+    """AABB-primitive accel (gprtAABBsSetPositions + gprtAABBAccelCreate): 4,194,304 boxes (centres
+    U[0,1]^3, half extents U[5e-4, 2.5e-3] per axis, seed 11); a step is one LBVH build over the
+    boxes and a closest-hit trace of 4,194,304 rays (origins U[0,1]^3, Gaussian directions, seed 12)."""
+    from gprt_amd import api
+
+    lib = api.load()
+    boxes = _box_scene(N_BOXES, 11)
+    rays = _box_rays(N_BOXES, 12)
+    ctx = api.Context(0)
+    acc = api.AABBAccel(ctx, boxes)
+    rb = ctx.device_buffer(rays)
+    hb = ctx.device_buffer(nbytes=16 * len(rays))
+    vis = ctx.device_buffer(nbytes=4 * len(rays))
+
+    def step():
+        acc.build()
+        acc.trace(rb, hb, len(rays))
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    K = args.steps
+    pk = kernel_report(lib, ctx)
+    build_ms = ms_of(pk, "lbvh_")[0] / K
+    trace_ms = ms_of(pk, "trace_closest")[0] / K
+    api.trace_detail(ctx, acc, rb, hb, len(rays), visits_buf=vis)
+    visits = float(vis.read(np.uint32).mean())
+    hits = hb.read(api.HIT_DTYPE)
+    roof = _roof_hbm("LBVH build over boxes (all build kernels)", N_BOXES * BOX_LBVH_BYTES_PER_PRIM, build_ms,
+                     bytes_per_prim=BOX_LBVH_BYTES_PER_PRIM)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as orc
+        m = args.cpu_build_tris
+        t0 = time.perf_counter()
+        orc.lbvh_build_boxes(boxes[:m])
+        cpu_build_s = time.perf_counter() - t0
+        sel = np.random.default_rng(0).choice(len(rays), 256, replace=False)
+        sub = boxes[: 1 << 16]
+        t0 = time.perf_counter()
+        orc.trace_boxes_bruteforce(rays[sel], sub, threads=1)
+        bf_s = time.perf_counter() - t0
+        cpu = {"value": m / cpu_build_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
+               "sample": f"oracle LBVH build of the first {m} boxes, single thread; oracle brute-force trace of 256 "
+                         f"rays against the first 65,536 boxes ({256 * 65536 / bf_s / 1e6:.1f} M ray-box tests/s)"}
+    out = {
+        "metric": "AABB-primitive LBVH build Mprims/s + closest-hit Mrays/s, 1 GPU",
+        "value": N_BOXES / (build_ms * 1e3), "unit": "Mprims/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded random boxes and rays)",
+        "config": {"workload": "4,194,304 boxes + 4,194,304 random rays; step = LBVH build + closest-hit trace",
+                   "boxes": N_BOXES, "rays": len(rays)},
+        "components": {"build_ms": build_ms, "trace_ms": trace_ms, "trace_mrays_s": len(rays) / (trace_ms * 1e3),
+                       "node_visits_per_ray": visits, "hit_rate": float((hits["primID"] >= 0).mean())},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": pk,
+    }
+    print(json.dumps(out))
+    for b in (rb, hb, vis):
+        b.destroy()
+    acc.destroy()
+    ctx.close()
+
+
+def compute_workload(args):
+    """User compute programs through gprtComputeLaunch: the reference sample's ComputeBounds
+    (sphere centre + radius -> box) over 16,777,216 spheres, timed with the library's per-launch
+    HIP events, and the host cost of a launch (back-to-back one-group launches, wall clock)."""
+    from gprt_amd import api
+
+    lib = api.load()
+    code = os.path.join(ROOT, "tests", "programs", "compute_programs.co")
+    n = N_SPHERES
+    rng = np.random.default_rng(13)
+    centres = rng.random((n, 3), dtype=np.float32)
+    radii = (rng.random(n, dtype=np.float32) * np.float32(0.01)).astype(np.float32)
+    ctx = api.Context(0)
+    module = api.Module(ctx, code)
+    prog = module.compute("ComputeBounds")
+    vb, rbuf = ctx.device_buffer(centres), ctx.device_buffer(radii)
+    ab = ctx.device_buffer(nbytes=24 * n)
+
+    class Rec(C.Structure):
+        _fields_ = [("vertex", C.c_void_p), ("radius", C.c_void_p), ("aabbs", C.c_void_p)]
+
+    dp = lambda b: lib.gprtBufferGetDevicePointer(b.h, 0)
+    rec = Rec(dp(vb), dp(rbuf), dp(ab))
+    groups = ((n + 1023) // 1024,)  # at most 65,535 groups per dimension
+    for _ in range(args.warmup):
+        prog.launch(groups, (1024,), rec, C.c_uint32(n))
+    ctx.sync()
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prog.launch(groups, (1024,), rec, C.c_uint32(n))
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    K = args.steps
+    launches = C.c_uint32(0)
+    ms = lib.gprtKernelTimerQuery(ctx.h, b"ComputeBounds", C.byref(launches)) / max(launches.value, 1)
+    got = ab.read(np.float32).reshape(-1, 6)
+    want_lo = centres - radii[:, None]
+    ok = bool(np.array_equal(got[:, :3], want_lo) and np.array_equal(got[:, 3:], centres + radii[:, None]))
+    reps = 2000
+    one = (1,)
+    small = Rec(dp(vb), dp(rbuf), dp(ab))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prog.launch(one, (64,), small, C.c_uint32(64))
+    host_us = (time.perf_counter() - t0) / reps * 1e6
+    ctx.sync()
+    roof = _roof_hbm("ComputeBounds (user program via gprtComputeLaunch)", n * BOUNDS_BYTES_PER_PRIM, ms,
+                     bytes_per_prim=BOUNDS_BYTES_PER_PRIM)
+    cpu = None
+    if not args.no_cpu:
+        m = 1 << 22
+        t0 = time.perf_counter()
+        for _ in range(3):
+            np.concatenate([centres[:m] - radii[:m, None], centres[:m] + radii[:m, None]], 1)
+        cpu_s = (time.perf_counter() - t0) / 3
+        cpu = {"value": m / cpu_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
+               "sample": f"numpy sphere bounds of the first {m} spheres, single thread"}
+    out = {
+        "metric": "user compute program (ComputeBounds) Mprims/s through gprtComputeLaunch, 1 GPU",
+        "value": n / (ms * 1e3), "unit": "Mprims/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded sphere centres and radii)",
+        "config": {"workload": "16,777,216 spheres -> boxes, 1024-thread groups", "spheres": n},
+        "components": {"kernel_ms": ms, "host_us_per_launch": host_us, "boxes_match_numpy": ok},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    for b in (vb, rbuf, ab):
+        b.destroy()
+    prog.destroy()
+    module.destroy()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="scene", choices=["scene", "batched", "dynamic", "copy"])
+    ap.add_argument("--workload", default="scene",
+                    choices=["scene", "batched", "dynamic", "copy", "refit", "boxes", "compute"])
     ap.add_argument("--frames", type=int, default=256, help="dynamic: timed frames")
     ap.add_argument("--tune-rays", type=int, default=262144, help="dynamic: labelled rays per fine-tune")
     ap.add_argument("--tune-epochs", type=int, default=2, help="dynamic: epochs per fine-tune")
@@ -549,6 +818,12 @@ def main():
         copy_workload(args)
     elif args.workload == "dynamic":
         dynamic_workload(args)
+    elif args.workload == "refit":
+        refit_workload(args)
+    elif args.workload == "boxes":
+        boxes_workload(args)
+    elif args.workload == "compute":
+        compute_workload(args)
     else:
         scene_workload(args)
 

@@ -139,15 +139,12 @@ struct BVHStore {
   void *centroids = nullptr;  // float[prims][3]: centroids in primitive order (written by the bounds pass)
   uint32_t *parentLeaf = nullptr, *parentInner = nullptr;  // filled on demand by lbvh_parents
   float *headers = nullptr;  // [meshes][16]
-  uint32_t *entries = nullptr;  // [prims] per-region lists of build edges into splits on sub-tile edges
-  void *pending = nullptr;      // [inner][2] x 32 B: child records waiting at a split (box, ref, far end)
-  void *pendDelta = nullptr;    // [inner][2] int8: the waiting child's outer key delta
-  uint32_t *entriesTop = nullptr;   // [prims] per-mesh lists of edges into splits on region edges
-  uint32_t *edgeCount = nullptr;    // [meshes] (in the zeroed range)
-  uint32_t *subCount = nullptr;     // [tiles * tileKeys / 1024]: edges of each 1024-key sub-tile
-  // Child arrivals per split: zeroed when the arrays are laid out; every build leaves them zero.
-  uint32_t *counters = nullptr;  // [inner]
-  size_t cntWords = 0;
+  // Finished subtrees passed between the hierarchy levels (32 B each, in leaf order) and the
+  // key delta before each: list0 from the 1024-key sub-tiles, list1 from the 65536-key regions.
+  void *list0 = nullptr, *list1 = nullptr;      // [prims]
+  void *listD0 = nullptr, *listD1 = nullptr;    // [prims] int8
+  uint32_t *subCount = nullptr;     // [tiles * tileKeys / 1024]: outputs of each 1024-key sub-tile (zeroed range)
+  uint32_t *regionCount = nullptr;  // [tiles]: outputs of each region, at its first tile (zeroed range)
   // Zeroed at the start of every build (one contiguous range):
   uint32_t *zeroBase = nullptr;
   size_t zeroWords = 0;

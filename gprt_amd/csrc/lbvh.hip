@@ -38,10 +38,10 @@ constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr int kMortonBits = 30;
 constexpr int kThreads = 256;  // non-sort kernels
 
-struct Pending {  // a child waiting at its parent's split: box[0..3], {box[4], box[5], ref, far end}
+struct Subtree {  // a finished subtree passed up a level: {box lo, ref}, {box hi, first leaf}
   float4 a, b;
 };
-static_assert(sizeof(Pending) == 32, "Pending must be 32 bytes");
+static_assert(sizeof(Subtree) == 32, "Subtree must be 32 bytes");
 
 struct BuildArgs {
   const MeshDesc *desc;
@@ -52,20 +52,17 @@ struct BuildArgs {
   BVHNode *nodes;
   TriRecord *tris;
   float *centroids;  // [prims][3]: triangle centroids in primitive order (bounds kernel -> key kernel)
-  uint32_t *parentLeaf, *parentInner, *counters;
+  uint32_t *parentLeaf, *parentInner;
   float *headers;
   uint32_t *hist, *status, *tickets;
-  Pending *pending;      // [inner][2]: child records waiting at a split, per side
-  int8_t *pendDelta;     // [inner][2]: each waiting child's outer delta (delta(l-1) of a left
-                         // child, delta(r) of a right child): the merged node's two deltas
-  // Build edges (split << 1 | side of a child whose parent a level cannot finish), at the mesh's
-  // primOffset: entries: from sub-tiles into splits on sub-tile edges, each sub-tile's list at its
-  // own key positions (count at subCount[tile * subPerTile + sub]); entriesTop: from regions into
-  // splits on region edges, one list per mesh (count at edgeCount[mesh]). counters: arrivals
-  // per split.
-  uint32_t *entries, *entriesTop;
+  // Subtrees that a level passes to the next, in leaf order, with delta(first leaf - 1) beside
+  // them: list0 holds each 1024-key sub-tile's outputs at the sub-tile's own key positions (count
+  // at subCount[tile * subPerTile + sub]); list1 each 65536-key region's outputs at the region's
+  // key positions (count at regionCount[first tile of the region]). The top level reuses list0.
+  Subtree *list0, *list1;
+  int8_t *listD0, *listD1;
   uint32_t *subCount;
-  uint32_t *edgeCount;
+  uint32_t *regionCount;
   int passes;
   int digitBits;
 };
@@ -462,26 +459,23 @@ __global__ void __launch_bounds__(256) k_copy_back(BuildArgs a, uint32_t prims) 
 // writes the finished 64-byte node at its Karras index and climbs on (the atomic refit of
 // gprt_lbvh.hlsl:400-428, with every node combined exactly once).
 //
-// Three levels, so that no arrival counter needs ordering across XCDs:
+// Three levels, each merging in LDS within one workgroup (no arrival counter in global memory):
 //  1. k_build_sub: one 1024-thread workgroup per 1024-key sub-tile. The sub-tile's deltas, leaf
 //     boxes, finished node boxes and the meeting slots of its interior splits live in LDS. The
 //     climb runs in rounds separated by barriers: round 0 sends every leaf to its parent's split,
 //     and each later round merges the splits whose second child arrived in the previous round,
 //     one per thread from a queue, so the shrinking work stays packed in the first waves instead
-//     of leaving one active lane in every wave. A child whose parent is split on the sub-tile's
-//     edge leaves its record in `pending` and becomes an edge of its 65536-key region; a meeting
-//     slot still half-filled at the end (the sibling spans sub-tiles) is copied to `pending` with
-//     its split's counter set to 1.
-//  2. k_build_region: one workgroup per region climbs the region's edges through global counters
-//     ordered at workgroup scope (the workgroup's waves share one CU); climbs that leave the
-//     region become edges of the mesh.
-//  3. k_build_top: one workgroup per mesh finishes the nodes spanning regions.
+//     of leaving one active lane in every wave. The subtrees whose parents span sub-tiles (their
+//     parent's split lies on the sub-tile's edge, or their sibling never arrived) partition the
+//     sub-tile; they are written in leaf order as its outputs.
+//  2. k_build_region: one workgroup per 65536-key region treats its sub-tiles' outputs as leaves
+//     and merges them by the same rule (merge_sequence below); its outputs partition the region.
+//  3. k_build_top: one workgroup per mesh merges its regions' outputs up to the root.
 // Kernel boundaries order the levels.
 // ------------------------------------------------------------------------------------------------
 constexpr int kBuildSub = 1024;
 constexpr uint32_t kBuildRegion = 65536;  // keys per level-2 region (a multiple of every tile size)
-constexpr int kBuildTop = 512;
-constexpr uint32_t kNoEdge = 0xFFFFFFFFu;
+constexpr int kNoOutput = 0x7FFFFFFF;  // no output subtree starts at this leaf
 
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
 __device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
@@ -491,25 +485,6 @@ __device__ __forceinline__ void lds_barrier() {
   lds_release();
   __builtin_amdgcn_s_barrier();
   lds_acquire();
-}
-
-__device__ __forceinline__ void put_pending(Pending *p, int8_t *pd, const float b[6], int ref, uint32_t end,
-                                            int outerDelta) {
-  p->a = make_float4(b[0], b[1], b[2], b[3]);
-  p->b = make_float4(b[4], b[5], __int_as_float(ref), __uint_as_float(end));
-  *pd = (int8_t)outerDelta;
-}
-
-__device__ __forceinline__ void get_pending(const Pending *p, float b[6], int &ref, uint32_t &end) {
-  const float4 x = p->a, y = p->b;
-  b[0] = x.x;
-  b[1] = x.y;
-  b[2] = x.z;
-  b[3] = x.w;
-  b[4] = y.x;
-  b[5] = y.y;
-  ref = __float_as_int(y.z);
-  end = __float_as_uint(y.w);
 }
 
 __device__ __forceinline__ void load_child_box(const BVHNode *nodes, uint32_t node, uint32_t side, float b[6]) {
@@ -542,6 +517,11 @@ __device__ __forceinline__ void write_node(BVHNode *node, const float b0[6], con
   dst[3] = make_float4(__int_as_float(c0), __int_as_float(c1), __uint_as_float(other), 0.0f);
 }
 
+__device__ __forceinline__ void put_entry(Subtree *p, const float b[6], int ref, uint32_t first) {
+  p->a = make_float4(b[0], b[1], b[2], __int_as_float(ref));
+  p->b = make_float4(b[3], b[4], b[5], __uint_as_float(first));
+}
+
 __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
   __shared__ int8_t s_delta[kBuildSub + 4];        // delta(lo - 1 + q), in [-1, 63]
   __shared__ uint16_t s_split[kBuildSub];          // split of finished local node lo + q (0xFFFF: none)
@@ -549,9 +529,11 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   __shared__ float s_box[kBuildSub][6];            // finished inner node boxes (index - lo)
   __shared__ uint32_t s_meet[kBuildSub];           // sides arrived at split lo + q: bit 0 left, bit 1 right
   __shared__ int s_ref[kBuildSub][2];              // the arrived child per side (inner index or ~leaf)
-  __shared__ uint32_t s_end[kBuildSub][2];         // its far leaf (l of a left child, r of a right child)
-  __shared__ uint32_t s_ready[2][kBuildSub / 2];   // splits whose two children have arrived
-  __shared__ uint32_t s_nready[4];                 // [0..2] ready-list sizes by round, [3] edges
+  __shared__ uint16_t s_end[kBuildSub][2];         // its far leaf - lo (l of a left child, r of a right child)
+  __shared__ uint16_t s_ready[2][kBuildSub / 2];   // splits whose two children have arrived
+  __shared__ uint32_t s_nready[3];                 // ready-list sizes by round
+  __shared__ int s_out[kBuildSub];                 // reference of the output subtree starting at lo + q
+  __shared__ uint32_t s_wsum[kBuildSub / 64 + 1];  // output counts per wave, then their prefix
   const uint32_t t = tileBase + blockIdx.x / subPerTile, sub = blockIdx.x % subPerTile;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
@@ -567,8 +549,6 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   const uint32_t *__restrict__ codes = a.codes + d.primOffset;
   const uint32_t *__restrict__ ids = a.ids + d.primOffset;
   BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
-  Pending *__restrict__ pending = a.pending + 2 * (size_t)d.innerOffset;
-  int8_t *__restrict__ pendDelta = a.pendDelta + 2 * (size_t)d.innerOffset;
   float *__restrict__ header = a.headers + (size_t)m * 16;
   const uint32_t tid = threadIdx.x;
   const uint32_t k = lo + tid;  // leaf position
@@ -586,9 +566,10 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
     cp = codes[lo - 1];
     ip = ids[lo - 1];
   }
-  if (tid < 4) s_nready[tid] = 0u;
+  if (tid < 3) s_nready[tid] = 0u;
   s_meet[tid] = 0u;
   s_split[tid] = 0xFFFFu;
+  s_out[tid] = kNoOutput;
   float box[6];
   if (k < hi) {
     float3 p, q, s;
@@ -619,9 +600,9 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   lds_barrier();
 
   // The node [l, rr] (reference `ref`, box `b`, already in LDS) arrives at its parent's split: the
-  // root ends the climb, a split on the sub-tile's edge becomes an edge for the region level, and
-  // the second arrival at an interior split queues it for the next round.
-  auto arrive = [&](uint32_t l, uint32_t rr, int ref, const float b[6], uint32_t nextQ, uint32_t &edge) {
+  // root ends the climb, a split on the sub-tile's edge makes the node an output of the sub-tile,
+  // and the second arrival at an interior split queues it for the next round.
+  auto arrive = [&](uint32_t l, uint32_t rr, int ref, const float b[6], uint32_t nextQ) {
     const int dl = s_delta[l - lo], dr = s_delta[rr - lo + 1];
     if (dl < 0 && dr < 0) {  // [l, rr] = [0, n - 1]: the root (or the only leaf)
       for (int c = 0; c < 6; ++c) header[8 + c] = b[c];
@@ -630,48 +611,25 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
     const bool leftChild = dr > dl;
     const uint32_t split = leftChild ? rr : l - 1u;
     const uint32_t side = leftChild ? 0u : 1u;
-    const uint32_t end = leftChild ? l : rr;
     if (split < lo || split + 1u >= hi) {
-      put_pending(&pending[2 * (size_t)split + side], &pendDelta[2 * (size_t)split + side], b, ref, end,
-                  leftChild ? dl : dr);
-      edge = (split << 1) | side;
+      s_out[l - lo] = ref;
       return;
     }
     const uint32_t g = split - lo;
     s_ref[g][side] = ref;
-    s_end[g][side] = end;
+    s_end[g][side] = (uint16_t)((leftChild ? l : rr) - lo);
     const uint32_t old =
         __hip_atomic_fetch_or(&s_meet[g], 1u << side, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old != 0u) {
       const uint32_t slot =
           __hip_atomic_fetch_add(&s_nready[nextQ % 3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      s_ready[nextQ & 1][slot] = g;
+      s_ready[nextQ & 1][slot] = (uint16_t)g;
     }
-  };
-  // Appends this wave's edges to the sub-tile's list (one LDS atomic per wave). The list lives at
-  // the sub-tile's own key positions: it holds at most one edge per key (each edge's child roots a
-  // disjoint subtree of the sub-tile's leaves); the guard keeps writes inside the sub-tile.
-  auto flush_edges = [&](uint32_t edge) {
-    const uint64_t mask = __ballot(edge != kNoEdge);
-    if (!mask) return;
-    const int lane = (int)(threadIdx.x & 63u);
-    const int first = __ffsll((long long)mask) - 1;
-    const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (lane == first)
-      base = __hip_atomic_fetch_add(&s_nready[3], (uint32_t)__popcll(mask), __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-    base = __shfl(base, first, 64);
-    if (edge != kNoEdge && base + before < hi - lo) a.entries[d.primOffset + lo + base + before] = edge;
   };
 
   // Round 0: the leaves. Round i > 0 merges the splits queued in round i - 1, so the active climbs
   // are packed into the first waves; the barrier between rounds orders every LDS write.
-  {
-    uint32_t edge = kNoEdge;
-    if (k < hi) arrive(k, k, ~(int)k, box, 1u, edge);
-    flush_edges(edge);
-  }
+  if (k < hi) arrive(k, k, ~(int)k, box, 1u);
   lds_barrier();
   // A split is queued once, so rounds never exceed the sub-tile's keys; the bound only guards
   // against malformed keys.
@@ -679,11 +637,10 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
     const uint32_t count = s_nready[round % 3];
     if (count == 0u) break;  // uniform: read after the barrier
     if (tid == 0) s_nready[(round + 2) % 3] = 0u;  // the list the next round fills
-    uint32_t edge = kNoEdge;
     if (tid < count) {
       const uint32_t g = s_ready[round & 1][tid];
       const int c0 = s_ref[g][0], c1 = s_ref[g][1];
-      const uint32_t pl = s_end[g][0], pr = s_end[g][1];
+      const uint32_t pl = lo + s_end[g][0], pr = lo + s_end[g][1];
       const float *p0 = c0 < 0 ? s_leaf[(uint32_t)(~c0) - lo] : s_box[(uint32_t)c0 - lo];
       const float *p1 = c1 < 0 ? s_leaf[(uint32_t)(~c1) - lo] : s_box[(uint32_t)c1 - lo];
       float b0[6], b1[6];
@@ -696,18 +653,16 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
       s_split[P - lo] = (uint16_t)g;  // written whole in the final pass
       merge_box(b0, b1);
       for (int c = 0; c < 6; ++c) s_box[P - lo][c] = b0[c];
-      arrive(pl, pr, (int)P, b0, round + 1u, edge);
+      arrive(pl, pr, (int)P, b0, round + 1u);
     }
-    flush_edges(edge);
     lds_barrier();
   }
-  if (tid == 0) *subCount = min(s_nready[3], hi - lo);
   // Finished local nodes, written whole and coalesced (node lo + tid by thread tid).
   {
     const uint32_t g = s_split[tid];
     if (g != 0xFFFFu) {
       const int c0 = s_ref[g][0], c1 = s_ref[g][1];
-      const uint32_t pl = s_end[g][0], pr = s_end[g][1];
+      const uint32_t pl = lo + s_end[g][0], pr = lo + s_end[g][1];
       const float *p0 = c0 < 0 ? s_leaf[(uint32_t)(~c0) - lo] : s_box[(uint32_t)c0 - lo];
       const float *p1 = c1 < 0 ? s_leaf[(uint32_t)(~c1) - lo] : s_box[(uint32_t)c1 - lo];
       float b0[6], b1[6];
@@ -719,113 +674,352 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
       write_node(&nodes[P], b0, b1, c0, c1, P == pl ? pr : pl);
     }
   }
-  // Interior splits [lo, hi - 2] with one child arrived: the other child spans sub-tiles and
-  // arrives at a later level, which then finds the record here and the counter at 1.
+  // Interior splits [lo, hi - 2] with one child arrived: the other child spans sub-tiles, so the
+  // arrived one is an output too.
   if (tid + 1u < hi - lo) {
     const uint32_t meet = s_meet[tid];
     if (meet == 1u || meet == 2u) {
       const uint32_t side = meet >> 1;
-      const int ref = s_ref[tid][side];
+      s_out[side == 0 ? s_end[tid][0] : tid + 1u] = s_ref[tid][side];
+    }
+  }
+  lds_barrier();
+  // The outputs partition the sub-tile's leaves; written in leaf order, the next level reads them
+  // as a sequence.
+  {
+    const int ref = s_out[tid];
+    const bool out = ref != kNoOutput;
+    const uint64_t mask = __ballot(out);
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(mask);
+    lds_barrier();
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (uint32_t w = 0; w < kBuildSub / 64; ++w) {
+        const uint32_t c = s_wsum[w];
+        s_wsum[w] = run;
+        run += c;
+      }
+      s_wsum[kBuildSub / 64] = run;
+      *subCount = run;
+    }
+    lds_barrier();
+    if (out) {
+      const uint32_t rank = s_wsum[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
       const float *b = ref < 0 ? s_leaf[(uint32_t)(~ref) - lo] : s_box[(uint32_t)ref - lo];
       float cb[6];
       for (int c = 0; c < 6; ++c) cb[c] = b[c];
-      // The child's outer delta: delta(l-1) of a left child, delta(r) of a right one.
-      const uint32_t end = s_end[tid][side];
-      const int outer = side == 0 ? s_delta[end - lo] : s_delta[end - lo + 1];
-      put_pending(&pending[2 * (size_t)(lo + tid) + side], &pendDelta[2 * (size_t)(lo + tid) + side], cb, ref, end,
-                  outer);
-      a.counters[d.innerOffset + lo + tid] = 1u;
+      const size_t at = (size_t)d.primOffset + lo + rank;
+      put_entry(&a.list0[at], cb, ref, lo + tid);
+      a.listD0[at] = s_delta[tid];
     }
   }
 }
 
-// Climbs from the child record waiting at `edge` (split << 1 | side) while the parents' splits
-// are interior to [rangeLo, rangeHi); returns the edge that leaves the range (kNoEdge when the
-// climb ended at a first arrival or at the root).
-__device__ __forceinline__ uint32_t build_climb(const BuildArgs &a, const MeshDesc &d, uint32_t m, uint32_t edge,
-                                                uint32_t rangeLo, uint32_t rangeHi) {
-  const uint32_t n = d.ntri;
-  BVHNode *__restrict__ nodes = a.nodes + d.innerOffset;
-  Pending *__restrict__ pending = a.pending + 2 * (size_t)d.innerOffset;
-  int8_t *__restrict__ pendDelta = a.pendDelta + 2 * (size_t)d.innerOffset;
-  uint32_t *__restrict__ counters = a.counters + d.innerOffset;
-  for (int level = 0; level < 72; ++level) {
-    const uint32_t split = edge >> 1, side = edge & 1u;
-    if (split >= n - 1) return kNoEdge;  // only a malformed edge
-    if (split < rangeLo || split + 1u >= rangeHi) return edge;
-    const uint32_t old = __hip_atomic_fetch_add(&counters[split], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old != 1u) return kNoEdge;  // first arrival
-    // Both children are in: no one touches this counter again in this build, so it goes back to
-    // zero here and the next build needs no clearing pass.
-    counters[split] = 0u;
-    float b0[6], b1[6];
-    int c0, c1;
-    uint32_t pl, pr;
-    get_pending(&pending[2 * (size_t)split], b0, c0, pl);
-    get_pending(&pending[2 * (size_t)split + 1], b1, c1, pr);
-    const int dl = pendDelta[2 * (size_t)split], dr = pendDelta[2 * (size_t)split + 1];
-    const bool root = dl < 0 && dr < 0;
-    const uint32_t P = root ? 0u : (dr > dl ? pr : pl);
-    write_node(&nodes[P], b0, b1, c0, c1, P == pl ? pr : pl);
-    merge_box(b0, b1);
-    if (root) {
-      float *__restrict__ header = a.headers + (size_t)m * 16;
-      for (int c = 0; c < 6; ++c) header[8 + c] = b0[c];
-      return kNoEdge;
+// ------------------------------------------------------------------------------------------------
+// Levels 2 and 3: merging a sequence of subtrees in LDS.
+//
+// A level's input is the outputs of the level below, in leaf order: subtrees that partition its
+// range, each with its box, its reference (inner index or ~leaf), its first leaf L and delta(L-1).
+// Treating each subtree as one leaf, the same rule builds the nodes above them: with D[i] =
+// delta(L_i - 1) and D[S] the delta after the range, the node over subtrees [a, b] goes to the
+// boundary after b when D[b+1] > D[a] (a left child), else to the boundary before a. Merges run
+// in rounds in LDS as in the sub-tile; nodes whose parent lies outside the window, and nodes still
+// waiting for a sibling at the end, are the window's outputs, again in leaf order.
+//
+// A window holds at most kMergeCap subtrees. A longer sequence is merged window by window, the
+// outputs of each window replacing its inputs at the front of the same list, until one window
+// holds the rest: a window of more than 130 subtrees always merges some (adjacent boundary deltas
+// of subtrees differ, and a sequence of values in [-1, 63] without an interior local maximum has
+// at most 130 entries), and its outputs are at most 2 x 65 subtrees (one per tree level on each
+// side).
+// ------------------------------------------------------------------------------------------------
+constexpr int kMergeThreads = 1024;
+constexpr uint32_t kMergeCap = 2048;
+
+struct MergeLds {
+  float leafBox[kMergeCap][6];   // input subtrees' boxes
+  float nodeBox[kMergeCap][6];   // merged nodes' boxes, by boundary
+  int ref[kMergeCap];            // input subtrees' references
+  uint32_t node[kMergeCap];      // merged nodes' inner indices, by boundary
+  uint32_t L[kMergeCap + 1];     // first leaf of each input subtree; [S]: the position after the window
+  int8_t D[kMergeCap + 4];       // delta(L - 1) of each input subtree; [S]: the delta after the window
+  uint32_t meet[kMergeCap / 4];  // sides arrived per boundary, 8 bits each
+  uint16_t child[kMergeCap][2];  // the arrived child per side: local id (< kMergeCap input, else node)
+  uint16_t far[kMergeCap][2];    // its far input subtree (first of a left child, last of a right child)
+  uint16_t ready[2][kMergeCap / 2];
+  uint16_t out[kMergeCap];       // local id of the output starting at each input subtree (0xFFFF: none)
+  uint32_t nready[3];
+  uint32_t wsum[kMergeThreads / 64 + 1];
+};
+
+// Merges input subtrees [0, S) (already in LDS: leafBox, ref, L[0..S], D[0..S]) and appends the
+// window's outputs at out[*outCount...] (outD: their deltas). Called by every thread.
+__device__ void merge_window(MergeLds &sm, uint32_t S, uint32_t n, BVHNode *__restrict__ nodes, float *__restrict__ header,
+                             Subtree *__restrict__ out, int8_t *__restrict__ outD, uint32_t outBase,
+                             uint32_t &outCount) {
+  const uint32_t tid = threadIdx.x;
+  auto box_of = [&](uint32_t id) -> const float * { return id < kMergeCap ? sm.leafBox[id] : sm.nodeBox[id - kMergeCap]; };
+  auto ref_of = [&](uint32_t id) -> int { return id < kMergeCap ? sm.ref[id] : (int)sm.node[id - kMergeCap]; };
+  // The node over input subtrees [fa, fb] (local id `id`, box in LDS) arrives at its parent's
+  // boundary.
+  auto arrive = [&](uint32_t fa, uint32_t fb, uint32_t id, uint32_t nextQ) {
+    const int dl = sm.D[fa], dr = sm.D[fb + 1];
+    if (dl < 0 && dr < 0) {  // the whole mesh: the root
+      const float *b = box_of(id);
+      for (int c = 0; c < 6; ++c) header[8 + c] = b[c];
+      return;
     }
     const bool leftChild = dr > dl;
-    const uint32_t up = leftChild ? pr : pl - 1u;
-    const uint32_t upSide = leftChild ? 0u : 1u;
-    put_pending(&pending[2 * (size_t)up + upSide], &pendDelta[2 * (size_t)up + upSide], b0, (int)P,
-                leftChild ? pl : pr, leftChild ? dl : dr);
-    edge = (up << 1) | upSide;
+    if (leftChild ? fb + 1u >= S : fa == 0u) {  // the parent lies outside the window
+      sm.out[fa] = (uint16_t)id;
+      return;
+    }
+    const uint32_t g = leftChild ? fb : fa - 1u, side = leftChild ? 0u : 1u;
+    sm.child[g][side] = (uint16_t)id;
+    sm.far[g][side] = (uint16_t)(leftChild ? fa : fb);
+    const uint32_t shift = (g & 3u) * 8u;
+    const uint32_t old = __hip_atomic_fetch_or(&sm.meet[g >> 2], (1u << side) << shift, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    if ((old >> shift) & 3u) {
+      const uint32_t slot =
+          __hip_atomic_fetch_add(&sm.nready[nextQ % 3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      sm.ready[nextQ & 1][slot] = (uint16_t)g;
+    }
+  };
+  for (uint32_t i = tid; i < kMergeCap / 4; i += kMergeThreads) sm.meet[i] = 0u;
+  for (uint32_t i = tid; i < S; i += kMergeThreads) sm.out[i] = 0xFFFFu;
+  if (tid < 3) sm.nready[tid] = 0u;
+  lds_barrier();
+  for (uint32_t i = tid; i < S; i += kMergeThreads) arrive(i, i, i, 1u);
+  lds_barrier();
+  for (uint32_t round = 1; round <= kMergeCap; ++round) {
+    const uint32_t count = sm.nready[round % 3];
+    if (count == 0u) break;  // uniform: read after the barrier
+    if (tid == 0) sm.nready[(round + 2) % 3] = 0u;
+    for (uint32_t q = tid; q < count; q += kMergeThreads) {
+      const uint32_t g = sm.ready[round & 1][q];
+      const uint32_t i0 = sm.child[g][0], i1 = sm.child[g][1];
+      const uint32_t fa = sm.far[g][0], fb = sm.far[g][1];
+      const uint32_t pl = sm.L[fa], pr = sm.L[fb + 1] - 1u;
+      const int pdl = sm.D[fa], pdr = sm.D[fb + 1];
+      const uint32_t P = (pdl < 0 && pdr < 0) ? 0u : (pdr > pdl ? pr : pl);
+      const float *p0 = box_of(i0), *p1 = box_of(i1);
+      float b0[6], b1[6];
+      for (int c = 0; c < 6; ++c) {
+        b0[c] = p0[c];
+        b1[c] = p1[c];
+      }
+      if (P + 1u < n) write_node(&nodes[P], b0, b1, ref_of(i0), ref_of(i1), P == pl ? pr : pl);
+      merge_box(b0, b1);
+      for (int c = 0; c < 6; ++c) sm.nodeBox[g][c] = b0[c];
+      sm.node[g] = P;
+      arrive(fa, fb, kMergeCap + g, round + 1u);
+    }
+    lds_barrier();
   }
-  return kNoEdge;
+  // Boundaries with one child arrived: its sibling lies partly outside the window.
+  for (uint32_t g = tid; g + 1u < S; g += kMergeThreads) {
+    const uint32_t meet = (sm.meet[g >> 2] >> ((g & 3u) * 8u)) & 3u;
+    if (meet == 1u || meet == 2u) {
+      const uint32_t side = meet >> 1;
+      sm.out[side == 0 ? sm.far[g][0] : g + 1u] = sm.child[g][side];
+    }
+  }
+  lds_barrier();
+  // Outputs in leaf order: thread t takes input subtrees [2t, 2t + 1] (kMergeCap = 2 x threads).
+  const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+  const bool o0 = i0 < S && sm.out[i0] != 0xFFFFu, o1 = i1 < S && sm.out[i1] != 0xFFFFu;
+  const uint32_t mine = (uint32_t)o0 + (uint32_t)o1;
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  const uint32_t incl = wave_incl_sum(mine);
+  if (lane == 63u) sm.wsum[wave] = incl;
+  lds_barrier();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < kMergeThreads / 64; ++w) {
+      const uint32_t c = sm.wsum[w];
+      sm.wsum[w] = run;
+      run += c;
+    }
+    sm.wsum[kMergeThreads / 64] = run;
+  }
+  lds_barrier();
+  uint32_t rank = outBase + outCount + sm.wsum[wave] + incl - mine;
+  for (uint32_t j = 0; j < 2; ++j) {
+    const uint32_t i = i0 + j;
+    if (!(j == 0 ? o0 : o1)) continue;
+    const uint32_t id = sm.out[i];
+    const float *pb = box_of(id);
+    float b[6];
+    for (int c = 0; c < 6; ++c) b[c] = pb[c];
+    put_entry(&out[rank], b, ref_of(id), sm.L[i]);
+    outD[rank] = sm.D[i];
+    ++rank;
+  }
+  outCount += sm.wsum[kMergeThreads / 64];
+  // The outputs may overwrite inputs of this list that the next window or pass loads.
+  __syncthreads();
 }
 
-__global__ void __launch_bounds__(kBuildTop) k_build_region(BuildArgs a, uint32_t tileBase) {
+// Loads input subtrees [w0, w0 + S) of a sequence into LDS; `at(e)` gives entry e's index in
+// `in` / `inD`. endL / endD describe what follows the sequence when the window reaches its end.
+template <class At>
+__device__ void load_window(MergeLds &sm, const Subtree *__restrict__ in, const int8_t *__restrict__ inD, At at,
+                            uint32_t w0, uint32_t S, uint32_t total, uint32_t endL, int endD) {
+  for (uint32_t i = threadIdx.x; i <= S; i += kMergeThreads) {
+    const uint32_t e = w0 + i;
+    if (e < total) {
+      const size_t x = at(e);
+      const float4 p = in[x].a, q = in[x].b;
+      sm.L[i] = __float_as_uint(q.w);
+      sm.D[i] = inD[x];
+      if (i < S) {
+        sm.leafBox[i][0] = p.x;
+        sm.leafBox[i][1] = p.y;
+        sm.leafBox[i][2] = p.z;
+        sm.leafBox[i][3] = q.x;
+        sm.leafBox[i][4] = q.y;
+        sm.leafBox[i][5] = q.z;
+        sm.ref[i] = __float_as_int(p.w);
+      }
+    } else {  // i == S at the end of the sequence
+      sm.L[i] = endL;
+      sm.D[i] = (int8_t)endD;
+    }
+  }
+  lds_barrier();
+}
+
+// Merges a whole sequence of `total` input subtrees: window by window (outputs compacted into
+// `tmp` / `tmpD` from its start) until one window holds what is left, whose outputs go to `dst`.
+// `at` maps entry e of the first pass to its index in `in`. Returns the number of outputs.
+template <class At>
+__device__ uint32_t merge_sequence(MergeLds &sm, const Subtree *in, const int8_t *inD, At at, uint32_t total,
+                                   uint32_t endL, int endD, Subtree *tmp, int8_t *tmpD, Subtree *dst, int8_t *dstD,
+                                   uint32_t n, BVHNode *nodes, float *header) {
+  bool first = true;
+  // Each pass leaves at most 130 outputs per window of kMergeCap, so passes end; the bound only
+  // guards against malformed keys.
+  for (int pass = 0; pass < 16 && total > 0; ++pass) {
+    const bool last = total <= kMergeCap;
+    uint32_t outCount = 0;
+    for (uint32_t w0 = 0; w0 < total; w0 += kMergeCap) {
+      const uint32_t S = min(kMergeCap, total - w0);
+      if (first) load_window(sm, in, inD, at, w0, S, total, endL, endD);
+      else load_window(sm, tmp, tmpD, [](uint32_t e) { return (size_t)e; }, w0, S, total, endL, endD);
+      merge_window(sm, S, n, nodes, header, last ? dst : tmp, last ? dstD : tmpD, 0u, outCount);
+    }
+    if (last) return outCount;
+    first = false;
+    total = outCount;
+  }
+  return 0u;
+}
+
+// Level 2: one workgroup per 65536-key region merges the outputs of its sub-tiles (each sub-tile's
+// list at its own key positions in list0) into list1 at the region's key positions.
+__global__ void __launch_bounds__(kMergeThreads) k_build_region(BuildArgs a, uint32_t tileBase) {
+  __shared__ MergeLds sm;
+  __shared__ uint32_t s_pre[kBuildRegion / kBuildSub + 1];
   const uint32_t t = tileBase + blockIdx.x;
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const uint32_t lt = t - d.tileOffset;
   const uint32_t perRegion = kBuildRegion / a.tileKeys;
   if (lt % perRegion != 0) return;  // one workgroup per region: its first tile
+  const uint32_t n = d.ntri;
   const uint32_t regionLo = lt * a.tileKeys;
-  const uint32_t regionHi = min(regionLo + kBuildRegion, d.ntri);
-  // The region's sub-tiles are the sub-tiles of its consecutive tiles: gather their edge counts.
+  const uint32_t regionHi = min(regionLo + kBuildRegion, n);
   constexpr uint32_t kSubs = kBuildRegion / kBuildSub;
-  __shared__ uint32_t s_pre[kSubs + 1];
   const uint32_t subPerTile = a.tileKeys / kBuildSub;
   const uint32_t nsub = (regionHi - regionLo + kBuildSub - 1) / kBuildSub;
   if (threadIdx.x < 64) {
-    const uint32_t c = threadIdx.x < nsub ? min(a.subCount[(size_t)t * subPerTile + threadIdx.x], (uint32_t)kBuildSub) : 0u;
+    // a sub-tile's outputs partition its keys: never more than it holds
+    const uint32_t c = threadIdx.x < nsub ? min(a.subCount[(size_t)t * subPerTile + threadIdx.x],
+                                                min((uint32_t)kBuildSub, regionHi - regionLo - threadIdx.x * kBuildSub))
+                                          : 0u;
     s_pre[threadIdx.x + 1] = wave_incl_sum(c);
     if (threadIdx.x == 0) s_pre[0] = 0u;
   }
+  static_assert(kSubs == 64, "one wave gathers the sub-tile counts");
+  // delta(regionHi - 1): what follows the region's last subtree.
+  int endD = -1;
+  if (regionHi < n) {
+    const uint32_t *codes = a.codes + d.primOffset, *ids = a.ids + d.primOffset;
+    endD = __clzll((((uint64_t)codes[regionHi - 1] << 32) | ids[regionHi - 1]) ^
+                   (((uint64_t)codes[regionHi] << 32) | ids[regionHi]));
+  }
   __syncthreads();
-  const uint32_t count = s_pre[nsub];
-  for (uint32_t e = threadIdx.x; e < count; e += kBuildTop) {
+  const uint32_t total = s_pre[nsub];
+  const size_t base = (size_t)d.primOffset + regionLo;
+  auto at = [&](uint32_t e) -> size_t {
     uint32_t j = 0, jh = nsub;  // the sub-tile j with s_pre[j] <= e < s_pre[j + 1]
     while (jh - j > 1) {
       const uint32_t mid = (j + jh) >> 1;
       if (s_pre[mid] <= e) j = mid;
       else jh = mid;
     }
-    const uint32_t entry = a.entries[d.primOffset + regionLo + j * kBuildSub + (e - s_pre[j])];
-    const uint32_t up = build_climb(a, d, m, entry, regionLo, regionHi);
-    if (up != kNoEdge) {
-      const uint32_t slot = atomicAdd(&a.edgeCount[m], 1u);
-      if (slot < d.ntri) a.entriesTop[d.primOffset + slot] = up;
-    }
-  }
+    return base + (size_t)j * kBuildSub + (e - s_pre[j]);
+  };
+  const uint32_t outs = merge_sequence(sm, a.list0, a.listD0, at, total, regionHi, endD, a.list1 + base,
+                                       a.listD1 + base, a.list1 + base, a.listD1 + base, n, a.nodes + d.innerOffset,
+                                       a.headers + (size_t)m * 16);
+  if (threadIdx.x == 0) a.regionCount[t] = outs;
 }
 
-__global__ void __launch_bounds__(kBuildTop) k_build_top(BuildArgs a, uint32_t meshBase) {
+// Level 3: one workgroup per mesh gathers its regions' outputs into list0 (from the mesh's start)
+// and merges them up to the root.
+__global__ void __launch_bounds__(kMergeThreads) k_build_top(BuildArgs a, uint32_t meshBase) {
+  __shared__ MergeLds sm;
   const uint32_t m = meshBase + blockIdx.x;
   const MeshDesc d = a.desc[m];
-  const uint32_t count = min(a.edgeCount[m], d.ntri);
-  for (uint32_t e = threadIdx.x; e < count; e += kBuildTop)
-    build_climb(a, d, m, a.entriesTop[d.primOffset + e], 0u, d.ntri);
+  const uint32_t n = d.ntri;
+  if (n <= kBuildRegion) return;  // the region level reached the root
+  const uint32_t perRegion = kBuildRegion / a.tileKeys;
+  const uint32_t regions = (n + kBuildRegion - 1) / kBuildRegion;
+  const size_t base = d.primOffset;
+  uint32_t total = 0;
+  uint32_t *pre = sm.L;  // a chunk's region-count prefix (kMergeThreads + 1 words)
+  for (uint32_t r0 = 0; r0 < regions; r0 += kMergeThreads) {
+    const uint32_t r = r0 + threadIdx.x;
+    const uint32_t c = r < regions ? min(a.regionCount[d.tileOffset + r * perRegion], min(kBuildRegion, n - r * kBuildRegion)) : 0u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_sum(c);
+    if (lane == 63u) sm.wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (uint32_t w = 0; w < kMergeThreads / 64; ++w) {
+        const uint32_t v = sm.wsum[w];
+        sm.wsum[w] = run;
+        run += v;
+      }
+      sm.wsum[kMergeThreads / 64] = run;
+    }
+    __syncthreads();
+    pre[threadIdx.x + 1] = sm.wsum[wave] + incl;
+    if (threadIdx.x == 0) pre[0] = 0u;
+    __syncthreads();
+    const uint32_t chunk = pre[kMergeThreads], nr = min(regions - r0, (uint32_t)kMergeThreads);
+    for (uint32_t e = threadIdx.x; e < chunk; e += kMergeThreads) {
+      uint32_t j = 0, jh = nr;
+      while (jh - j > 1) {
+        const uint32_t mid = (j + jh) >> 1;
+        if (pre[mid] <= e) j = mid;
+        else jh = mid;
+      }
+      const size_t src = base + (size_t)(r0 + j) * kBuildRegion + (e - pre[j]);
+      const size_t dst = base + total + e;
+      a.list0[dst] = a.list1[src];
+      a.listD0[dst] = a.listD1[src];
+    }
+    total += chunk;
+    __syncthreads();
+  }
+  merge_sequence(sm, a.list0 + base, a.listD0 + base, [](uint32_t e) { return (size_t)e; }, total, n, -1,
+                 a.list0 + base, a.listD0 + base, a.list0 + base, a.listD0 + base, n, a.nodes + d.innerOffset,
+                 a.headers + (size_t)m * 16);
 }
 
 // Parent links (inner index << 1 | side; 0xFFFFFFFF at the root) of one mesh, derived from the
@@ -924,17 +1118,16 @@ BuildArgs args_of(BVHStore *st) {
   a.centroids = (float *)st->centroids;
   a.parentLeaf = st->parentLeaf;
   a.parentInner = st->parentInner;
-  a.counters = st->counters;
   a.headers = st->headers;
   a.hist = st->hist;
   a.status = st->status;
   a.tickets = st->tickets;
-  a.entries = st->entries;
-  a.pending = (Pending *)st->pending;
-  a.pendDelta = (int8_t *)st->pendDelta;
-  a.entriesTop = st->entriesTop;
+  a.list0 = (Subtree *)st->list0;
+  a.list1 = (Subtree *)st->list1;
+  a.listD0 = (int8_t *)st->listD0;
+  a.listD1 = (int8_t *)st->listD1;
   a.subCount = st->subCount;
-  a.edgeCount = st->edgeCount;
+  a.regionCount = st->regionCount;
   a.passes = st->passes;
   a.digitBits = st->cfg.digitBits;
   return a;
@@ -945,8 +1138,8 @@ void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a,
   const uint32_t subPerTile = (uint32_t)st->cfg.tileKeys / kBuildSub;
   launch(ctx, s, "lbvh_build_local", k_build_sub, dim3(tiles * subPerTile), dim3(kBuildSub), 0, a, tileBase,
          subPerTile);
-  launch(ctx, s, "lbvh_build_region", k_build_region, dim3(tiles), dim3(kBuildTop), 0, a, tileBase);
-  launch(ctx, s, "lbvh_build_top", k_build_top, dim3(meshes), dim3(kBuildTop), 0, a, meshBase);
+  launch(ctx, s, "lbvh_build_region", k_build_region, dim3(tiles), dim3(kMergeThreads), 0, a, tileBase);
+  launch(ctx, s, "lbvh_build_top", k_build_top, dim3(meshes), dim3(kMergeThreads), 0, a, meshBase);
 }
 
 // Extra dynamic LDS that caps a sort pass at `wavesPerCU` resident waves per CU (160 KB of LDS
@@ -1059,24 +1252,21 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   off += al((size_t)std::max<uint32_t>(I, 1) * 4);
   const size_t oHdr = off;
   off += al((size_t)M * 16 * 4);
-  const size_t oEnt = off;
-  off += al((size_t)prims * 4);
-  const size_t oEntTop = off;
-  off += al((size_t)prims * 4);
-  const size_t oPend = off;
-  off += al((size_t)std::max<uint32_t>(I, 1) * 2 * sizeof(Pending));
-  const size_t oPendD = off;
-  off += al((size_t)std::max<uint32_t>(I, 1) * 2);
-  const size_t cntWords = std::max<uint32_t>(I, 1);
-  const size_t oCnt = off;  // zero after every build (each build returns them to zero)
-  off += al(cntWords * 4);
+  const size_t oList0 = off;
+  off += al((size_t)prims * sizeof(Subtree));
+  const size_t oList1 = off;
+  off += al((size_t)prims * sizeof(Subtree));
+  const size_t oListD0 = off;
+  off += al((size_t)prims);
+  const size_t oListD1 = off;
+  off += al((size_t)prims);
   const size_t oZero = off;
   const size_t histWords = (size_t)M * passes * R;
   const size_t statWords = (size_t)passes * T * R;
   const size_t tickWords = 64;
-  const size_t edgeWords = M;
-  const size_t regionWords = (size_t)T * ((size_t)cfg.tileKeys / kBuildSub);
-  const size_t zeroWords = histWords + statWords + tickWords + edgeWords + regionWords;
+  const size_t subWords = (size_t)T * ((size_t)cfg.tileKeys / kBuildSub);
+  const size_t regionWords = T;
+  const size_t zeroWords = histWords + statWords + tickWords + subWords + regionWords;
   off += al(zeroWords * 4);
   if (off > st->blockBytes) {
     free_block(st);
@@ -1102,19 +1292,17 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->parentLeaf = (uint32_t *)(b + oPL);
   st->parentInner = (uint32_t *)(b + oPI);
   st->headers = (float *)(b + oHdr);
-  st->entries = (uint32_t *)(b + oEnt);
-  st->pending = b + oPend;
-  st->pendDelta = b + oPendD;
-  st->entriesTop = (uint32_t *)(b + oEntTop);
+  st->list0 = b + oList0;
+  st->list1 = b + oList1;
+  st->listD0 = b + oListD0;
+  st->listD1 = b + oListD1;
   st->zeroBase = (uint32_t *)(b + oZero);
   st->zeroWords = zeroWords;
-  st->counters = (uint32_t *)(b + oCnt);
-  st->cntWords = cntWords;
   st->hist = st->zeroBase;
   st->status = st->hist + histWords;
   st->tickets = st->status + statWords;
-  st->edgeCount = st->tickets + tickWords;
-  st->subCount = st->edgeCount + edgeWords;
+  st->subCount = st->tickets + tickWords;
+  st->regionCount = st->subCount + subWords;
   st->desc = desc;
   st->tileMesh = tileMesh;
   return true;
@@ -1206,7 +1394,6 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
     }
     st->graphValid = false;
     if (!store_layout(st, desc, tileMesh, cfg, (uint32_t)prims)) return false;
-    fill_u32(ctx, s, st->counters, 0u, st->cntWords);
     if (!hip_ok(hipMemcpyAsync(st->dDesc, st->desc.data(), desc.size() * sizeof(MeshDesc), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync(mesh table)"))
       return false;
@@ -1262,7 +1449,6 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
 bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   BVHStore *st = acc->store;
   const MeshDesc &d = st->desc[acc->mesh];
-  fill_u32(ctx, s, st->edgeCount + acc->mesh, 0u, 1);
   const BuildArgs a = args_of(st);
   launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u);
   return true;
