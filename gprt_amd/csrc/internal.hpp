@@ -151,9 +151,13 @@ struct BVHStore {
   uint32_t *hist = nullptr;      // [meshes][passes][R]
   uint32_t *status = nullptr;    // [passes][tiles][R]
   uint32_t *tickets = nullptr;   // [passes]
-  // Host copies of the plan (to detect when a cached graph can be replayed).
+  // Host copies of the plan (to detect when a cached graph can be replayed): the mesh table, and
+  // tile -> mesh followed by the first tile of each 65536-key region and the meshes with more
+  // than one region (the device copy is dTileMesh).
   std::vector<MeshDesc> desc;
   std::vector<uint32_t> tileMesh;
+  uint32_t regions = 0, bigMeshes = 0;
+  std::vector<uint32_t> regionFirst;  // [meshes + 1]: each mesh's first region
   hipGraphExec_t graph = nullptr;
   bool graphValid = false;
 };

@@ -63,6 +63,8 @@ struct BuildArgs {
   int8_t *listD0, *listD1;
   uint32_t *subCount;
   uint32_t *regionCount;
+  const uint32_t *regionTile;  // first tile of every 65536-key region, mesh by mesh
+  const uint32_t *bigMesh;     // meshes of more than 65536 keys (the ones with a top level)
   int passes;
   int digitBits;
 };
@@ -729,15 +731,20 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
 //
 // A window holds at most kMergeCap subtrees. A longer sequence is merged window by window, the
 // outputs of each window replacing its inputs at the front of the same list, until one window
-// holds the rest: a window of more than 130 subtrees always merges some (adjacent boundary deltas
+// holds the rest (a region's sub-tiles leave about 12 subtrees each, so one window usually takes a
+// whole region): a window of more than 130 subtrees always merges some (adjacent boundary deltas
 // of subtrees differ, and a sequence of values in [-1, 63] without an interior local maximum has
 // at most 130 entries), and its outputs are at most 2 x 65 subtrees (one per tree level on each
 // side).
 // ------------------------------------------------------------------------------------------------
-constexpr int kMergeThreads = 1024;
-constexpr uint32_t kMergeCap = 2048;
+// Regions: windows of 1024 subtrees (512 threads, 75 KB of LDS: two workgroups per CU). The top
+// level runs one workgroup per large mesh and takes windows of 2048 (1024 threads, 150 KB), so a
+// mesh of up to ~128 regions merges in one window.
+constexpr uint32_t kRegionCap = 1024, kTopCap = 2048;
 
+template <uint32_t kMergeCap>
 struct MergeLds {
+  static constexpr int kMergeThreads = kMergeCap / 2;  // the output pass takes two subtrees per thread
   float leafBox[kMergeCap][6];   // input subtrees' boxes
   float nodeBox[kMergeCap][6];   // merged nodes' boxes, by boundary
   int ref[kMergeCap];            // input subtrees' references
@@ -755,9 +762,11 @@ struct MergeLds {
 
 // Merges input subtrees [0, S) (already in LDS: leafBox, ref, L[0..S], D[0..S]) and appends the
 // window's outputs at out[*outCount...] (outD: their deltas). Called by every thread.
-__device__ void merge_window(MergeLds &sm, uint32_t S, uint32_t n, BVHNode *__restrict__ nodes, float *__restrict__ header,
+template <uint32_t kMergeCap>
+__device__ void merge_window(MergeLds<kMergeCap> &sm, uint32_t S, uint32_t n, BVHNode *__restrict__ nodes, float *__restrict__ header,
                              Subtree *__restrict__ out, int8_t *__restrict__ outD, uint32_t outBase,
                              uint32_t &outCount) {
+  constexpr int kMergeThreads = MergeLds<kMergeCap>::kMergeThreads;
   const uint32_t tid = threadIdx.x;
   auto box_of = [&](uint32_t id) -> const float * { return id < kMergeCap ? sm.leafBox[id] : sm.nodeBox[id - kMergeCap]; };
   auto ref_of = [&](uint32_t id) -> int { return id < kMergeCap ? sm.ref[id] : (int)sm.node[id - kMergeCap]; };
@@ -827,7 +836,7 @@ __device__ void merge_window(MergeLds &sm, uint32_t S, uint32_t n, BVHNode *__re
     }
   }
   lds_barrier();
-  // Outputs in leaf order: thread t takes input subtrees [2t, 2t + 1] (kMergeCap = 2 x threads).
+  // Outputs in leaf order: thread t takes input subtrees [2t, 2t + 1].
   const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
   const bool o0 = i0 < S && sm.out[i0] != 0xFFFFu, o1 = i1 < S && sm.out[i1] != 0xFFFFu;
   const uint32_t mine = (uint32_t)o0 + (uint32_t)o1;
@@ -864,10 +873,10 @@ __device__ void merge_window(MergeLds &sm, uint32_t S, uint32_t n, BVHNode *__re
 
 // Loads input subtrees [w0, w0 + S) of a sequence into LDS; `at(e)` gives entry e's index in
 // `in` / `inD`. endL / endD describe what follows the sequence when the window reaches its end.
-template <class At>
-__device__ void load_window(MergeLds &sm, const Subtree *__restrict__ in, const int8_t *__restrict__ inD, At at,
+template <uint32_t kMergeCap, class At>
+__device__ void load_window(MergeLds<kMergeCap> &sm, const Subtree *__restrict__ in, const int8_t *__restrict__ inD, At at,
                             uint32_t w0, uint32_t S, uint32_t total, uint32_t endL, int endD) {
-  for (uint32_t i = threadIdx.x; i <= S; i += kMergeThreads) {
+  for (uint32_t i = threadIdx.x; i <= S; i += MergeLds<kMergeCap>::kMergeThreads) {
     const uint32_t e = w0 + i;
     if (e < total) {
       const size_t x = at(e);
@@ -894,8 +903,8 @@ __device__ void load_window(MergeLds &sm, const Subtree *__restrict__ in, const 
 // Merges a whole sequence of `total` input subtrees: window by window (outputs compacted into
 // `tmp` / `tmpD` from its start) until one window holds what is left, whose outputs go to `dst`.
 // `at` maps entry e of the first pass to its index in `in`. Returns the number of outputs.
-template <class At>
-__device__ uint32_t merge_sequence(MergeLds &sm, const Subtree *in, const int8_t *inD, At at, uint32_t total,
+template <uint32_t kMergeCap, class At>
+__device__ uint32_t merge_sequence(MergeLds<kMergeCap> &sm, const Subtree *in, const int8_t *inD, At at, uint32_t total,
                                    uint32_t endL, int endD, Subtree *tmp, int8_t *tmpD, Subtree *dst, int8_t *dstD,
                                    uint32_t n, BVHNode *nodes, float *header) {
   bool first = true;
@@ -919,15 +928,13 @@ __device__ uint32_t merge_sequence(MergeLds &sm, const Subtree *in, const int8_t
 
 // Level 2: one workgroup per 65536-key region merges the outputs of its sub-tiles (each sub-tile's
 // list at its own key positions in list0) into list1 at the region's key positions.
-__global__ void __launch_bounds__(kMergeThreads) k_build_region(BuildArgs a, uint32_t tileBase) {
-  __shared__ MergeLds sm;
+__global__ void __launch_bounds__(kRegionCap / 2) k_build_region(BuildArgs a, uint32_t regionBase) {
+  __shared__ MergeLds<kRegionCap> sm;
   __shared__ uint32_t s_pre[kBuildRegion / kBuildSub + 1];
-  const uint32_t t = tileBase + blockIdx.x;
+  const uint32_t t = a.regionTile[regionBase + blockIdx.x];  // the region's first tile
   const uint32_t m = a.tileMesh[t];
   const MeshDesc d = a.desc[m];
   const uint32_t lt = t - d.tileOffset;
-  const uint32_t perRegion = kBuildRegion / a.tileKeys;
-  if (lt % perRegion != 0) return;  // one workgroup per region: its first tile
   const uint32_t n = d.ntri;
   const uint32_t regionLo = lt * a.tileKeys;
   const uint32_t regionHi = min(regionLo + kBuildRegion, n);
@@ -970,9 +977,10 @@ __global__ void __launch_bounds__(kMergeThreads) k_build_region(BuildArgs a, uin
 
 // Level 3: one workgroup per mesh gathers its regions' outputs into list0 (from the mesh's start)
 // and merges them up to the root.
-__global__ void __launch_bounds__(kMergeThreads) k_build_top(BuildArgs a, uint32_t meshBase) {
-  __shared__ MergeLds sm;
-  const uint32_t m = meshBase + blockIdx.x;
+__global__ void __launch_bounds__(kTopCap / 2) k_build_top(BuildArgs a, uint32_t fromList, uint32_t meshBase) {
+  constexpr int kMergeThreads = kTopCap / 2;
+  __shared__ MergeLds<kTopCap> sm;
+  const uint32_t m = fromList ? a.bigMesh[blockIdx.x] : meshBase + blockIdx.x;
   const MeshDesc d = a.desc[m];
   const uint32_t n = d.ntri;
   if (n <= kBuildRegion) return;  // the region level reached the root
@@ -1128,18 +1136,30 @@ BuildArgs args_of(BVHStore *st) {
   a.listD1 = (int8_t *)st->listD1;
   a.subCount = st->subCount;
   a.regionCount = st->regionCount;
+  a.regionTile = st->dTileMesh + st->tiles;
+  a.bigMesh = a.regionTile + st->regions;
   a.passes = st->passes;
   a.digitBits = st->cfg.digitBits;
   return a;
 }
 
+// The hierarchy of meshes [meshBase, meshBase + meshes) (all of the store's, or one): one
+// workgroup per sub-tile, per region, and per mesh that has more than one region.
 void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a, uint32_t tileBase, uint32_t tiles,
                   uint32_t meshBase, uint32_t meshes) {
   const uint32_t subPerTile = (uint32_t)st->cfg.tileKeys / kBuildSub;
   launch(ctx, s, "lbvh_build_local", k_build_sub, dim3(tiles * subPerTile), dim3(kBuildSub), 0, a, tileBase,
          subPerTile);
-  launch(ctx, s, "lbvh_build_region", k_build_region, dim3(tiles), dim3(kMergeThreads), 0, a, tileBase);
-  launch(ctx, s, "lbvh_build_top", k_build_top, dim3(meshes), dim3(kMergeThreads), 0, a, meshBase);
+  const uint32_t r0 = st->regionFirst[meshBase], r1 = st->regionFirst[meshBase + meshes];
+  if (r1 > r0) launch(ctx, s, "lbvh_build_region", k_build_region, dim3(r1 - r0), dim3(kRegionCap / 2), 0, a, r0);
+  if (meshes == st->meshes) {
+    if (st->bigMeshes)
+      launch(ctx, s, "lbvh_build_top", k_build_top, dim3(st->bigMeshes), dim3(kTopCap / 2), 0, a, 1u, 0u);
+  } else {
+    for (uint32_t m = meshBase; m < meshBase + meshes; ++m)
+      if (st->desc[m].ntri > kBuildRegion)
+        launch(ctx, s, "lbvh_build_top", k_build_top, dim3(1), dim3(kTopCap / 2), 0, a, 0u, m);
+  }
 }
 
 // Extra dynamic LDS that caps a sort pass at `wavesPerCU` resident waves per CU (160 KB of LDS
@@ -1230,8 +1250,21 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   size_t off = 0;
   const size_t oDesc = off;
   off += al((size_t)M * sizeof(MeshDesc));
+  // tile -> mesh, then the region table (first tile of each region) and the meshes with a top
+  // level: one upload
+  std::vector<uint32_t> regionTile, bigMesh, regionFirst(M + 1, 0u);
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t regions = (desc[m].ntri + kBuildRegion - 1) / kBuildRegion;
+    regionFirst[m] = (uint32_t)regionTile.size();
+    for (uint32_t r = 0; r < regions; ++r) regionTile.push_back(desc[m].tileOffset + r * (kBuildRegion / cfg.tileKeys));
+    if (desc[m].ntri > kBuildRegion) bigMesh.push_back(m);
+  }
+  regionFirst[M] = (uint32_t)regionTile.size();
+  std::vector<uint32_t> plan(tileMesh);
+  plan.insert(plan.end(), regionTile.begin(), regionTile.end());
+  plan.insert(plan.end(), bigMesh.begin(), bigMesh.end());
   const size_t oTile = off;
-  off += al((size_t)T * 4);
+  off += al(plan.size() * 4);
   const size_t oCodes = off;
   off += al((size_t)prims * 4);
   const size_t oIds = off;
@@ -1304,7 +1337,10 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   st->subCount = st->tickets + tickWords;
   st->regionCount = st->subCount + subWords;
   st->desc = desc;
-  st->tileMesh = tileMesh;
+  st->tileMesh = plan;
+  st->regions = (uint32_t)regionTile.size();
+  st->bigMeshes = (uint32_t)bigMesh.size();
+  st->regionFirst = regionFirst;
   return true;
 }
 
@@ -1397,7 +1433,7 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
     if (!hip_ok(hipMemcpyAsync(st->dDesc, st->desc.data(), desc.size() * sizeof(MeshDesc), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync(mesh table)"))
       return false;
-    if (!hip_ok(hipMemcpyAsync(st->dTileMesh, st->tileMesh.data(), tileMesh.size() * 4, hipMemcpyHostToDevice, s),
+    if (!hip_ok(hipMemcpyAsync(st->dTileMesh, st->tileMesh.data(), st->tileMesh.size() * 4, hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync(tile table)"))
       return false;
   }
