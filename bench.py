@@ -56,11 +56,17 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 11937.0e6 + 69.8e6,
-    "mlp_infer_top1": 1325.4e6 + 16.8e6,  # one launch of 4,194,304 rays
-    "lbvh_sort_pass": 35.9e6 + 42.8e6,
-    "build": (48.0 + 12.0 + 12.0 + 4.7 + 4 * (8.5 + 10.2) + 257.0 + 113.6 + 3.7 + 1.7 + 0.1) * N_TRIS,
-    "batched_build_per_prim": 48.0 + 12.3 + 12.1 + 4.8 + 4 * (8.1 + 10.5) + 156.7 + 112.7 + 5.5 + 1.9 + 0.2,
+    "trace_closest": 11961.8e6 + 69.8e6,
+    "mlp_infer_top1": 1329.3e6 + 16.8e6,  # one launch of 4,194,304 rays
+    "lbvh_sort_pass": 31.7e6 + 37.4e6,
+    # per primitive, summed over the build kernels (init, bounds, keys, histogram scan, 4 sort
+    # passes, sub-tile, region, top)
+    "build_per_prim": 0.5 + 60.1 + 16.4 + 4 * (7.56 + 8.92) + 257.3 + 112.4 + 1.2 + 0.1,
+    "batched_build_per_prim": 0.7 + 60.1 + 16.5 + 0.1 + 4 * (7.44 + 9.16) + 157.7 + 112.1 + 1.4 + 0.04,
+    "boxes_build_per_prim": 0.5 + 36.1 + 16.4 + 4 * (7.58 + 8.97) + 210.3 + 112.2 + 1.2 + 0.04,
+    "refit_per_prim": 257.0 + 112.4 + 1.2 + 0.1,  # sub-tile, region, top
+    "trace_boxes": 14945.6e6 + 77.8e6,  # one launch of 4,194,304 rays
+    "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
 # commands and per-kernel table in profiles/cache_hit_rates_scene.md).
@@ -183,7 +189,7 @@ def scene_workload(args):
     roof_build = {"kernel": "lbvh build (all build kernels)", "bound": "hbm",
                   "achieved": N_TRIS * LBVH_BYTES_PER_PRIM / (build_ms * 1e-3) / 1e9,
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": LBVH_BYTES_PER_PRIM,
-                  "traffic": TRAFFIC_BYTES["build"]}
+                  "traffic": TRAFFIC_BYTES["build_per_prim"] * N_TRIS}
     roof_build["frac"] = roof_build["achieved"] / HBM_PEAK_GBS
     out_ms_tot, out_n = ms_of(per_kernel, "mlp_infer_top1")
     out_avg = out_ms_tot / max(out_n, 1)
@@ -465,6 +471,20 @@ def dynamic_workload(args):
     wall = time.perf_counter() - t0
     pct = lambda a, q: float(np.percentile(np.asarray(a), q))
     rays = 2 * n
+    # Node fetches per ray of the last frame's plain traces (untimed), for the traversal's
+    # algorithmic bytes: 32 B ray + 16 B hit + 64 B per inner node (top and instance levels);
+    # leaf records are not counted, so this is a lower bound.
+    vis = ctx.device_buffer(nbytes=4 * n)
+    api.trace_detail(ctx, tlas, pb, hp, n, visits_buf=vis)
+    v1 = float(vis.read(np.uint32).mean())
+    api.trace_detail(ctx, tlas, bb, hb, n, visits_buf=vis)
+    v2 = float(vis.read(np.uint32).mean())
+    vis.destroy()
+    visits = (v1 + v2) / 2
+    launch_ms = pct(tplain, 50) / 2  # two traversal launches per frame (primary, bounce)
+    roof = _roof_hbm("trace_closest_instances", n * (48 + 64 * visits), launch_ms, bytes_per_ray=48 + 64 * visits,
+                     nodes_per_ray=visits, note="issue- and L2-bound like the scene traversal; leaf records "
+                                               "not counted")
     out = {
         "metric": "dynamic scene frame time (64 x 65,536 tris rebuilt per frame + 2,097,152 rays), plain vs "
                   "learned proposal, 1 GPU",
@@ -487,6 +507,7 @@ def dynamic_workload(args):
             "learned_vs_plain_trace": pct(tplain, 50) / pct(tlearn, 50),
             "fine_tune_frame_wall_ms_mean": float(np.mean(tune_ms)) if tune_ms else None,
             "proposal_hit_rate_by_check": hit_rates, "hit_mismatches_learned_vs_plain": mism},
+        "roofline": roof,
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -589,6 +610,7 @@ def refit_workload(args):
     trace_rebuilt_ms = ms_of(pk2, "trace_closest")[0] / K
     roof = _roof_hbm("lbvh_build_local + region + top (refit over the kept topology)", N_TRIS * REFIT_BYTES_PER_PRIM,
                      refit_ms, bytes_per_prim=REFIT_BYTES_PER_PRIM)
+    roof["traffic"] = TRAFFIC_BYTES["refit_per_prim"] * N_TRIS
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc
@@ -676,6 +698,7 @@ def boxes_workload(args):
     hits = hb.read(api.HIT_DTYPE)
     roof = _roof_hbm("LBVH build over boxes (all build kernels)", N_BOXES * BOX_LBVH_BYTES_PER_PRIM, build_ms,
                      bytes_per_prim=BOX_LBVH_BYTES_PER_PRIM)
+    roof["traffic"] = TRAFFIC_BYTES["boxes_build_per_prim"] * N_BOXES
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc
@@ -699,6 +722,8 @@ def boxes_workload(args):
         "config": {"workload": "4,194,304 boxes + 4,194,304 random rays; step = LBVH build + closest-hit trace",
                    "boxes": N_BOXES, "rays": len(rays)},
         "components": {"build_ms": build_ms, "trace_ms": trace_ms, "trace_mrays_s": len(rays) / (trace_ms * 1e3),
+                       "trace_hbm_bytes_per_launch": TRAFFIC_BYTES["trace_boxes"],
+                       "trace_hbm_gbs": TRAFFIC_BYTES["trace_boxes"] / (trace_ms * 1e-3) / 1e9,
                        "node_visits_per_ray": visits, "hit_rate": float((hits["primID"] >= 0).mean())},
         "roofline": roof, "cpu_baseline": cpu, "kernels": pk,
     }
@@ -761,6 +786,7 @@ def compute_workload(args):
     ctx.sync()
     roof = _roof_hbm("ComputeBounds (user program via gprtComputeLaunch)", n * BOUNDS_BYTES_PER_PRIM, ms,
                      bytes_per_prim=BOUNDS_BYTES_PER_PRIM)
+    roof["traffic"] = TRAFFIC_BYTES["compute_bounds"]
     cpu = None
     if not args.no_cpu:
         m = 1 << 22
