@@ -135,6 +135,12 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+// Header word set while every triangle of the mesh indexes its own three vertices in order
+// (triangle i: vertices 3i, 3i+1, 3i+2, as triangle soups and the blob meshes are laid out): the
+// bounds pass checks it on the way, and the hierarchy pass of the same build then reads the
+// vertices without gathering the index triple first.
+constexpr uint32_t kIdentityWord = 15;
+
 // ------------------------------------------------------------------------------------------------
 // Init: zero histograms / look-back state / edge counts; box accumulators to min/max identities.
 // ------------------------------------------------------------------------------------------------
@@ -144,7 +150,8 @@ __global__ void __launch_bounds__(256) k_init(uint32_t *__restrict__ zero, size_
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t j = i; j < (size_t)meshes * 16; j += stride) {
     const uint32_t q = (uint32_t)(j & 15);
-    ((uint32_t *)headers)[j] = q < 3 ? 0xFFFFFFFFu : 0u;
+    // box accumulators; word 15: "every triangle i indexes vertices 3i, 3i+1, 3i+2" until refuted
+    ((uint32_t *)headers)[j] = q < 3 ? 0xFFFFFFFFu : (q == kIdentityWord ? 1u : 0u);
   }
   for (; i < words; i += stride) zero[i] = 0u;
 }
@@ -172,9 +179,19 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
   const TileRange r = tile_range(a, d, t, m);
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool ident = true;
   for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kBoundsThreads) {
     float3 p, q, s;
-    load_prim(d, i, p, q, s);
+    if (d.boxes) {
+      load_prim(d, i, p, q, s);
+    } else {
+      const auto *ix = global_ptr<uint32_t>(d.idx + (size_t)i * d.istride);
+      const uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
+      ident = ident && i0 == 3 * i && i1 == 3 * i + 1 && i2 == 3 * i + 2;
+      p = load_vertex(d, i0);
+      q = load_vertex(d, i1);
+      s = load_vertex(d, i2);
+    }
     const float3 c = prim_centroid(d, p, q, s);
     float *cd = a.centroids + 3 * ((size_t)d.primOffset + i);
     cd[0] = c.x;
@@ -200,7 +217,9 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
       sb[w][3 + c] = mx[c];
     }
   }
-  __syncthreads();
+  const int allIdent = __syncthreads_and(ident ? 1 : 0);
+  if (threadIdx.x == 6 && !allIdent && !d.boxes)
+    atomicAnd((uint32_t *)a.headers + (size_t)m * 16 + kIdentityWord, 0u);
   if (threadIdx.x < 6) {
     const int c = threadIdx.x;
     float v = sb[0][c];
@@ -283,15 +302,19 @@ __global__ void __launch_bounds__(256) k_hist_scan(BuildArgs a) {
 // ------------------------------------------------------------------------------------------------
 // Segmented one-sweep pass over (u32 Morton code, u32 primID).
 // ------------------------------------------------------------------------------------------------
+// 1024-thread tiles ask for at most 64 VGPRs so two workgroups (8 waves per SIMD) stay resident.
 template <int BITS, int THREADS, int ITEMS>
-__global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
+__global__ void __launch_bounds__(THREADS, THREADS >= 1024 ? 8 : 1) k_onesweep(BuildArgs a, int pass) {
   constexpr int R = 1 << BITS;
   constexpr int W = THREADS / 64;
   constexpr int TILE = THREADS * ITEMS;
   constexpr int DPT = (R + THREADS - 1) / THREADS;
-  __shared__ __attribute__((aligned(16))) uint32_t s_buf[TILE];   // keys in tile order
-  __shared__ __attribute__((aligned(16))) uint32_t s_vbuf[TILE];  // their payloads
-  __shared__ uint32_t s_whist[W][R];
+  // Keys and payloads in tile order; until the reorder the front of s_buf holds the per-wave
+  // digit counts instead (W x R words), so the tile needs no more LDS than its two buffers.
+  __shared__ __attribute__((aligned(16))) uint32_t s_buf[TILE];
+  __shared__ __attribute__((aligned(16))) uint32_t s_vbuf[TILE];
+  static_assert(W * R <= TILE, "per-wave digit counts must fit in the key buffer");
+  uint32_t(*s_whist)[R] = reinterpret_cast<uint32_t(*)[R]>(s_buf);
   __shared__ uint32_t s_tdigit[R];
   __shared__ uint32_t s_gbase[R];
   __shared__ uint32_t s_wtot[W];
@@ -394,18 +417,30 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
     }
     uint32_t excl = 0;
     if (lt > 0) {
+      // Look back four predecessors per step (four loads in flight): sum aggregates up to the
+      // first inclusive value; an unpublished tile is waited for and the walk resumes there.
       int64_t tt = (int64_t)t - 1;
       const int64_t first = d.tileOffset;
       while (tt >= first) {
-        const uint32_t s = ld_agent(&a.status[((size_t)pass * a.tiles + tt) * R + dd]);
-        const uint32_t flag = s & ~kValueMask;
-        if (flag == 0u) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
+        const int nb = (int)min((int64_t)4, tt - first + 1);
+        uint32_t sv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i < nb) sv[i] = ld_agent(&a.status[((size_t)pass * a.tiles + (tt - i)) * R + dd]);
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (done || i >= nb || used < i) continue;
+          const uint32_t flag = sv[i] & ~kValueMask;
+          if (flag == 0u) continue;  // not published: stop consuming here
+          excl += sv[i] & kValueMask;
+          used = i + 1;
+          done = flag == kFlagInc;
         }
-        excl += s & kValueMask;
-        if (flag == kFlagInc) break;
-        --tt;
+        if (done) break;
+        tt -= used;
+        if (used < nb) __builtin_amdgcn_s_sleep(1);
       }
       st_agent(&a.status[((size_t)pass * a.tiles + t) * R + dd], kFlagInc | (excl + cnt[q]));
     }
@@ -419,6 +454,10 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
   for (int j = 0; j < ITEMS; ++j) {
     const uint32_t dj = digit(k[j]);
     pos[j] = s_tdigit[dj] + s_whist[w][dj] + pos[j];
+  }
+  __syncthreads();  // the counts are read; the buffer takes the keys
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
     s_buf[pos[j]] = k[j];
     s_vbuf[pos[j]] = v[j];
   }
@@ -524,7 +563,10 @@ __device__ __forceinline__ void put_entry(Subtree *p, const float b[6], int ref,
   p->b = make_float4(b[3], b[4], b[5], __uint_as_float(first));
 }
 
-__global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile) {
+// identOk: this launch follows the same build's bounds pass, so the mesh's identity-index word is
+// current (a refit leaves it as the last build set it and gathers the indices).
+__global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile,
+                                                         uint32_t identOk) {
   __shared__ int8_t s_delta[kBuildSub + 4];        // delta(lo - 1 + q), in [-1, 63]
   __shared__ uint16_t s_split[kBuildSub];          // split of finished local node lo + q (0xFFFF: none)
   __shared__ float s_leaf[kBuildSub][6];           // leaf boxes (position - lo)
@@ -573,9 +615,16 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   s_split[tid] = 0xFFFFu;
   s_out[tid] = kNoOutput;
   float box[6];
+  const bool ident = identOk && !d.boxes && ((const uint32_t *)header)[kIdentityWord] != 0u;
   if (k < hi) {
     float3 p, q, s;
-    load_prim(d, prim, p, q, s);
+    if (ident) {
+      p = load_vertex(d, 3 * prim);
+      q = load_vertex(d, 3 * prim + 1);
+      s = load_vertex(d, 3 * prim + 2);
+    } else {
+      load_prim(d, prim, p, q, s);
+    }
     s_delta[tid + 1] = k + 1 < n ? __clzll((((uint64_t)ck << 32) | prim) ^ (((uint64_t)cn << 32) | in)) : -1;
     if (tid == 0) s_delta[0] = lo > 0 ? __clzll((((uint64_t)cp << 32) | ip) ^ (((uint64_t)ck << 32) | prim)) : -1;
     box[0] = fmin_ord(p.x, fmin_ord(q.x, s.x));
@@ -1146,10 +1195,10 @@ BuildArgs args_of(BVHStore *st) {
 // The hierarchy of meshes [meshBase, meshBase + meshes) (all of the store's, or one): one
 // workgroup per sub-tile, per region, and per mesh that has more than one region.
 void launch_refit(Context *ctx, hipStream_t s, BVHStore *st, const BuildArgs &a, uint32_t tileBase, uint32_t tiles,
-                  uint32_t meshBase, uint32_t meshes) {
+                  uint32_t meshBase, uint32_t meshes, bool afterBounds) {
   const uint32_t subPerTile = (uint32_t)st->cfg.tileKeys / kBuildSub;
   launch(ctx, s, "lbvh_build_local", k_build_sub, dim3(tiles * subPerTile), dim3(kBuildSub), 0, a, tileBase,
-         subPerTile);
+         subPerTile, afterBounds ? 1u : 0u);
   const uint32_t r0 = st->regionFirst[meshBase], r1 = st->regionFirst[meshBase + meshes];
   if (r1 > r0) launch(ctx, s, "lbvh_build_region", k_build_region, dim3(r1 - r0), dim3(kRegionCap / 2), 0, a, r0);
   if (meshes == st->meshes) {
@@ -1168,7 +1217,7 @@ template <int BITS, int THREADS, int ITEMS>
 size_t sort_lds_pad(int wavesPerCU) {
   if (wavesPerCU <= 0) return 0;
   constexpr int R = 1 << BITS, W = THREADS / 64;
-  const size_t natural = (size_t)(2 * THREADS * ITEMS + W * R + 2 * R + W + 1) * 4;
+  const size_t natural = (size_t)(2 * THREADS * ITEMS + 2 * R + W + 1) * 4;
   const int wgs = std::max(1, wavesPerCU / W);
   const size_t target = std::min<size_t>((160u * 1024u) / wgs, 64u * 1024u);
   return target > natural ? target - natural : 0;
@@ -1185,8 +1234,8 @@ void launch_sort_passes(Context *ctx, hipStream_t s, BVHStore *st, const BuildAr
       launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 16>, dim3(st->tiles), dim3(256),
              sort_lds_pad<BITS, 256, 16>(wpc), a, p);
     else
-      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 512, 16>, dim3(st->tiles), dim3(512),
-             sort_lds_pad<BITS, 512, 16>(wpc), a, p);
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 1024, 8>, dim3(st->tiles), dim3(1024),
+             sort_lds_pad<BITS, 1024, 8>(wpc), a, p);
   }
 }
 
@@ -1218,7 +1267,7 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
   }
   if (st->passes & 1)
     launch(ctx, s, "lbvh_copy_back", k_copy_back, dim3(grid_for(st->prims, 256, 8192)), dim3(256), 0, a, st->prims);
-  launch_refit(ctx, s, st, a, 0u, st->tiles, 0u, st->meshes);
+  launch_refit(ctx, s, st, a, 0u, st->tiles, 0u, st->meshes, true);
 }
 
 bool same_plan(const BVHStore *st, const std::vector<MeshDesc> &desc, const SortConfig &cfg) {
@@ -1486,7 +1535,7 @@ bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   BVHStore *st = acc->store;
   const MeshDesc &d = st->desc[acc->mesh];
   const BuildArgs a = args_of(st);
-  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u);
+  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u, false);
   return true;
 }
 

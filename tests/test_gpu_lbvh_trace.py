@@ -63,6 +63,37 @@ def test_lbvh_shared_vertices_and_flat_axis(ctx, orc):
     _check_lbvh(ctx, orc, verts, idx).destroy()
 
 
+def _permuted(verts, idx, seed):
+    """The same triangles with the vertex array shuffled and the indices remapped."""
+    perm = np.random.default_rng(seed).permutation(len(verts))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    return verts[perm], inv[idx].astype(np.uint32)
+
+
+def test_lbvh_non_identity_indices(ctx, orc):
+    # Triangle i indexing vertices 3i..3i+2 takes a path without the index gather; shuffled
+    # vertices, and a mesh that breaks the pattern in its last triangle only, take the general one.
+    verts, idx = scenes.blob_scene(200_000, seed=5)
+    pv, pi = _permuted(verts, idx, 6)
+    _check_lbvh(ctx, orc, pv, pi).destroy()
+    last = idx.copy()
+    last[-1] = last[-1][::-1]
+    _check_lbvh(ctx, orc, verts, last).destroy()
+    accs = [api.TriangleAccel(ctx, verts, idx), api.TriangleAccel(ctx, pv, pi), api.TriangleAccel(ctx, verts, last)]
+    api.build_batch(ctx, accs)
+    assert api.last_error() is None, api.last_error()
+    rk, rn, ra = orc.lbvh_build(verts, idx)
+    for acc, (v, ix) in zip(accs, [(verts, idx), (pv, pi), (verts, last)]):
+        keys, nodes, aabbs = acc.export()
+        if ix is not idx:
+            rk, rn, ra = orc.lbvh_build(v, ix)
+        np.testing.assert_array_equal(keys, rk)
+        np.testing.assert_array_equal(nodes, rn)
+        np.testing.assert_array_equal(aabbs.view(np.uint32), ra.view(np.uint32))
+        acc.destroy()
+
+
 def test_lbvh_update_refit(ctx, orc):
     # 200,000 triangles: the refit also runs the region and top hierarchy levels
     verts, idx = scenes.blob_scene(200_000, seed=6)
