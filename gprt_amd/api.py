@@ -120,6 +120,8 @@ _SIGS = {
     "gprtProposalParamCount": (C.c_size_t, []),
     "gprtProposalSetParams": (None, [_P, _P, _P, C.c_size_t]),
     "gprtProposalGetParams": (None, [_P, _P, _P, C.c_size_t]),
+    "gprtProposalSerialize": (C.c_size_t, [_P, _P, _P, C.c_size_t, C.c_bool]),
+    "gprtProposalDeserialize": (C.c_bool, [_P, _P, _P, C.c_size_t]),
     "gprtProposalTrain": (C.c_float, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     "gprtProposalInfer": (None, [_P, _P, _P, C.c_uint32, _P]),
     "gprtProposalLogits": (None, [_P, _P, _P, C.c_uint32, _P]),
@@ -446,6 +448,18 @@ class Proposal:
         loss = self.lib.gprtProposalTrainLabels(self.ctx.h, self.h, rays_buf.h, labels_buf.h, count, epochs, lr,
                                                 C.byref(acc))
         return float(loss), float(acc.value)
+
+    def serialize(self, with_optimizer_state: bool = True) -> bytes:
+        """gprtProposalSerialize: the versioned weight checkpoint."""
+        n = int(self.lib.gprtProposalSerialize(self.ctx.h, self.h, None, 0, with_optimizer_state))
+        buf = C.create_string_buffer(n)
+        got = int(self.lib.gprtProposalSerialize(self.ctx.h, self.h, buf, n, with_optimizer_state))
+        if got != n:
+            raise RuntimeError("gprtProposalSerialize failed: %s" % last_error())
+        return buf.raw
+
+    def deserialize(self, blob: bytes) -> bool:
+        return bool(self.lib.gprtProposalDeserialize(self.ctx.h, self.h, blob, len(blob)))
 
     def destroy(self):
         if self.h:

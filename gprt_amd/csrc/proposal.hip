@@ -179,6 +179,134 @@ GPRT_API void gprtProposalGetParams(GPRTContext _ctx, GPRTProposal _p, float *ho
 
 GPRT_API size_t gprtProposalParamCount(void) { return kMlpParams; }
 
+// ------------------------------------------------------------------------------------------------
+// Weight checkpoints: a 64-byte header, the fp32 parameters and optionally the Adam moments.
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+struct CheckpointHeader {
+  char magic[8];
+  uint32_t version, headerBytes;
+  uint64_t params;
+  uint32_t in, hidden, layers, out;
+  uint32_t step, flags, crc;
+  uint32_t reserved[3];
+};
+static_assert(sizeof(CheckpointHeader) == 64, "checkpoint header is 64 bytes");
+constexpr char kCheckpointMagic[8] = {'G', 'P', 'R', 'T', 'M', 'L', 'P', '\0'};
+constexpr uint32_t kCheckpointVersion = 1;
+
+// CRC-32 with the reflected IEEE 802.3 polynomial 0xEDB88320, initial value and final xor ~0:
+// the checksum of zlib's crc32 and of PNG / gzip (table-driven, one byte at a time).
+struct Crc32Table {
+  uint32_t t[256];
+  Crc32Table() {
+    for (uint32_t i = 0; i < 256; ++i) {
+      uint32_t c = i;
+      for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+      t[i] = c;
+    }
+  }
+};
+
+uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n) {
+  static const Crc32Table tab;  // initialised once, thread-safe
+  const uint32_t *table = tab.t;
+  crc = ~crc;
+  for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+  return ~crc;
+}
+
+}  // namespace
+
+GPRT_API size_t gprtProposalSerialize(GPRTContext _ctx, GPRTProposal _p, void *out, size_t capacity,
+                                      bool withOptimizerState) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  if (!ctx || !p) {
+    fail("gprtProposalSerialize: null argument");
+    return 0;
+  }
+  const size_t arrays = withOptimizerState ? 3 : 1;
+  const size_t bytes = sizeof(CheckpointHeader) + arrays * kMlpParams * 4;
+  if (!out || capacity < bytes) return bytes;
+  CheckpointHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kCheckpointMagic, 8);
+  h.version = kCheckpointVersion;
+  h.headerBytes = sizeof(CheckpointHeader);
+  h.params = kMlpParams;
+  h.in = kMlpIn;
+  h.hidden = kMlpHid;
+  h.layers = 3;
+  h.out = kMlpOut;
+  h.step = p->step;
+  h.flags = withOptimizerState ? 1u : 0u;
+  uint8_t *payload = (uint8_t *)out + sizeof(CheckpointHeader);
+  const float *src[3] = {p->w, p->m, p->v};
+  for (size_t k = 0; k < arrays; ++k)
+    if (!hip_ok(hipMemcpyAsync(payload + k * kMlpParams * 4, src[k], kMlpParams * 4, hipMemcpyDeviceToHost,
+                               ctx->stream),
+                "hipMemcpyAsync(checkpoint)"))
+      return 0;
+  if (!hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(checkpoint)")) return 0;
+  h.crc = crc32_update(0u, payload, arrays * kMlpParams * 4);
+  std::memcpy(out, &h, sizeof(h));
+  return bytes;
+}
+
+GPRT_API bool gprtProposalDeserialize(GPRTContext _ctx, GPRTProposal _p, const void *in, size_t bytes) {
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  if (!ctx || !p || !in) {
+    fail("gprtProposalDeserialize: null argument");
+    return false;
+  }
+  if (bytes < sizeof(CheckpointHeader)) {
+    fail("gprtProposalDeserialize: %zu bytes is shorter than the 64-byte header", bytes);
+    return false;
+  }
+  CheckpointHeader h;
+  std::memcpy(&h, in, sizeof(h));
+  if (std::memcmp(h.magic, kCheckpointMagic, 8) != 0) {
+    fail("gprtProposalDeserialize: not a proposal checkpoint (bad magic)");
+    return false;
+  }
+  if (h.version != kCheckpointVersion || h.headerBytes != sizeof(CheckpointHeader)) {
+    fail("gprtProposalDeserialize: unsupported checkpoint version %u (header %u bytes)", h.version, h.headerBytes);
+    return false;
+  }
+  if (h.params != kMlpParams || h.in != (uint32_t)kMlpIn || h.hidden != (uint32_t)kMlpHid || h.layers != 3 ||
+      h.out != (uint32_t)kMlpOut || (h.flags & ~1u) != 0u) {
+    fail("gprtProposalDeserialize: network shape %llu params (%u-%u x%u-%u), flags %u does not match this build's "
+         "%zu params (%d-%d x3-%d)",
+         (unsigned long long)h.params, h.in, h.hidden, h.layers, h.out, h.flags, kMlpParams, kMlpIn, kMlpHid, kMlpOut);
+    return false;
+  }
+  const size_t arrays = (h.flags & 1u) ? 3 : 1;
+  const size_t payloadBytes = arrays * kMlpParams * 4;
+  if (bytes != sizeof(CheckpointHeader) + payloadBytes) {
+    fail("gprtProposalDeserialize: %zu bytes, expected %zu", bytes, sizeof(CheckpointHeader) + payloadBytes);
+    return false;
+  }
+  const uint8_t *payload = (const uint8_t *)in + sizeof(CheckpointHeader);
+  if (crc32_update(0u, payload, payloadBytes) != h.crc) {
+    fail("gprtProposalDeserialize: checksum mismatch (the checkpoint is damaged)");
+    return false;
+  }
+  if (!mlp_set_params(ctx, ctx->stream, p, (const float *)payload)) return false;
+  if (arrays == 3) {
+    if (!hip_ok(hipMemcpyAsync(p->m, payload + kMlpParams * 4, kMlpParams * 4, hipMemcpyHostToDevice, ctx->stream),
+                "hipMemcpyAsync(checkpoint m)") ||
+        !hip_ok(hipMemcpyAsync(p->v, payload + 2 * kMlpParams * 4, kMlpParams * 4, hipMemcpyHostToDevice,
+                               ctx->stream),
+                "hipMemcpyAsync(checkpoint v)"))
+      return false;
+    p->step = h.step;
+  }
+  return hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(checkpoint)");
+}
+
 GPRT_API void gprtProposalLogits(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, uint32_t count,
                                  GPRTBuffer _logits) {
   Context *ctx = (Context *)_ctx;

@@ -327,7 +327,7 @@ GPRT_API void gprtAccelTraceClosestHit(GPRTContext context, GPRTAccel accel, GPR
  * It is trained on the device (hand-written forward / backward / Adam) with labels taken from
  * the plain traversal's hits. gprtAccelTraceClosestHitEx with a proposal first traverses the
  * proposed treelet to get a hit that tightens tmax, then runs the exact traversal from the root:
- * hits are identical to the plain trace (the plain traversal is the fallback and the judge).
+ * hits are identical to the plain trace (the exact traversal always decides the result).
  * Parameters use the flat fp32 layout W1t b1 W2t b2 W3t b3 W4t b4 with Wt[out][in] and the
  * first layer's input padded to 128 (gprtProposalParamCount() values). */
 typedef struct GPRTProposal_T *GPRTProposal;
@@ -341,6 +341,19 @@ GPRT_API uint32_t gprtProposalBindAccel(GPRTContext context, GPRTProposal propos
 GPRT_API size_t gprtProposalParamCount(void);
 GPRT_API void gprtProposalSetParams(GPRTContext context, GPRTProposal proposal, const float *params, size_t count);
 GPRT_API void gprtProposalGetParams(GPRTContext context, GPRTProposal proposal, float *params, size_t count);
+/* Weight checkpoints (the on-disk form of a trained proposal; no counterpart in the reference,
+ * which has no serialisation). Little-endian, version 1: a 64-byte header {char magic[8] =
+ * "GPRTMLP\0", u32 version = 1, u32 headerBytes = 64, u64 parameter count, u32 input, hidden,
+ * hidden layers, output widths (102, 128, 3, 4096), u32 Adam step, u32 flags (bit 0: Adam
+ * moments follow), u32 CRC-32 (IEEE 802.3, as zlib's crc32) of the payload, 12 reserved zero
+ * bytes}, then the fp32 parameters in the layout above and, with bit 0, the first and second
+ * Adam moments. Serialize returns the checkpoint's size and writes it when `capacity` suffices
+ * (query with out = NULL). Deserialize checks every header field and the CRC and returns false
+ * (through the error path) on any mismatch; it restores the optimizer state when present, else
+ * resets it as gprtProposalSetParams does. Treelet bindings are not part of a checkpoint. */
+GPRT_API size_t gprtProposalSerialize(GPRTContext context, GPRTProposal proposal, void *out, size_t capacity,
+                                      bool withOptimizerState);
+GPRT_API bool gprtProposalDeserialize(GPRTContext context, GPRTProposal proposal, const void *in, size_t bytes);
 /* Epochs of minibatch Adam over the rays the plain traversal hit (hits: GPRTHit per ray, as
  * written by gprtAccelTraceClosestHit). Returns the last epoch's mean loss; *accuracy = top-1. */
 GPRT_API float gprtProposalTrain(GPRTContext context, GPRTProposal proposal, GPRTBuffer rays, GPRTBuffer hits,

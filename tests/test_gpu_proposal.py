@@ -216,3 +216,67 @@ def test_adam_step_matches_closed_form(ctx, scene):
     for b in (rb, lb, rb2, hb2, pcb, crb):
         b.destroy()
     prop.destroy()
+
+
+def test_checkpoint_round_trip(ctx, scene):
+    """Weight checkpoints: header fields, the CRC against zlib's crc32, parameters and Adam state
+    restored exactly (a re-serialised checkpoint is byte-identical), inference unchanged, and
+    damaged or foreign data rejected."""
+    import zlib
+    verts, idx, acc = scene
+    a = api.Proposal(ctx, max_batch=4096, seed=31)
+    a.bind(acc, 12)
+    rays = scenes.jittered_pinhole_rays(3000, seed=32)
+    hits, _ = _trace(ctx, acc, rays)
+    rb, hb = ctx.device_buffer(rays), ctx.device_buffer(hits)
+    a.train(rb, hb, len(rays), epochs=2, lr=1e-3)
+    blob = a.serialize(True)
+    hdr = np.frombuffer(blob[:64], dtype=np.dtype([("magic", "S8"), ("version", "<u4"), ("hb", "<u4"),
+                                                   ("params", "<u8"), ("dims", "<u4", 4), ("step", "<u4"),
+                                                   ("flags", "<u4"), ("crc", "<u4"), ("res", "<u4", 3)]))[0]
+    assert hdr["magic"] == b"GPRTMLP" and hdr["version"] == 1 and hdr["hb"] == 64
+    assert hdr["params"] == a.n_params and list(hdr["dims"]) == [102, 128, 3, 4096]
+    assert hdr["step"] > 0 and hdr["flags"] == 1 and not hdr["res"].any()
+    assert len(blob) == 64 + 3 * 4 * a.n_params
+    assert hdr["crc"] == zlib.crc32(blob[64:])
+    np.testing.assert_array_equal(np.frombuffer(blob[64:64 + 4 * a.n_params], np.float32), a.get_params())
+
+    b = api.Proposal(ctx, max_batch=4096, seed=33)
+    b.bind(acc, 12)
+    assert b.deserialize(blob), api.last_error()
+    assert b.serialize(True) == blob  # parameters, both moments and the step
+    np.testing.assert_array_equal(b.get_params(), a.get_params())
+    n = 2048
+    cls_a, cls_b = ctx.device_buffer(nbytes=4 * n), ctx.device_buffer(nbytes=4 * n)
+    ctx.lib.gprtProposalInfer(ctx.h, a.h, rb.h, n, cls_a.h)
+    ctx.lib.gprtProposalInfer(ctx.h, b.h, rb.h, n, cls_b.h)
+    np.testing.assert_array_equal(cls_a.read(np.int32), cls_b.read(np.int32))
+
+    plain = a.serialize(False)  # parameters only: the optimizer state starts over
+    assert len(plain) == 64 + 4 * a.n_params
+    assert b.deserialize(plain), api.last_error()
+    again = b.serialize(True)
+    assert np.frombuffer(again[40:44], "<u4")[0] == 0  # step reset
+    assert not np.frombuffer(again[64 + 4 * a.n_params:], np.float32).any()  # moments zero
+    np.testing.assert_array_equal(b.get_params(), a.get_params())
+
+    ctx.lib.gprtClearLastError()
+    bad = bytearray(blob)
+    bad[64 + 1000] ^= 0x40
+    assert not b.deserialize(bytes(bad)) and "checksum" in api.last_error()
+    ctx.lib.gprtClearLastError()
+    assert not b.deserialize(b"NOTAMLP!" + blob[8:]) and "magic" in api.last_error()
+    ctx.lib.gprtClearLastError()
+    assert not b.deserialize(blob[:-4]) and "expected" in api.last_error()
+    ctx.lib.gprtClearLastError()
+    wrong = bytearray(blob)
+    wrong[8] = 2  # version
+    assert not b.deserialize(bytes(wrong)) and "version" in api.last_error()
+    ctx.lib.gprtClearLastError()
+    assert not b.deserialize(blob[:10]) and "header" in api.last_error()
+    ctx.lib.gprtClearLastError()
+    np.testing.assert_array_equal(b.get_params(), a.get_params())  # rejected data changed nothing
+    for buf in (rb, hb, cls_a, cls_b):
+        buf.destroy()
+    a.destroy()
+    b.destroy()
