@@ -120,6 +120,8 @@ _SIGS = {
     "gprtProposalParamCount": (C.c_size_t, []),
     "gprtProposalSetParams": (None, [_P, _P, _P, C.c_size_t]),
     "gprtProposalGetParams": (None, [_P, _P, _P, C.c_size_t]),
+    "gprtAccelSerialize": (C.c_size_t, [_P, _P, _P, C.c_size_t]),
+    "gprtAccelDeserialize": (C.c_bool, [_P, _P, _P, C.c_size_t]),
     "gprtProposalSerialize": (C.c_size_t, [_P, _P, _P, C.c_size_t, C.c_bool]),
     "gprtProposalDeserialize": (C.c_bool, [_P, _P, _P, C.c_size_t]),
     "gprtProposalTrain": (C.c_float, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
@@ -304,6 +306,17 @@ class TriangleAccel:
 
     def trace(self, rays_buf: Buffer, hits_buf: Buffer, count: int):
         self.lib.gprtAccelTraceClosestHit(self.ctx.h, self.h, rays_buf.h, hits_buf.h, count)
+
+    def serialize(self) -> bytes:
+        """gprtAccelSerialize: the built accel as bytes."""
+        n = int(self.lib.gprtAccelSerialize(self.ctx.h, self.h, None, 0))
+        buf = C.create_string_buffer(max(n, 1))
+        if int(self.lib.gprtAccelSerialize(self.ctx.h, self.h, buf, n)) != n or n == 0:
+            raise RuntimeError("gprtAccelSerialize failed: %s" % last_error())
+        return buf.raw[:n]
+
+    def deserialize(self, blob: bytes) -> bool:
+        return bool(self.lib.gprtAccelDeserialize(self.ctx.h, self.h, blob, len(blob)))
 
     def destroy(self):
         if self.h:

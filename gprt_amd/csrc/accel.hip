@@ -226,6 +226,155 @@ GPRT_API void gprtAccelBuild(GPRTContext _ctx, GPRTAccel _a, GPRTBuildParams par
   lbvh_build_batch(ctx, ctx->stream, &a, 1, false);
 }
 
+// ------------------------------------------------------------------------------------------------
+// Serialised accels: a 64-byte header, then the built arrays of one triangle or AABB accel (inner
+// nodes, leaf records, sorted keys, mesh header) as the library lays them out.
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+struct AccelFileHeader {
+  char magic[8];
+  uint32_t version, headerBytes;
+  uint32_t kind;  // 0 triangles, 1 AABBs
+  uint32_t prims;
+  uint32_t nodeBytes, recordBytes;
+  uint32_t buildMode;
+  uint32_t crc;
+  uint32_t reserved[6];
+};
+static_assert(sizeof(AccelFileHeader) == 64, "serialised accel header is 64 bytes");
+constexpr char kAccelMagic[8] = {'G', 'P', 'R', 'T', 'B', 'V', 'H', '\0'};
+constexpr uint32_t kAccelVersion = 1;
+
+// Payload pieces of an accel of n primitives, in file order: nodes, records, codes, ids, header.
+struct AccelPieces {
+  size_t bytes[5];
+  size_t total() const { return bytes[0] + bytes[1] + bytes[2] + bytes[3] + bytes[4]; }
+};
+AccelPieces accel_pieces(uint32_t n) {
+  return {{(size_t)(n - 1) * sizeof(BVHNode), (size_t)n * sizeof(TriRecord), (size_t)n * 4, (size_t)n * 4, 16 * 4}};
+}
+
+void *accel_piece(Accel *a, int k) {
+  const MeshDesc &d = a->store->desc[a->mesh];
+  switch (k) {
+    case 0: return a->nodes;
+    case 1: return a->tris;
+    case 2: return a->store->codes + d.primOffset;
+    case 3: return a->store->ids + d.primOffset;
+    default: return a->header;
+  }
+}
+
+}  // namespace
+
+GPRT_API size_t gprtAccelSerialize(GPRTContext _ctx, GPRTAccel _a, void *out, size_t capacity) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  if (!ctx || !a) {
+    fail("gprtAccelSerialize: null argument");
+    return 0;
+  }
+  if (a->kind == AccelKind::Instances) {
+    fail("gprtAccelSerialize: instance accels are not serialised (serialise their triangle accels)");
+    return 0;
+  }
+  if (!a->built || !a->store) {
+    fail("gprtAccelSerialize: the accel is not built");
+    return 0;
+  }
+  const AccelPieces pc = accel_pieces(a->n);
+  const size_t bytes = sizeof(AccelFileHeader) + pc.total();
+  if (!out || capacity < bytes) return bytes;
+  uint8_t *dst = (uint8_t *)out + sizeof(AccelFileHeader);
+  for (int k = 0; k < 5; ++k) {
+    if (pc.bytes[k] && !hip_ok(hipMemcpyAsync(dst, accel_piece(a, k), pc.bytes[k], hipMemcpyDeviceToHost, ctx->stream),
+                               "hipMemcpyAsync(serialise)"))
+      return 0;
+    dst += pc.bytes[k];
+  }
+  if (!hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(serialise)")) return 0;
+  AccelFileHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kAccelMagic, 8);
+  h.version = kAccelVersion;
+  h.headerBytes = sizeof(AccelFileHeader);
+  h.kind = a->kind == AccelKind::AABBs ? 1u : 0u;
+  h.prims = a->n;
+  h.nodeBytes = sizeof(BVHNode);
+  h.recordBytes = sizeof(TriRecord);
+  h.buildMode = (uint32_t)a->mode;
+  h.crc = crc32_update(0u, (const uint8_t *)out + sizeof(AccelFileHeader), pc.total());
+  std::memcpy(out, &h, sizeof(h));
+  return bytes;
+}
+
+GPRT_API bool gprtAccelDeserialize(GPRTContext _ctx, GPRTAccel _a, const void *in, size_t bytes) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  if (!ctx || !a || !in) {
+    fail("gprtAccelDeserialize: null argument");
+    return false;
+  }
+  if (a->kind == AccelKind::Instances) {
+    fail("gprtAccelDeserialize: instance accels are not serialised (build them over deserialised triangle accels)");
+    return false;
+  }
+  if (bytes < sizeof(AccelFileHeader)) {
+    fail("gprtAccelDeserialize: %zu bytes is shorter than the 64-byte header", bytes);
+    return false;
+  }
+  AccelFileHeader h;
+  std::memcpy(&h, in, sizeof(h));
+  if (std::memcmp(h.magic, kAccelMagic, 8) != 0) {
+    fail("gprtAccelDeserialize: not a serialised accel (bad magic)");
+    return false;
+  }
+  if (h.version != kAccelVersion || h.headerBytes != sizeof(AccelFileHeader) || h.nodeBytes != sizeof(BVHNode) ||
+      h.recordBytes != sizeof(TriRecord)) {
+    fail("gprtAccelDeserialize: unsupported version %u (header %u, node %u, record %u bytes)", h.version, h.headerBytes,
+         h.nodeBytes, h.recordBytes);
+    return false;
+  }
+  if (h.buildMode > (uint32_t)GPRT_BUILD_MODE_FAST_TRACE_AND_UPDATE) {
+    fail("gprtAccelDeserialize: unknown build mode %u", h.buildMode);
+    return false;
+  }
+  const uint32_t kind = a->kind == AccelKind::AABBs ? 1u : 0u;
+  if (h.kind != kind) {
+    fail("gprtAccelDeserialize: the data is a %s accel, the target a %s accel", h.kind ? "AABB" : "triangle",
+         kind ? "AABB" : "triangle");
+    return false;
+  }
+  if (!validate_geom(a->geom, "gprtAccelDeserialize")) return false;
+  if (h.prims == 0 || h.prims != a->geom->icount) {
+    fail("gprtAccelDeserialize: the data holds %u primitives, the accel's geometry %u", h.prims, a->geom->icount);
+    return false;
+  }
+  const AccelPieces pc = accel_pieces(h.prims);
+  if (bytes != sizeof(AccelFileHeader) + pc.total()) {
+    fail("gprtAccelDeserialize: %zu bytes, expected %zu", bytes, sizeof(AccelFileHeader) + pc.total());
+    return false;
+  }
+  const uint8_t *src = (const uint8_t *)in + sizeof(AccelFileHeader);
+  if (crc32_update(0u, src, pc.total()) != h.crc) {
+    fail("gprtAccelDeserialize: checksum mismatch (the data is damaged)");
+    return false;
+  }
+  if (a->store && a->store->refs > 1) accel_detach(a);
+  if (!lbvh_build_batch(ctx, ctx->stream, &a, 1, false, true)) return false;
+  for (int k = 0; k < 5; ++k) {
+    if (pc.bytes[k] && !hip_ok(hipMemcpyAsync(accel_piece(a, k), src, pc.bytes[k], hipMemcpyHostToDevice, ctx->stream),
+                               "hipMemcpyAsync(deserialise)"))
+      return false;
+    src += pc.bytes[k];
+  }
+  if (!hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(deserialise)")) return false;
+  a->mode = (GPRTBuildMode)h.buildMode;
+  a->built = true;
+  return true;
+}
+
 GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t count, GPRTBuildParams params) {
   Context *ctx = (Context *)_ctx;
   if (!ctx || !_accels) {

@@ -21,6 +21,8 @@ namespace gprt {
 // ------------------------------------------------------------------------------------------------
 void fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 bool hip_ok(hipError_t e, const char *what);
+// CRC-32 (reflected IEEE 802.3 polynomial, as zlib's crc32) of checkpoints and serialised accels.
+uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n);
 
 // ------------------------------------------------------------------------------------------------
 // Context
@@ -264,7 +266,9 @@ void store_release(BVHStore *st);
 void accel_detach(Accel *a);
 // Builds every accel in `accels` into one store with one launch sequence (captured into a
 // hipGraph and replayed when `useGraph` and the batch shape is unchanged).
-bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph);
+// layoutOnly: lay out the store and the accels' views without building (deserialisation).
+bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph,
+                      bool layoutOnly = false);
 bool lbvh_refit(Context *ctx, hipStream_t s, Accel *a);
 bool lbvh_export(Context *ctx, hipStream_t s, Accel *a, int32_t *nodes, float *aabbs, uint64_t *keys);
 // Fills the store's parent links of one built triangle accel from its nodes.

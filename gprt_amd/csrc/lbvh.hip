@@ -1422,7 +1422,8 @@ void accel_detach(Accel *a) {
   }
 }
 
-bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph) {
+bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph,
+                      bool layoutOnly) {
   const SortConfig cfg = ctx->sortCfg;
   const uint32_t tileKeys = (uint32_t)cfg.tileKeys;
   std::vector<MeshDesc> desc(count);
@@ -1490,6 +1491,7 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
     set_views(accels[m]);
     if (!blas_desc_refresh(ctx, s, accels[m])) return false;
   }
+  if (layoutOnly) return true;  // the caller fills the arrays (a deserialised accel)
 
   if (useGraph) {
     if (!st->graphValid) {

@@ -196,27 +196,6 @@ static_assert(sizeof(CheckpointHeader) == 64, "checkpoint header is 64 bytes");
 constexpr char kCheckpointMagic[8] = {'G', 'P', 'R', 'T', 'M', 'L', 'P', '\0'};
 constexpr uint32_t kCheckpointVersion = 1;
 
-// CRC-32 with the reflected IEEE 802.3 polynomial 0xEDB88320, initial value and final xor ~0:
-// the checksum of zlib's crc32 and of PNG / gzip (table-driven, one byte at a time).
-struct Crc32Table {
-  uint32_t t[256];
-  Crc32Table() {
-    for (uint32_t i = 0; i < 256; ++i) {
-      uint32_t c = i;
-      for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-      t[i] = c;
-    }
-  }
-};
-
-uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n) {
-  static const Crc32Table tab;  // initialised once, thread-safe
-  const uint32_t *table = tab.t;
-  crc = ~crc;
-  for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
-  return ~crc;
-}
-
 }  // namespace
 
 GPRT_API size_t gprtProposalSerialize(GPRTContext _ctx, GPRTProposal _p, void *out, size_t capacity,

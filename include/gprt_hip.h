@@ -447,6 +447,23 @@ GPRT_API void gprtSynthBounceRays(GPRTContext context, GPRTAccel accel, GPRTBuff
                                   GPRTBuffer instanceIDs, uint32_t rayCount, GPRTBuffer out, uint32_t outCount,
                                   uint64_t seed, float tmin);
 
+/* Serialised accels (the on-disk form of a built triangle or AABB accel; no counterpart in the
+ * reference, which has no serialisation). Little-endian, version 1: a 64-byte header {char
+ * magic[8] = "GPRTBVH\0", u32 version = 1, u32 headerBytes = 64, u32 kind (0 triangles, 1
+ * AABBs), u32 primitive count n, u32 node bytes (64), u32 record bytes (48), u32 build mode, u32
+ * CRC-32 (IEEE 802.3, as zlib's crc32) of the payload, 24 reserved zero bytes}, then the inner
+ * nodes (n - 1 x 64 B), the leaf records (n x 48 B), the sorted Morton codes and primIDs (n x 4 B
+ * each) and the mesh header (16 floats: scene box, root box). Serialize returns the size and
+ * writes it when `capacity` suffices (query with out = NULL). Deserialize loads the data into an
+ * accel created over geometry of the same kind and primitive count (the geometry the data was
+ * built from: leaf records hold copies of the primitives, and an update refits from the
+ * geometry), which is then built: it traces, exports and, in an update mode, updates as after
+ * gprtAccelBuild. Every header field and the CRC are checked; a mismatch fails through the error
+ * path and leaves the accel as it was. Instance accels are not serialised (rebuild them over
+ * deserialised triangle accels, as their records name accels by address). */
+GPRT_API size_t gprtAccelSerialize(GPRTContext context, GPRTAccel accel, void *out, size_t capacity);
+GPRT_API bool gprtAccelDeserialize(GPRTContext context, GPRTAccel accel, const void *in, size_t bytes);
+
 /* Library version string, for logs. */
 GPRT_API const char *gprtVersion(void);
 
