@@ -70,6 +70,7 @@ _SIGS = {
     # name: (restype, argtypes)
     "gprtVersion": (C.c_char_p, []),
     "gprtSetErrorMode": (None, [C.c_int]),
+    "gprtSetSyncTimeout": (None, [_P, C.c_double]),
     "gprtGetLastError": (C.c_char_p, []),
     "gprtClearLastError": (None, []),
     "gprtContextCreate": (_P, [C.POINTER(C.c_int32), C.c_int]),

@@ -207,6 +207,7 @@ static bool validate_geom(Geom *g, const char *fn) {
 }
 
 GPRT_API void gprtAccelBuild(GPRTContext _ctx, GPRTAccel _a, GPRTBuildParams params) {
+  const StageRange range("gprtAccelBuild");
   Context *ctx = (Context *)_ctx;
   Accel *a = (Accel *)_a;
   if (!ctx || !a) {
@@ -269,6 +270,7 @@ void *accel_piece(Accel *a, int k) {
 }  // namespace
 
 GPRT_API size_t gprtAccelSerialize(GPRTContext _ctx, GPRTAccel _a, void *out, size_t capacity) {
+  const StageRange range("gprtAccelSerialize");
   Context *ctx = (Context *)_ctx;
   Accel *a = (Accel *)_a;
   if (!ctx || !a) {
@@ -310,6 +312,7 @@ GPRT_API size_t gprtAccelSerialize(GPRTContext _ctx, GPRTAccel _a, void *out, si
 }
 
 GPRT_API bool gprtAccelDeserialize(GPRTContext _ctx, GPRTAccel _a, const void *in, size_t bytes) {
+  const StageRange range("gprtAccelDeserialize");
   Context *ctx = (Context *)_ctx;
   Accel *a = (Accel *)_a;
   if (!ctx || !a || !in) {
@@ -376,6 +379,7 @@ GPRT_API bool gprtAccelDeserialize(GPRTContext _ctx, GPRTAccel _a, const void *i
 }
 
 GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t count, GPRTBuildParams params) {
+  const StageRange range("gprtAccelBuildBatch");
   Context *ctx = (Context *)_ctx;
   if (!ctx || !_accels) {
     fail("gprtAccelBuildBatch: null argument");
@@ -406,6 +410,7 @@ GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t
 }
 
 GPRT_API void gprtAccelUpdate(GPRTContext _ctx, GPRTAccel _a) {
+  const StageRange range("gprtAccelUpdate");
   Context *ctx = (Context *)_ctx;
   Accel *a = (Accel *)_a;
   if (!ctx || !a) {
@@ -458,6 +463,7 @@ GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel _a) {
 
 GPRT_API void gprtAccelTraceClosestHit(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _rays, GPRTBuffer _hits,
                                        uint32_t count) {
+  const StageRange range("gprtAccelTraceClosestHit");
   Context *ctx = (Context *)_ctx;
   Accel *a = (Accel *)_a;
   Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits;
@@ -516,6 +522,7 @@ GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _no
 // of the best configuration.
 GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_accels, uint32_t count, uint32_t reps,
                                               char *report, size_t cap) {
+  const StageRange range("gprtAccelBuildBatchAutotune");
   Context *ctx = (Context *)_ctx;
   if (!ctx || !_accels || count == 0) {
     fail("gprtAccelBuildBatchAutotune: null argument or empty batch");

@@ -112,6 +112,7 @@ GPRT_API void gprtComputeDestroy(GPRTCompute _c) {
 
 GPRT_API uint64_t gprtComputeLaunchArgs(GPRTCompute _c, const uint32_t numGroups[3], const uint32_t groupSize[3],
                                         const void *args, size_t bytes) {
+  const StageRange range("gprtComputeLaunchArgs");
   Compute *c = (Compute *)_c;
   if (!c || !numGroups || !groupSize || (bytes && !args)) {
     fail("gprtComputeLaunch: null argument");

@@ -24,6 +24,18 @@ bool hip_ok(hipError_t e, const char *what);
 // CRC-32 (reflected IEEE 802.3 polynomial, as zlib's crc32) of checkpoints and serialised accels.
 uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n);
 
+// Waits for the work enqueued on `s` with the context's deadline (gprtSetSyncTimeout; 6 s by
+// default, the reference's syncTDR of gprt/gprt.cpp:144): the stream is polled, and a wait that
+// outlives the deadline fails through the error path instead of blocking (a hung kernel).
+struct Context;
+bool stream_wait(Context *ctx, hipStream_t s, const char *what);
+
+// A named range around a C-ABI call for rocprofv3 --marker-trace (roctx).
+struct StageRange {
+  explicit StageRange(const char *name);
+  ~StageRange();
+};
+
 // ------------------------------------------------------------------------------------------------
 // Context
 // ------------------------------------------------------------------------------------------------
@@ -50,6 +62,7 @@ struct SortConfig {
 
 struct Context {
   SortConfig sortCfg;  // used by every sort and LBVH build issued on this context
+  double syncTimeout = 6.0;  // seconds a host wait may take before it is reported (0: no limit)
   int device = 0;
   int cuCount = 256;
   hipStream_t stream = nullptr;

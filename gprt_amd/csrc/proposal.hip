@@ -118,6 +118,7 @@ GPRT_API void gprtProposalDestroy(GPRTContext _ctx, GPRTProposal _p) {
 }
 
 GPRT_API uint32_t gprtProposalBindAccel(GPRTContext _ctx, GPRTProposal _p, GPRTAccel _a, uint32_t cutDepth) {
+  const StageRange range("gprtProposalBindAccel");
   Context *ctx = (Context *)_ctx;
   Proposal *p = (Proposal *)_p;
   Accel *a = (Accel *)_a;
@@ -391,6 +392,7 @@ float train_core(Context *ctx, Proposal *p, const void *rays, const void *hits, 
 
 GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _hits,
                                  uint32_t count, uint32_t epochs, float lr, float *accuracy) {
+  const StageRange range("gprtProposalTrain");
   Context *ctx = (Context *)_ctx;
   Proposal *p = (Proposal *)_p;
   Buffer *rays = (Buffer *)_rays, *hits = (Buffer *)_hits;
@@ -417,6 +419,7 @@ GPRT_API float gprtProposalTrain(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _
 
 GPRT_API float gprtProposalTrainLabels(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, GPRTBuffer _labels,
                                        uint32_t count, uint32_t epochs, float lr, float *accuracy) {
+  const StageRange range("gprtProposalTrainLabels");
   Context *ctx = (Context *)_ctx;
   Proposal *p = (Proposal *)_p;
   Buffer *rays = (Buffer *)_rays, *labels = (Buffer *)_labels;
@@ -434,6 +437,7 @@ GPRT_API float gprtProposalTrainLabels(GPRTContext _ctx, GPRTProposal _p, GPRTBu
 
 GPRT_API void gprtProposalInfer(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _rays, uint32_t count,
                                 GPRTBuffer _classes) {
+  const StageRange range("gprtProposalInfer");
   Context *ctx = (Context *)_ctx;
   Proposal *p = (Proposal *)_p;
   Buffer *rays = (Buffer *)_rays, *cls = (Buffer *)_classes;
@@ -525,6 +529,7 @@ void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *r
 
 GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
                                          GPRTBuffer _hits, uint32_t count, GPRTBuffer _visits) {
+  const StageRange range("gprtAccelTraceClosestHitEx");
   trace_detail("gprtAccelTraceClosestHitEx", (Context *)_ctx, (Accel *)_a, (Proposal *)_p, (Buffer *)_rays,
                (Buffer *)_hits, count, nullptr, nullptr, (Buffer *)_visits);
 }
@@ -532,6 +537,7 @@ GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTPro
 GPRT_API void gprtAccelTraceClosestHitDetail(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
                                              GPRTBuffer _hits, uint32_t count, GPRTBuffer _inst, GPRTBuffer _leaf,
                                              GPRTBuffer _visits) {
+  const StageRange range("gprtAccelTraceClosestHitDetail");
   trace_detail("gprtAccelTraceClosestHitDetail", (Context *)_ctx, (Accel *)_a, (Proposal *)_p, (Buffer *)_rays,
                (Buffer *)_hits, count, (Buffer *)_inst, (Buffer *)_leaf, (Buffer *)_visits);
 }

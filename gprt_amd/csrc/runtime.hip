@@ -2,10 +2,14 @@
 // the buffer / scan / sort families. Replaces the Vulkan runtime core of GPRT
 // (gprt/gprt.cpp:451-710 Context, :814-1196 Buffer, :6672-6840 queues) with one HIP device, one
 // in-order stream and hipMalloc'd HBM.
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <chrono>
 #include <csignal>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <thread>
 
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -40,6 +44,31 @@ bool hip_ok(hipError_t e, const char *what) {
 
 // CRC-32 with the reflected IEEE 802.3 polynomial 0xEDB88320, initial value and final xor ~0:
 // the checksum of zlib's crc32 and of PNG / gzip (table-driven, one byte at a time).
+bool stream_wait(Context *ctx, hipStream_t s, const char *what) {
+  const double limit = ctx ? ctx->syncTimeout : 0.0;
+  if (limit <= 0.0) return hip_ok(hipStreamSynchronize(s), what);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t poll = 0;; ++poll) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e == hipSuccess) return true;
+    if (e != hipErrorNotReady) return hip_ok(e, what);
+    const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (waited > limit) {
+      fail("%s: the GPU did not finish the context's work within %.3g s (sync timeout, gprtSetSyncTimeout)", what,
+           limit);
+      return false;
+    }
+    // Short waits stay responsive; long ones stop competing for the CPU.
+    if (poll < 4096)
+      std::this_thread::yield();
+    else
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+}
+
+StageRange::StageRange(const char *name) { roctxRangePushA(name); }
+StageRange::~StageRange() { roctxRangePop(); }
+
 struct Crc32Table {
   uint32_t t[256];
   Crc32Table() {
@@ -311,7 +340,7 @@ GPRT_API void gprtBufferMap(GPRTBuffer _b, int) {
     return;
   }
   // Wait for every enqueued kernel that may write this buffer before copying it out.
-  if (!hip_ok(hipStreamSynchronize(b->ctx->stream), "hipStreamSynchronize(map)")) return;
+  if (!stream_wait(b->ctx, b->ctx->stream, "gprtBufferMap")) return;
   if (b->kind == BufKind::Host) {
     b->mapped = true;
     return;
@@ -406,19 +435,22 @@ static uint32_t buffer_scan(GPRTContext _ctx, GPRTBuffer _in, GPRTBuffer _out, G
   scan_run(ctx, ctx->stream, (const uint32_t *)in->d, (uint32_t *)out->d, n, mode, selectPositive, sp, totalDev);
   if (!hip_ok(hipMemcpyAsync(ctx->pinnedWord, totalDev, 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync"))
     return (uint32_t)-1;
-  if (!hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(scan)")) return (uint32_t)-1;
+  if (!stream_wait(ctx, ctx->stream, fn)) return (uint32_t)-1;
   return *ctx->pinnedWord;
 }
 
 GPRT_API uint32_t gprtBufferExclusiveSum(GPRTContext c, GPRTBuffer in, GPRTBuffer out, GPRTBuffer scratch) {
+  const StageRange range("gprtBufferExclusiveSum");
   return buffer_scan(c, in, out, scratch, 0, false, "gprtBufferExclusiveSum");
 }
 GPRT_API uint32_t gprtBufferPartition(GPRTContext c, GPRTBuffer in, bool selectPositive, GPRTBuffer out,
                                       GPRTBuffer scratch) {
+  const StageRange range("gprtBufferPartition");
   return buffer_scan(c, in, out, scratch, 1, selectPositive, "gprtBufferPartition");
 }
 GPRT_API uint32_t gprtBufferSelect(GPRTContext c, GPRTBuffer in, bool selectPositive, GPRTBuffer out,
                                    GPRTBuffer scratch) {
+  const StageRange range("gprtBufferSelect");
   return buffer_scan(c, in, out, scratch, 2, selectPositive, "gprtBufferSelect");
 }
 
@@ -458,6 +490,7 @@ static void buffer_sort(Context *ctx, Buffer *keys, Buffer *vals, size_t n, int 
 }
 
 GPRT_API void gprtBufferSort(GPRTContext _ctx, GPRTBuffer _buf, GPRTBuffer _scratch) {
+  const StageRange range("gprtBufferSort");
   Context *ctx = (Context *)_ctx;
   Buffer *b = (Buffer *)_buf;
   if (!ctx || !b) {
@@ -472,6 +505,7 @@ GPRT_API void gprtBufferSort(GPRTContext _ctx, GPRTBuffer _buf, GPRTBuffer _scra
 }
 
 GPRT_API void gprtBufferSortPayload(GPRTContext _ctx, GPRTBuffer _keys, GPRTBuffer _vals, GPRTBuffer _scratch) {
+  const StageRange range("gprtBufferSortPayload");
   Context *ctx = (Context *)_ctx;
   Buffer *k = (Buffer *)_keys, *v = (Buffer *)_vals;
   if (!ctx || !k) {
@@ -491,6 +525,7 @@ GPRT_API void gprtBufferSortPayload(GPRTContext _ctx, GPRTBuffer _keys, GPRTBuff
 
 GPRT_API void gprtBufferSortKV(GPRTContext _ctx, GPRTBuffer _keys, GPRTBuffer _vals, uint32_t count, uint32_t keyBits,
                                uint32_t valueBits, GPRTBuffer _scratch) {
+  const StageRange range("gprtBufferSortKV");
   Context *ctx = (Context *)_ctx;
   Buffer *k = (Buffer *)_keys, *v = (Buffer *)_vals;
   if (!ctx || !k) {
@@ -565,15 +600,28 @@ GPRT_API void gprtGetSortConfig(GPRTContext _ctx, uint32_t *digitBits, uint32_t 
 GPRT_API uint64_t gprtComputeSynchronize(GPRTContext _ctx) {
   Context *ctx = (Context *)_ctx;
   if (!ctx) return 0;
-  hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  stream_wait(ctx, ctx->stream, "gprtComputeSynchronize");
   return 0;
 }
 
 GPRT_API uint64_t gprtDeviceSynchronize(GPRTContext _ctx) {
   Context *ctx = (Context *)_ctx;
   if (!ctx) return 0;
-  hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  if (stream_wait(ctx, ctx->stream, "gprtDeviceSynchronize")) hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
   return 0;
+}
+
+GPRT_API void gprtSetSyncTimeout(GPRTContext _ctx, double seconds) {
+  Context *ctx = (Context *)_ctx;
+  if (!ctx) {
+    fail("gprtSetSyncTimeout: null context");
+    return;
+  }
+  if (!(seconds >= 0.0)) {
+    fail("gprtSetSyncTimeout: %g s is not a duration (0 waits without limit)", seconds);
+    return;
+  }
+  ctx->syncTimeout = seconds;
 }
 
 GPRT_API void gprtBeginProfile(GPRTContext _ctx) {

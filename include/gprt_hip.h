@@ -268,6 +268,12 @@ GPRT_API float gprtEndProfile(GPRTContext context);
 #define GPRT_ERROR_RAISE 0  /* default: print and raise(SIGINT), as gprt/gprt.cpp:247-251 */
 #define GPRT_ERROR_RECORD 1 /* print, record the message, return without GPU work */
 GPRT_API void gprtSetErrorMode(int mode);
+/* Host-wait deadline (the reference waits with vkWaitSemaphores and syncTDR = 6 s,
+ * gprt/gprt.cpp:144, :6785-6840, and treats a timeout as an error). gprtComputeSynchronize,
+ * gprtDeviceSynchronize, gprtBufferMap and the scan family's returned totals poll the context's
+ * stream and report work that outlives `seconds` through the error path (default 6; 0 waits
+ * without limit). The work itself is not cancelled. */
+GPRT_API void gprtSetSyncTimeout(GPRTContext context, double seconds);
 GPRT_API const char *gprtGetLastError(void); /* NULL if no error since the last clear */
 GPRT_API void gprtClearLastError(void);
 

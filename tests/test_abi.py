@@ -149,3 +149,8 @@ def test_cpp_argument_packing_matches_python(tmp_path):
                    "int main() { B b{}; return (int)gprtComputeLaunch((GPRTCompute)0, {1, 1, 1}, {1, 1, 1}, 1.0, b); }\n")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, str(big)], capture_output=True, text=True)
     assert r.returncode != 0 and "PUSH_CONSTANTS_LIMIT" in r.stderr
+
+
+def test_sync_timeout_validated(rec):
+    rec.gprtSetSyncTimeout(None, 1.0)
+    assert "null context" in api.last_error()
