@@ -56,16 +56,16 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 11961.8e6 + 69.8e6,
-    "mlp_infer_top1": 1329.3e6 + 16.8e6,  # one launch of 4,194,304 rays
-    "lbvh_sort_pass": 31.7e6 + 37.4e6,
+    "trace_closest": 11943.0e6 + 69.8e6,
+    "mlp_infer_top1": 1325.0e6 + 16.8e6,  # one launch of 4,194,304 rays
+    "lbvh_sort_pass": 32.6e6 + 37.2e6,
     # per primitive, summed over the build kernels (init, bounds, keys, histogram scan, 4 sort
     # passes, sub-tile, region, top)
-    "build_per_prim": 0.5 + 60.1 + 16.4 + 4 * (7.56 + 8.92) + 257.3 + 112.4 + 1.2 + 0.1,
-    "batched_build_per_prim": 0.7 + 60.1 + 16.5 + 0.1 + 4 * (7.44 + 9.16) + 157.7 + 112.1 + 1.4 + 0.04,
-    "boxes_build_per_prim": 0.5 + 36.1 + 16.4 + 4 * (7.58 + 8.97) + 210.3 + 112.2 + 1.2 + 0.04,
-    "refit_per_prim": 257.0 + 112.4 + 1.2 + 0.1,  # sub-tile, region, top
-    "trace_boxes": 14945.6e6 + 77.8e6,  # one launch of 4,194,304 rays
+    "build_per_prim": 0.5 + 60.1 + 16.4 + 4 * (7.77 + 8.88) + 263.9 + 1.2 + 0.1,
+    "batched_build_per_prim": 0.7 + 60.1 + 16.5 + 0.1 + 4 * (7.61 + 9.18) + 220.1 + 1.4 + 0.04,
+    "boxes_build_per_prim": 0.5 + 36.1 + 16.4 + 4 * (7.78 + 8.93) + 293.9 + 1.2 + 0.04,
+    "refit_per_prim": 300.2 + 1.2 + 0.1,  # sub-tile, region, top
+    "trace_boxes": 14945.5e6 + 77.8e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;

@@ -302,9 +302,8 @@ __global__ void __launch_bounds__(256) k_hist_scan(BuildArgs a) {
 // ------------------------------------------------------------------------------------------------
 // Segmented one-sweep pass over (u32 Morton code, u32 primID).
 // ------------------------------------------------------------------------------------------------
-// 1024-thread tiles ask for at most 64 VGPRs so two workgroups (8 waves per SIMD) stay resident.
 template <int BITS, int THREADS, int ITEMS>
-__global__ void __launch_bounds__(THREADS, THREADS >= 1024 ? 8 : 1) k_onesweep(BuildArgs a, int pass) {
+__global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
   constexpr int R = 1 << BITS;
   constexpr int W = THREADS / 64;
   constexpr int TILE = THREADS * ITEMS;
@@ -1234,8 +1233,8 @@ void launch_sort_passes(Context *ctx, hipStream_t s, BVHStore *st, const BuildAr
       launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 16>, dim3(st->tiles), dim3(256),
              sort_lds_pad<BITS, 256, 16>(wpc), a, p);
     else
-      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 1024, 8>, dim3(st->tiles), dim3(1024),
-             sort_lds_pad<BITS, 1024, 8>(wpc), a, p);
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 512, 16>, dim3(st->tiles), dim3(512),
+             sort_lds_pad<BITS, 512, 16>(wpc), a, p);
   }
 }
 
