@@ -826,7 +826,7 @@ def main():
     ap.add_argument("--tune-epochs", type=int, default=2, help="dynamic: epochs per fine-tune")
     ap.add_argument("--cpu-rays", type=int, default=16384)
     ap.add_argument("--cpu-build-tris", type=int, default=1 << 20)
-    ap.add_argument("--train-rays", type=int, default=4 << 20)
+    ap.add_argument("--train-rays", type=int, default=16 << 20, help="proposal training rays (16,777,216: C4)")
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
