@@ -17,7 +17,8 @@ algorithmic bytes per launch (fixed by the workload, not by tuning knobs):
   * LBVH sort pass (u32 Morton key + u32 primID payload): read 8 B + write 8 B per key;
   * trace: 32 B ray + 16 B hit + 64 B per inner node fetched + 48 B per triangle tested, with
     the node / triangle counts measured by the CPU reference traversal on a ray sample;
-  * proposal inference: 1,140,224 FLOP per ray against the bf16 dense MFMA peak.
+  * proposal inference: 2 x (102 x 128 + 2 x 128^2 + 128 x classes) FLOP per ray against the bf16
+    dense MFMA peak (classes: the treelets bound, 2,897 on this scene; at most 4,096).
 The CPU baseline times the oracle (single thread) on a bounded sample and is reported beside it.
 
 Other workloads (one JSON line each): --workload batched (4,096 meshes, log-uniform 64..262,144
@@ -46,8 +47,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-MLP_FLOP_PER_RAY = 2 * (102 * 128 + 2 * 128 * 128 + 128 * 4096)
-MLP_OUT_FLOP_PER_RAY = 2 * 128 * 4096
+def mlp_flop_per_ray(classes):
+    """Inference FLOP per ray: encode-width input layer, two hidden layers, and the output layer
+    over the bound classes (the output kernel multiplies only the 128-class chunks that hold them)."""
+    return 2 * (102 * 128 + 2 * 128 * 128 + 128 * classes)
+
+
+def mlp_out_flop_per_ray(classes):
+    return 2 * 128 * classes
 N_TRIS = 4_194_304
 IMG = 2048
 N_BOUNCE = 4_194_304
@@ -195,10 +202,11 @@ def scene_workload(args):
     out_avg = out_ms_tot / max(out_n, 1)
     out_rays = n_rays * K / max(out_n, 1)  # rays per launch (large batches run in rounds)
     roof_mlp = {"kernel": "mlp_infer_top1 (output layer + argmax)", "bound": "mfma",
-                "achieved": out_rays * MLP_OUT_FLOP_PER_RAY / (out_avg * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "avg_launch_ms": out_avg, "flop_per_ray": MLP_OUT_FLOP_PER_RAY,
+                "achieved": out_rays * mlp_out_flop_per_ray(classes) / (out_avg * 1e-3) / 1e12,
+                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "avg_launch_ms": out_avg,
+                "flop_per_ray": mlp_out_flop_per_ray(classes), "classes": classes,
                 "rays_per_launch": out_rays, "traffic": TRAFFIC_BYTES["mlp_infer_top1"],
-                "whole_inference_tflops": n_rays * MLP_FLOP_PER_RAY / (infer_ms * 1e-3) / 1e12}
+                "whole_inference_tflops": n_rays * mlp_flop_per_ray(classes) / (infer_ms * 1e-3) / 1e12}
     roof_mlp["frac"] = roof_mlp["achieved"] / BF16_PEAK_TFLOPS
 
     cpu = None

@@ -541,9 +541,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(dst), "v"(g) : "memory", "m0");
     }
   };
+  // Only the chunks that hold bound classes are multiplied: later classes could never win.
+  const int nChunks = (int)min((uint32_t)kChunks, max(1u, (numClasses + kOutChunk - 1) / kOutChunk));
   issue(0, 0);
-  for (int c = 0; c < kChunks; ++c) {
-    if (c + 1 < kChunks) {
+  for (int c = 0; c < nChunks; ++c) {
+    if (c + 1 < nChunks) {
       issue(c + 1, (c + 1) & 1);
       __builtin_amdgcn_s_waitcnt(0x0F78);  // vmcnt(8): this wave's copies of chunk c have landed
     } else {
