@@ -20,10 +20,10 @@
 //   trace     closest hit with the open interval tmin < t (gprt/gprt_fallbacks.slang:108,139);
 //             triangle test = watertight ray/triangle intersection (Woop, Benthin, Wald, JCGT
 //             2013), restated here independently of the device code.
-//   boxes     AABB geometry (gprt/gprt_host.h:614, :1282; built by the driver at
-//             gprt/gprt.cpp:3961-4003): the LBVH stages above with a box's corners as its leaf box
+//   boxes     AABB geometry (gprt/gprt_host.h:614, :1282; the reference hands the boxes to the
+//             Vulkan implementation's builder, gprt/gprt.cpp:3961-4003): the LBVH stages above with a box's corners as its leaf box
 //             and its centre (lo + hi) * 0.5 as its key point; the closest hit is the box surface
-//             (entry distance, or the exit distance from inside), the build's stand-in for a user
+//             (entry distance, or the exit distance from inside), standing in for a user
 //             intersection program.
 //
 // Pinning (see tests/test_oracle.py): Morton known-answer values, closed forms of the t03
