@@ -63,9 +63,9 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 11943.0e6 + 69.8e6,
-    "mlp_infer_top1": 1325.0e6 + 16.8e6,  # one launch of 4,194,304 rays
-    "lbvh_sort_pass": 32.6e6 + 37.2e6,
+    "trace_closest": 11961.7e6 + 69.8e6,
+    "mlp_infer_top1": 1219.3e6 + 16.8e6,  # one launch of 4,194,304 rays, 2,897 classes bound
+    "lbvh_sort_pass": 32.6e6 + 37.5e6,
     # per primitive, summed over the build kernels (init, bounds, keys, histogram scan, 4 sort
     # passes, sub-tile, region, top)
     "build_per_prim": 0.5 + 60.1 + 16.4 + 4 * (7.77 + 8.88) + 263.9 + 1.2 + 0.1,
