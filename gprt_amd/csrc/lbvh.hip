@@ -180,29 +180,53 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
   float mn[3] = {INFINITY, INFINITY, INFINITY};
   float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   bool ident = true;
-  for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kBoundsThreads) {
-    float3 p, q, s;
+  // kBoundsBatch primitives per thread at a time: every index load of the batch is issued, then
+  // every vertex load, so a thread waits for two memory round trips per batch, not two per
+  // primitive (the centroid stores would otherwise keep the next primitive's loads behind them).
+  constexpr int kBoundsBatch = 4;
+  for (uint32_t i0 = r.lo + threadIdx.x; i0 < r.hi; i0 += kBoundsThreads * kBoundsBatch) {
+    float3 p[kBoundsBatch], q[kBoundsBatch], s[kBoundsBatch];
     if (d.boxes) {
-      load_prim(d, i, p, q, s);
+#pragma unroll
+      for (int u = 0; u < kBoundsBatch; ++u) {
+        const uint32_t i = min(i0 + u * kBoundsThreads, r.hi - 1);  // past the end: repeats a valid one
+        load_prim(d, i, p[u], q[u], s[u]);
+      }
     } else {
-      const auto *ix = global_ptr<uint32_t>(d.idx + (size_t)i * d.istride);
-      const uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
-      ident = ident && i0 == 3 * i && i1 == 3 * i + 1 && i2 == 3 * i + 2;
-      p = load_vertex(d, i0);
-      q = load_vertex(d, i1);
-      s = load_vertex(d, i2);
+      uint32_t ia[kBoundsBatch], ib[kBoundsBatch], ic[kBoundsBatch];
+#pragma unroll
+      for (int u = 0; u < kBoundsBatch; ++u) {
+        const uint32_t i = min(i0 + u * kBoundsThreads, r.hi - 1);
+        const auto *ix = global_ptr<uint32_t>(d.idx + (size_t)i * d.istride);
+        ia[u] = ix[0];
+        ib[u] = ix[1];
+        ic[u] = ix[2];
+      }
+#pragma unroll
+      for (int u = 0; u < kBoundsBatch; ++u) {
+        const uint32_t i = min(i0 + u * kBoundsThreads, r.hi - 1);
+        ident = ident && ia[u] == 3 * i && ib[u] == 3 * i + 1 && ic[u] == 3 * i + 2;
+        p[u] = load_vertex(d, ia[u]);
+        q[u] = load_vertex(d, ib[u]);
+        s[u] = load_vertex(d, ic[u]);
+      }
     }
-    const float3 c = prim_centroid(d, p, q, s);
-    float *cd = a.centroids + 3 * ((size_t)d.primOffset + i);
-    cd[0] = c.x;
-    cd[1] = c.y;
-    cd[2] = c.z;
-    mn[0] = fmin_ord(mn[0], fmin_ord(p.x, fmin_ord(q.x, s.x)));
-    mn[1] = fmin_ord(mn[1], fmin_ord(p.y, fmin_ord(q.y, s.y)));
-    mn[2] = fmin_ord(mn[2], fmin_ord(p.z, fmin_ord(q.z, s.z)));
-    mx[0] = fmax_ord(mx[0], fmax_ord(p.x, fmax_ord(q.x, s.x)));
-    mx[1] = fmax_ord(mx[1], fmax_ord(p.y, fmax_ord(q.y, s.y)));
-    mx[2] = fmax_ord(mx[2], fmax_ord(p.z, fmax_ord(q.z, s.z)));
+#pragma unroll
+    for (int u = 0; u < kBoundsBatch; ++u) {
+      const uint32_t i = i0 + u * kBoundsThreads;
+      if (i >= r.hi) break;  // uniform per thread; the min/max below only see real primitives
+      const float3 c = prim_centroid(d, p[u], q[u], s[u]);
+      float *cd = a.centroids + 3 * ((size_t)d.primOffset + i);
+      cd[0] = c.x;
+      cd[1] = c.y;
+      cd[2] = c.z;
+      mn[0] = fmin_ord(mn[0], fmin_ord(p[u].x, fmin_ord(q[u].x, s[u].x)));
+      mn[1] = fmin_ord(mn[1], fmin_ord(p[u].y, fmin_ord(q[u].y, s[u].y)));
+      mn[2] = fmin_ord(mn[2], fmin_ord(p[u].z, fmin_ord(q[u].z, s[u].z)));
+      mx[0] = fmax_ord(mx[0], fmax_ord(p[u].x, fmax_ord(q[u].x, s[u].x)));
+      mx[1] = fmax_ord(mx[1], fmax_ord(p[u].y, fmax_ord(q[u].y, s[u].y)));
+      mx[2] = fmax_ord(mx[2], fmax_ord(p[u].z, fmax_ord(q[u].z, s[u].z)));
+    }
   }
   __shared__ float sb[kBoundsThreads / 64][6];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -252,19 +276,34 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
   const uint32_t *h = (const uint32_t *)a.headers + (size_t)m * 16;
   const float lox = unord(h[0]), loy = unord(h[1]), loz = unord(h[2]);
   const float ex = unord(h[3]) - lox, ey = unord(h[4]) - loy, ez = unord(h[5]) - loz;
-  for (uint32_t i = r.lo + threadIdx.x; i < r.hi; i += kThreads) {
-    const float *cs = a.centroids + 3 * ((size_t)d.primOffset + i);
-    const float cx = cs[0], cy = cs[1], cz = cs[2];
-    const float qx = fminf(fmaxf((cx - lox) / ex * 1024.0f, 0.0f), 1023.0f);
-    const float qy = fminf(fmaxf((cy - loy) / ey * 1024.0f, 0.0f), 1023.0f);
-    const float qz = fminf(fmaxf((cz - loz) / ez * 1024.0f, 0.0f), 1023.0f);
-    const uint32_t code = morton30((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
-    a.codes[d.primOffset + i] = code;
-    if (multi) {
-      for (int ps = 0; ps < a.passes; ++ps) {
-        const int shift = ps * BITS;
-        const int bits = min(BITS, kMortonBits - shift);
-        atomicAdd(&s_h[ps * R + ((code >> shift) & ((1u << bits) - 1u))], 1u);
+  // kKeysBatch keys per thread at a time: the centroid loads of a batch are issued together (the
+  // code stores would otherwise keep the next key's loads behind them).
+  constexpr int kKeysBatch = 8;
+  for (uint32_t i0 = r.lo + threadIdx.x; i0 < r.hi; i0 += kThreads * kKeysBatch) {
+    float cx[kKeysBatch], cy[kKeysBatch], cz[kKeysBatch];
+#pragma unroll
+    for (int u = 0; u < kKeysBatch; ++u) {
+      const uint32_t i = min(i0 + u * kThreads, r.hi - 1);  // past the end: repeats a valid one
+      const float *cs = a.centroids + 3 * ((size_t)d.primOffset + i);
+      cx[u] = cs[0];
+      cy[u] = cs[1];
+      cz[u] = cs[2];
+    }
+#pragma unroll
+    for (int u = 0; u < kKeysBatch; ++u) {
+      const uint32_t i = i0 + u * kThreads;
+      if (i >= r.hi) break;
+      const float qx = fminf(fmaxf((cx[u] - lox) / ex * 1024.0f, 0.0f), 1023.0f);
+      const float qy = fminf(fmaxf((cy[u] - loy) / ey * 1024.0f, 0.0f), 1023.0f);
+      const float qz = fminf(fmaxf((cz[u] - loz) / ez * 1024.0f, 0.0f), 1023.0f);
+      const uint32_t code = morton30((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
+      a.codes[d.primOffset + i] = code;
+      if (multi) {
+        for (int ps = 0; ps < a.passes; ++ps) {
+          const int shift = ps * BITS;
+          const int bits = min(BITS, kMortonBits - shift);
+          atomicAdd(&s_h[ps * R + ((code >> shift) & ((1u << bits) - 1u))], 1u);
+        }
       }
     }
   }
