@@ -25,11 +25,13 @@ Other workloads (one JSON line each): --workload batched (4,096 meshes, log-unif
 triangles: per-mesh launch sequence vs one hipGraph-captured batch launch vs the autotuned
 configuration); dynamic (64 meshes rebuilt per frame under an instance accel, plain vs learned
 traces); refit (refit-only updates of the blob scene); boxes (LBVH build and trace over 4,194,304
-AABB primitives); compute (a user program through gprtComputeLaunch); copy (HBM counter
-calibration). bench.py --gpus N with N > 1 exits: this build measures one GPU.
+AABB primitives); compute (a user program through gprtComputeLaunch); sort (the radix-sort
+microbenchmark: 2^26 u32 keys + u32 payloads, 68 B/key for the whole sort, 16 B/key per pass);
+scan (the exclusive-scan microbenchmark: 2^28 u32, 8 B/element); copy (HBM counter calibration).
+bench.py --gpus N with N > 1 exits: this build measures one GPU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                       [--workload scene|batched|dynamic|refit|boxes|compute|copy]
+                       [--workload scene|batched|dynamic|refit|boxes|compute|sort|scan|copy]
 """
 from __future__ import annotations
 
@@ -74,6 +76,8 @@ TRAFFIC_BYTES = {
     "refit_per_prim": 300.2 + 1.2 + 0.1,  # sub-tile, region, top
     "trace_boxes": 14945.5e6 + 77.8e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
+    "sort_onesweep": 583.4e6 + 615.4e6,  # one pass over 2^26 u32 keys + u32 payloads (8,192-key tiles)
+    "scan_lookback": 1084.5e6 + 1075.3e6,  # one launch over 2^28 u32
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
 # commands and per-kernel table in profiles/cache_hit_rates_scene.md).
@@ -825,6 +829,172 @@ def compute_workload(args):
     ctx.close()
 
 
+SORT_KEYS = 1 << 26  # u32 keys uniform over 32 bits + u32 payload = index
+SORT_BYTES_PER_KEY = 4 + 4 * 2 * (4 + 4)  # histogram read + 4 passes x (read + write) x (key + payload)
+SORT_PASS_BYTES_PER_KEY = 2 * (4 + 4)
+SCAN_ELEMS = 1 << 28  # u32 uniform in [0, 255]
+SCAN_BYTES_PER_ELEM = 4 + 4
+
+
+SORT_KERNELS = ("fill_u32", "sort_hist", "sort_hist_scan", "sort_onesweep", "sort_final_copy")  # fill: look-back state
+
+
+def _device_ms(per_kernel, names):
+    return sum(per_kernel[k]["total_ms"] for k in names if k in per_kernel)
+
+
+def sort_workload(args):
+    """The radix-sort microbenchmark: gprtBufferSortKV over 2^26 u32 keys uniform over 32 bits
+    with a u32 payload (its index). Every step restores the unsorted input with a device copy
+    first (outside the sort's kernels), so each sort sees the same random keys; the time is the
+    sum of the sort's own kernels from the library's per-launch HIP events."""
+    from gprt_amd import api, scenes
+
+    lib = api.load()
+    n = SORT_KEYS
+    keys = (scenes.rand_u64(4, 0, n) >> np.uint64(32)).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    ctx = api.Context(0)
+    k0, v0 = ctx.device_buffer(keys), ctx.device_buffer(vals)
+    kb, vb = ctx.device_buffer(nbytes=4 * n), ctx.device_buffer(nbytes=4 * n)
+    scratch = ctx.device_buffer(nbytes=16 * n)  # ample; the sort takes what it needs
+
+    def one():
+        lib.gprtBufferCopy(ctx.h, k0.h, kb.h, 0, 0, 4, n, 0, 0)
+        lib.gprtBufferCopy(ctx.h, v0.h, vb.h, 0, 0, 4, n, 0, 0)
+        lib.gprtBufferSortKV(ctx.h, kb.h, vb.h, n, 32, 32, scratch.h)
+
+    for _ in range(args.warmup):
+        one()
+    ctx.sync()
+    assert api.last_error() is None, api.last_error()
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    pk = kernel_report(lib, ctx)
+    K = args.steps
+    sort_ms = _device_ms(pk, SORT_KERNELS) / K
+    passes = pk.get("sort_onesweep", {"launches": 0})["launches"] // max(K, 1)
+    pass_ms = pk["sort_onesweep"]["total_ms"] / pk["sort_onesweep"]["launches"]
+    # The same sort under each tile size of the sort configuration (8-bit digits), for comparison.
+    sweep = {}
+    for tile in (2048, 4096, 8192):
+        lib.gprtSetSortConfig(ctx.h, 8, tile, 0)
+        one()
+        ctx.sync()
+        lib.gprtKernelTimerReset(ctx.h)
+        lib.gprtKernelTimerEnable(ctx.h, True)
+        for _ in range(5):
+            one()
+        ctx.sync()
+        lib.gprtKernelTimerEnable(ctx.h, False)
+        sweep[tile] = _device_ms(kernel_report(lib, ctx), SORT_KERNELS) / 5
+    lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+    got_k, got_v = kb.read(np.uint32), vb.read(np.uint32)
+    order = np.argsort(keys, kind="stable")
+    ok = bool(np.array_equal(got_k, keys[order]) and np.array_equal(got_v, order.astype(np.uint32)))
+    roof = _roof_hbm("sort_onesweep (one 8-bit pass over key + payload)", n * SORT_PASS_BYTES_PER_KEY, pass_ms,
+                     bytes_per_key=SORT_PASS_BYTES_PER_KEY)
+    roof["traffic"] = TRAFFIC_BYTES.get("sort_onesweep")
+    whole = SORT_BYTES_PER_KEY * n / (sort_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as orc  # the CPU restatement, timed as the baseline only
+        m = 1 << 24
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            orc.stable_sort(keys[:m], vals[:m])
+            ts.append(time.perf_counter() - t1)
+        cpu = {"value": m / float(np.median(ts)) / 1e6, "unit": "Mkeys/s", "cores": 1, "kind": "port",
+               "sample": f"oracle std::stable_sort of the first {m} key/payload pairs (u32 + u32), median of 3, "
+                         f"single thread"}
+    out = {
+        "metric": "radix sort Mkeys/s (2^26 u32 keys + u32 payload, gprtBufferSortKV), 1 GPU",
+        "value": n / (sort_ms * 1e3), "unit": "Mkeys/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic (SplitMix64 keys, seed 4; payload = index)",
+        "config": {"workload": "sort microbenchmark: 67,108,864 u32 keys + u32 payloads", "keys": n},
+        "components": {"sort_device_ms": sort_ms, "passes": passes, "pass_ms": pass_ms,
+                       "whole_sort_gbs": whole, "whole_sort_frac": whole / HBM_PEAK_GBS,
+                       "whole_sort_bytes_per_key": SORT_BYTES_PER_KEY, "sorted_matches_numpy": ok,
+                       "sort_ms_by_tile_keys": sweep,
+                       "wall_ms_per_step_includes": "two 256 MiB device copies restoring the input"},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": pk,
+    }
+    print(json.dumps(out))
+    for b in (k0, v0, kb, vb, scratch):
+        b.destroy()
+    ctx.close()
+
+
+def scan_workload(args):
+    """The exclusive-scan microbenchmark: gprtBufferExclusiveSum over 2^28 u32 uniform in
+    [0, 255] (the total wraps modulo 2^32), timed from the library's per-launch HIP events."""
+    from gprt_amd import api, scenes
+
+    lib = api.load()
+    n = SCAN_ELEMS
+    x = (scenes.rand_u64(5, 0, n) >> np.uint64(56)).astype(np.uint32)
+    ctx = api.Context(0)
+    inp, outb = ctx.device_buffer(x), ctx.device_buffer(nbytes=4 * n)
+    scratch = ctx.device_buffer(nbytes=4)
+    for _ in range(args.warmup):
+        lib.gprtBufferExclusiveSum(ctx.h, inp.h, outb.h, scratch.h)
+    ctx.sync()
+    assert api.last_error() is None, api.last_error()
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    t0 = time.perf_counter()
+    total = 0
+    for _ in range(args.steps):
+        total = lib.gprtBufferExclusiveSum(ctx.h, inp.h, outb.h, scratch.h)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    pk = kernel_report(lib, ctx)
+    K = args.steps
+    lb = pk["scan_lookback"]
+    ms = lb["total_ms"] / lb["launches"]
+    scan_ms = sum(v["total_ms"] for k, v in pk.items() if k.startswith("scan")) / K
+    got = outb.read(np.uint32)
+    c = np.cumsum(x, dtype=np.uint64)
+    ok = bool(int(total) == int(c[-1]) & 0xFFFFFFFF and got[0] == 0 and
+              np.array_equal(got[1:], (c[:-1] & np.uint64(0xFFFFFFFF)).astype(np.uint32)))
+    del c
+    roof = _roof_hbm("scan_lookback (decoupled look-back exclusive sum)", n * SCAN_BYTES_PER_ELEM, ms,
+                     bytes_per_elem=SCAN_BYTES_PER_ELEM)
+    roof["traffic"] = TRAFFIC_BYTES.get("scan_lookback")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as orc  # the CPU restatement, timed as the baseline only
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            orc.exclusive_sum(x)
+            ts.append(time.perf_counter() - t1)
+        cpu = {"value": n / float(np.median(ts)) / 1e9, "unit": "Gelems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle exclusive scan of all {n} elements, median of 3, single thread"}
+    out = {
+        "metric": "exclusive scan Gelems/s (2^28 u32, gprtBufferExclusiveSum), 1 GPU",
+        "value": n / (scan_ms * 1e-3) / 1e9, "unit": "Gelems/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic (SplitMix64 values in [0, 255], seed 5)",
+        "config": {"workload": "scan microbenchmark: 268,435,456 u32", "elements": n},
+        "components": {"scan_device_ms": scan_ms, "lookback_ms": ms, "total_matches_numpy": ok},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": pk,
+    }
+    print(json.dumps(out))
+    for b in (inp, outb, scratch):
+        b.destroy()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -832,7 +1002,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="scene",
-                    choices=["scene", "batched", "dynamic", "copy", "refit", "boxes", "compute"])
+                    choices=["scene", "batched", "dynamic", "copy", "refit", "boxes", "compute", "sort", "scan"])
     ap.add_argument("--frames", type=int, default=256, help="dynamic: timed frames")
     ap.add_argument("--tune-rays", type=int, default=262144, help="dynamic: labelled rays per fine-tune")
     ap.add_argument("--tune-epochs", type=int, default=2, help="dynamic: epochs per fine-tune")
@@ -863,6 +1033,10 @@ def main():
         boxes_workload(args)
     elif args.workload == "compute":
         compute_workload(args)
+    elif args.workload == "sort":
+        sort_workload(args)
+    elif args.workload == "scan":
+        scan_workload(args)
     else:
         scene_workload(args)
 

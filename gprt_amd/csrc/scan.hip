@@ -25,9 +25,10 @@ namespace {
 constexpr uint64_t kAgg = 1ull << 32;
 constexpr uint64_t kInc = 2ull << 32;
 
-template <int WAVES, int K>
+// MODE 0: exclusive sum; 1: partition; 2: select.
+template <int WAVES, int K, int MODE>
 __global__ void __launch_bounds__(WAVES * 64)
-    k_scan(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint32_t n, int mode, int selectPositive,
+    k_scan(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint32_t n, int selectPositive,
            uint64_t *__restrict__ state, uint32_t numTiles) {
   constexpr uint32_t WAVE_ITEMS = 64 * K * 4;
   constexpr uint32_t TILE = WAVES * WAVE_ITEMS;
@@ -44,8 +45,10 @@ __global__ void __launch_bounds__(WAVES * 64)
   const uint32_t tile = s_tile;
   const size_t waveBase = (size_t)tile * TILE + (size_t)w * WAVE_ITEMS;
 
+  // Only the items stay in registers; what an item counts (itself for the sum, its selection flag
+  // for partition / select) is recomputed from it, and its prefix from its 4-item group's exclusive
+  // offset, when the results are written.
   uint32_t v[K][4];
-  uint32_t f[K][4];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const size_t idx = waveBase + ((size_t)k * 64 + l) * 4;
@@ -59,30 +62,22 @@ __global__ void __launch_bounds__(WAVES * 64)
 #pragma unroll
       for (int c = 0; c < 4; ++c) v[k][c] = (idx + c < n) ? in[idx + c] : 0u;
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (mode == 0) {
-        f[k][c] = v[k][c];
-      } else {
-        const bool inRange = idx + c < n;
-        const bool sel = selectPositive ? ((int32_t)v[k][c] >= 0) : ((int32_t)v[k][c] < 0);
-        f[k][c] = (inRange && sel) ? 1u : 0u;
-      }
-    }
   }
+  auto counted = [&](int k, int c) -> uint32_t {
+    if (MODE == 0) return v[k][c];
+    const size_t idx = waveBase + ((size_t)k * 64 + l) * 4 + c;
+    const bool sel = selectPositive ? ((int32_t)v[k][c] >= 0) : ((int32_t)v[k][c] < 0);
+    return (idx < n && sel) ? 1u : 0u;
+  };
 
-  // Exclusive prefix of every item within the wave's chunk (order: k, lane, component).
-  uint32_t pre[K][4];
+  // Exclusive prefix of each lane's 4-item group within the wave's chunk (order: k, lane, item).
+  uint32_t gex[K];
   uint32_t carry = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint32_t s = f[k][0] + f[k][1] + f[k][2] + f[k][3];
+    const uint32_t s = counted(k, 0) + counted(k, 1) + counted(k, 2) + counted(k, 3);
     const uint32_t incl = wave_incl_sum(s);
-    const uint32_t excl = incl - s + carry;
-    pre[k][0] = excl;
-    pre[k][1] = excl + f[k][0];
-    pre[k][2] = pre[k][1] + f[k][1];
-    pre[k][3] = pre[k][2] + f[k][2];
+    gex[k] = incl - s + carry;
     carry += wave_read(incl, 63);
   }
   if (l == 0) s_wsum[w] = carry;
@@ -138,24 +133,27 @@ __global__ void __launch_bounds__(WAVES * 64)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const size_t idx = waveBase + ((size_t)k * 64 + l) * 4;
-    if (mode == 0) {
+    uint32_t r[4];
+    r[0] = prefix + gex[k];
+#pragma unroll
+    for (int c = 1; c < 4; ++c) r[c] = r[c - 1] + counted(k, c - 1);
+    if (MODE == 0) {
       if (idx + 3 < n) {
-        *(uint4 *)(out + idx) = make_uint4(prefix + pre[k][0], prefix + pre[k][1], prefix + pre[k][2],
-                                           prefix + pre[k][3]);
+        *(uint4 *)(out + idx) = make_uint4(r[0], r[1], r[2], r[3]);
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-          if (idx + c < n) out[idx + c] = prefix + pre[k][c];
+          if (idx + c < n) out[idx + c] = r[c];
       }
     } else {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const size_t i = idx + c;
         if (i >= n) continue;
-        const uint32_t rank = prefix + pre[k][c];
-        if (f[k][c]) {
+        const uint32_t rank = r[c];
+        if (counted(k, c)) {
           out[rank] = v[k][c];
-        } else if (mode == 1) {
+        } else if (MODE == 1) {
           out[(n - 1) - ((uint32_t)i - rank)] = v[k][c];
         }
       }
@@ -163,24 +161,37 @@ __global__ void __launch_bounds__(WAVES * 64)
   }
 }
 
-constexpr int kScanWaves = 4;
-constexpr int kScanK = 4;
-constexpr uint32_t kScanTile = kScanWaves * 64 * kScanK * 4;
+// Tiles: 16,384 items for the sum (8 waves x 32 items per lane: fewer tiles, so fewer tickets and
+// look-back steps per item), 4,096 for partition / select (4 waves x 16 items per lane), whose
+// flags and scattered writes need more registers.
+constexpr int kScanWavesSum = 8, kScanWavesSel = 4;
+constexpr int kScanKSum = 8, kScanKSel = 4;
+constexpr uint32_t kScanTileSum = kScanWavesSum * 64 * kScanKSum * 4;
+constexpr uint32_t kScanTileSel = kScanWavesSel * 64 * kScanKSel * 4;
 
 }  // namespace
 
 size_t scan_scratch_bytes(size_t n) {
-  size_t tiles = (n + kScanTile - 1) / kScanTile;
+  size_t tiles = (n + kScanTileSel - 1) / kScanTileSel;  // the smaller tile: enough for either
   return (tiles + 2) * sizeof(uint64_t);
 }
 
 void scan_run(Context *ctx, hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, int mode, bool selectPositive,
               void *scratch, uint32_t *totalDev) {
   (void)totalDev;  // state[0] is the total; the caller reads it from the scratch base
-  const uint32_t tiles = (uint32_t)((n + kScanTile - 1) / kScanTile);
+  const uint32_t tileItems = mode == 0 ? kScanTileSum : kScanTileSel;
+  const uint32_t tiles = (uint32_t)((n + tileItems - 1) / tileItems);
   fill_u32(ctx, s, scratch, 0u, ((size_t)tiles + 2) * 2);
-  launch(ctx, s, "scan_lookback", k_scan<kScanWaves, kScanK>, dim3(tiles), dim3(kScanWaves * 64), 0, in, out,
-         (uint32_t)n, mode, selectPositive ? 1 : 0, (uint64_t *)scratch, tiles);
+  const int sel = selectPositive ? 1 : 0;
+  if (mode == 0)
+    launch(ctx, s, "scan_lookback", k_scan<kScanWavesSum, kScanKSum, 0>, dim3(tiles), dim3(kScanWavesSum * 64), 0, in,
+           out, (uint32_t)n, sel, (uint64_t *)scratch, tiles);
+  else if (mode == 1)
+    launch(ctx, s, "scan_lookback", k_scan<kScanWavesSel, kScanKSel, 1>, dim3(tiles), dim3(kScanWavesSel * 64), 0, in,
+           out, (uint32_t)n, sel, (uint64_t *)scratch, tiles);
+  else
+    launch(ctx, s, "scan_lookback", k_scan<kScanWavesSel, kScanKSel, 2>, dim3(tiles), dim3(kScanWavesSel * 64), 0, in,
+           out, (uint32_t)n, sel, (uint64_t *)scratch, tiles);
 }
 
 }  // namespace gprt

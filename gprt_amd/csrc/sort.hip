@@ -61,15 +61,26 @@ __global__ void __launch_bounds__(256)
   extern __shared__ uint32_t s_h[];  // [passes][R]
   for (int i = threadIdx.x; i < passes * R; i += blockDim.x) s_h[i] = 0;
   __syncthreads();
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // kHistBatch keys per thread at a time, each load coalesced over the grid: the batch's loads are
+  // all in flight before the first key's counts (a load per thread at a time leaves HBM idle).
+  constexpr int kHistBatch = 8;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    const KT k = keys[i];
-    for (int p = 0; p < passes; ++p) {
-      const int shift = loBit + p * BITS;
-      const int bits = (hiBit - shift) < BITS ? (hiBit - shift) : BITS;
-      const uint32_t d = digit_of(k, shift, (1u << bits) - 1u);
-      atomicAdd(&s_h[p * R + d], 1u);
+  for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += stride * kHistBatch) {
+    KT k[kHistBatch];
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u) {
+      const size_t i = i0 + (size_t)u * stride;
+      k[u] = i < n ? keys[i] : (KT)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kHistBatch; ++u) {
+      if (i0 + (size_t)u * stride >= n) break;  // the rest of the batch is past the end too
+      for (int p = 0; p < passes; ++p) {
+        const int shift = loBit + p * BITS;
+        const int bits = (hiBit - shift) < BITS ? (hiBit - shift) : BITS;
+        const uint32_t d = digit_of(k[u], shift, (1u << bits) - 1u);
+        atomicAdd(&s_h[p * R + d], 1u);
+      }
     }
   }
   __syncthreads();
