@@ -9,7 +9,8 @@
 //    over all keys as inactive (they would be identity permutations);
 //  * one k_onesweep launch per pass: a tile ticket, wave64 multisplit ranking (BITS ballots per
 //    key give each lane the mask of peers with the same digit), per-digit decoupled look-back
-//    across tiles, a stable local reorder in LDS and a global write in digit runs;
+//    across tiles (four predecessors per step), a stable local reorder in LDS and a global write
+//    in digit runs;
 //  * ping-pong buffers are picked on the device from the active-pass mask, and k_final_copy
 //    moves the result back when an odd number of passes ran — no host round trip, so the whole
 //    sort can be captured in a hipGraph.
@@ -203,6 +204,17 @@ __global__ void __launch_bounds__(THREADS)
     k[j] = idx < n ? src[idx] : ~(KT)0;
     d[j] = digit_of(k[j], shift, dmask);
   }
+  // 32-bit payloads of 32-bit keys are loaded with the keys, so their loads overlap the ranking and
+  // the look-back (the registers fit without lowering occupancy); wider ones after the key write.
+  constexpr bool kEarlyVals = VB == 32 && sizeof(KT) == 4;
+  VT ve[kEarlyVals ? ITEMS : 1];
+  if constexpr (kEarlyVals) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const size_t idx = waveBase + (size_t)j * 64 + l;
+      ve[j] = idx < n ? vsrc[idx] : (VT)0;
+    }
+  }
 
   // Wave multisplit: peers with the same digit, rank among them, wave-private running counts.
 #pragma unroll
@@ -267,17 +279,29 @@ __global__ void __launch_bounds__(THREADS)
     if (dd >= R) continue;
     uint32_t excl = 0;
     if (tile > 0) {
+      // Four predecessors per step (four loads in flight): aggregates are summed up to the first
+      // inclusive value; at a tile that has not published yet the walk waits and resumes there.
       int64_t t = (int64_t)tile - 1;
       while (t >= 0) {
-        const uint32_t s = ld_agent(&st.status[((size_t)pass * numTiles + t) * R + dd]);
-        const uint32_t flag = s & ~kValueMask;
-        if (flag == 0u) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
+        const int nb = (int)(t + 1 < 4 ? t + 1 : 4);
+        uint32_t sv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i < nb) sv[i] = ld_agent(&st.status[((size_t)pass * numTiles + (t - i)) * R + dd]);
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (done || i >= nb || used < i) continue;
+          const uint32_t flag = sv[i] & ~kValueMask;
+          if (flag == 0u) continue;  // not published: stop consuming here
+          excl += sv[i] & kValueMask;
+          used = i + 1;
+          done = flag == kFlagInc;
         }
-        excl += s & kValueMask;
-        if (flag == kFlagInc) break;
-        --t;
+        if (done) break;
+        t -= used;
+        if (used < nb) __builtin_amdgcn_s_sleep(1);
       }
       st_agent(&st.status[((size_t)pass * numTiles + tile) * R + dd], kFlagInc | (excl + cnt[q]));
     }
@@ -320,8 +344,12 @@ __global__ void __launch_bounds__(THREADS)
     VT *s_vals = (VT *)s_buf;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      const size_t idx = waveBase + (size_t)j * 64 + l;
-      s_vals[pos[j]] = idx < n ? vsrc[idx] : (VT)0;
+      if constexpr (kEarlyVals) {
+        s_vals[pos[j]] = ve[j];
+      } else {
+        const size_t idx = waveBase + (size_t)j * 64 + l;
+        s_vals[pos[j]] = idx < n ? vsrc[idx] : (VT)0;
+      }
     }
     __syncthreads();
 #pragma unroll
