@@ -76,7 +76,7 @@ TRAFFIC_BYTES = {
     "refit_per_prim": 300.2 + 1.2 + 0.1,  # sub-tile, region, top
     "trace_boxes": 14945.5e6 + 77.8e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
-    "sort_onesweep": 583.4e6 + 615.4e6,  # one pass over 2^26 u32 keys + u32 payloads (8,192-key tiles)
+    "sort_onesweep": 599.1e6 + 617.4e6,  # one pass over 2^26 u32 keys + u32 payloads (8,192-key tiles)
     "scan_lookback": 1084.5e6 + 1075.3e6,  # one launch over 2^28 u32
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
