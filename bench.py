@@ -104,6 +104,37 @@ def ms_of(per_kernel, prefix):
     return ms, n
 
 
+def _median_time(fn, setup=None, reps=5, warmup=1):
+    """Seconds per call: one warm-up, then the median of `reps` timed calls (steady clock); `setup`
+    runs untimed before each call."""
+    ts = []
+    for i in range(warmup + reps):
+        if setup is not None:
+            setup()
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        if i >= warmup:
+            ts.append(dt)
+    return float(np.median(ts))
+
+
+def _cpu_baseline(value, unit, threads, sample, single_value, single_sample, **extra):
+    """cpu_baseline record: the all-cores OpenMP leg as `value` (`cores` = its threads), the
+    single-thread oracle leg beside it, and the host the two ran on (read at run time)."""
+    from oracle import binding as orc
+    info = orc.host_info()
+    return {"value": value, "unit": unit, "cores": threads, "kind": "port", "sample": sample,
+            "single_thread": {"value": single_value, "unit": unit, "cores": 1, "sample": single_sample},
+            "all_cores_vs_single": value / single_value if single_value else None,
+            "timing": "steady clock, 1 warm-up, median of 5", **info, **extra}
+
+
+def _cpu_threads():
+    from oracle import binding as orc
+    return int(orc.lib().orc_max_threads())
+
+
 def scene_workload(args):
     from gprt_amd import api, scenes
 
@@ -187,7 +218,7 @@ def scene_workload(args):
     infer_ms = infer_ms_tot / K
     learned_ms = infer_ms + prop_trace_ms
     h1, h2 = hb.read(api.HIT_DTYPE), hb2.read(api.HIT_DTYPE)
-    mismatches = int(np.sum((h1["primID"] != h2["primID"]) | (h1["t"].view(np.uint32) != h2["t"].view(np.uint32))))
+    mismatches = int((h1.view(np.uint32).reshape(-1, 4) != h2.view(np.uint32).reshape(-1, 4)).any(1).sum())
     hit_rate = float((h1["primID"] >= 0).mean())
 
     sw_ms, sw_n = ms_of(per_kernel, "lbvh_sort_pass")
@@ -213,49 +244,59 @@ def scene_workload(args):
                 "whole_inference_tflops": n_rays * mlp_flop_per_ray(classes) / (infer_ms * 1e-3) / 1e12}
     roof_mlp["frac"] = roof_mlp["achieved"] / BF16_PEAK_TFLOPS
 
+    from oracle import binding as orc
+    keys, nodes, aabbs = acc.export()
+    T = _cpu_threads()
+    # Inner nodes visited and triangles tested per ray by the oracle's traversal of this tree,
+    # on a seeded sample of the workload's rays (all cores; the counts fix the traversal's
+    # algorithmic bytes).
+    sel = np.random.default_rng(0).choice(n_rays, args.cpu_rays, replace=False)
+    t0 = time.perf_counter()
+    ch, counters = orc.trace_bvh(rays[sel], verts, idx, nodes, aabbs, threads=T)
+    count_s = time.perf_counter() - t0
+    trace_mism = int((ch.view(np.uint32).reshape(-1, 4) != h1[sel].view(np.uint32).reshape(-1, 4)).any(1).sum())
+    nodes_per_ray = counters[0] / len(sel)
+    tris_per_ray = counters[1] / len(sel)
+    # Algorithmic bytes per ray: 32 B ray + 8 B hit + 32 B per inner node visited + 36 B per
+    # triangle tested. The layout here fetches 64 B per node (both child boxes) and 48 B per
+    # triangle record (the `fetched` figure); HBM traffic is what rocprofv3 measured.
+    trace_bytes = 40 + 32 * nodes_per_ray + 36 * tris_per_ray
+    fetched_bytes = 48 + 64 * nodes_per_ray + 48 * tris_per_ray
+    trace_avg = plain_ms
+    ach = n_rays * trace_bytes / (trace_avg * 1e-3) / 1e9
+    roof_trace = {"kernel": "trace_closest", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["trace_closest"],
+                  "avg_launch_ms": trace_avg, "rays_per_launch": n_rays,
+                  "bytes_per_ray": trace_bytes, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray,
+                  "fetched_bytes_per_ray": fetched_bytes,
+                  "fetched_gbs": n_rays * fetched_bytes / (trace_avg * 1e-3) / 1e9,
+                  "dram_gbs": TRAFFIC_BYTES["trace_closest"] / (trace_avg * 1e-3) / 1e9,
+                  "dram_frac": TRAFFIC_BYTES["trace_closest"] / (trace_avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "l1_hit_rate": TRACE_CACHE_HIT["l1"], "l2_hit_rate": TRACE_CACHE_HIT["l2"],
+                  "counts_from": f"oracle traversal of {len(sel)} sampled rays ({count_s:.2f} s, {T} threads)"}
+    log(f"trace: {nodes_per_ray:.1f} nodes + {tris_per_ray:.2f} tris per ray -> {trace_bytes:.0f} B/ray, "
+        f"frac {roof_trace['frac']:.3f}; {trace_mism} sampled hits differ from the oracle")
     cpu = None
-    roof_trace = None
-    dominant = roof_mlp if out_avg >= plain_ms else None
     if not args.no_cpu:
-        from oracle import binding as orc
-        keys, nodes, aabbs = acc.export()
-        sel = np.random.default_rng(0).choice(n_rays, args.cpu_rays, replace=False)
-        t0 = time.perf_counter()
-        ch, counters = orc.trace_bvh(rays[sel], verts, idx, nodes, aabbs, threads=1)
-        cpu_trace_s = time.perf_counter() - t0
-        mism = int((ch["primID"] != h1["primID"][sel]).sum())
-        nodes_per_ray = counters[0] / len(sel)
-        tris_per_ray = counters[1] / len(sel)
-        trace_bytes = 32 + 16 + 64 * nodes_per_ray + 48 * tris_per_ray
-        m = min(args.cpu_build_tris, N_TRIS)
-        times = []
-        for _ in range(max(1, args.cpu_reps)):
-            t0 = time.perf_counter()
-            orc.lbvh_build(verts[: 3 * m], idx[:m])
-            times.append(time.perf_counter() - t0)
-        cpu_build_s = float(np.median(times))
-        what = "the whole scene" if m == N_TRIS else f"the first {m} triangles of the scene"
-        cpu = {"value": m / cpu_build_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
-               "sample": f"oracle LBVH build of {what}, single thread, median of {len(times)}; oracle BVH "
-                         f"traversal of {len(sel)} sampled rays of the workload, single thread "
-                         f"({len(sel) / cpu_trace_s / 1e6:.4f} Mrays/s)",
-               "trace_mrays_s": len(sel) / cpu_trace_s / 1e6, "host_cpus": os.cpu_count(),
-               "trace_mismatches_vs_gpu": mism}
-        trace_avg = plain_ms
-        ach = n_rays * trace_bytes / (trace_avg * 1e-3) / 1e9
-        # Algorithmic bytes exceed what HBM delivers: most node fetches hit in L2 / MALL, so the HBM
-        # roofline does not bind traversal (measured HBM-side traffic is `traffic`).
-        roof_trace = {"kernel": "trace_closest", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["trace_closest"],
-                      "traffic_gbs": TRAFFIC_BYTES["trace_closest"] / (trace_avg * 1e-3) / 1e9,
-                      "avg_launch_ms": trace_avg,
-                      "bytes_per_ray": trace_bytes, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray,
-                      "l1_hit_rate": TRACE_CACHE_HIT["l1"], "l2_hit_rate": TRACE_CACHE_HIT["l2"]}
-        log(f"cpu: build {cpu['value']:.3f} Mprims/s, trace {cpu['trace_mrays_s']:.4f} Mrays/s; "
-            f"{nodes_per_ray:.1f} nodes + {tris_per_ray:.1f} tris per ray")
-
-    if dominant is None:
-        dominant = roof_trace if roof_trace else roof_build
+        m1 = min(args.cpu_build_tris, N_TRIS)
+        single_build = _median_time(lambda: orc.lbvh_build(verts[: 3 * m1], idx[:m1]))
+        multi_build = _median_time(lambda: orc.par_lbvh_build(verts, idx, threads=T))
+        r1 = rays[sel[: args.cpu_rays_single]]
+        single_trace = _median_time(lambda: orc.trace_bvh(r1, verts, idx, nodes, aabbs, threads=1))
+        rm = rays[sel]
+        multi_trace = _median_time(lambda: orc.trace_bvh(rm, verts, idx, nodes, aabbs, threads=T))
+        cpu = _cpu_baseline(
+            N_TRIS / multi_build / 1e6, "Mprims/s", T,
+            f"oracle OpenMP LBVH build of the whole {N_TRIS}-triangle scene ({T} threads); OpenMP traversal "
+            f"of {len(rm)} sampled workload rays ({len(rm) / multi_trace / 1e6:.3f} Mrays/s)",
+            m1 / single_build / 1e6,
+            f"oracle LBVH build of the first {m1} triangles, 1 thread; traversal of {len(r1)} sampled rays "
+            f"({len(r1) / single_trace / 1e6:.4f} Mrays/s)",
+            trace_mrays_s=len(rm) / multi_trace / 1e6, trace_mrays_s_single=len(r1) / single_trace / 1e6,
+            trace_mismatches_vs_gpu=trace_mism)
+        log(f"cpu: build {cpu['value']:.2f} Mprims/s ({T} threads), {cpu['single_thread']['value']:.3f} "
+            f"(1 thread); trace {cpu['trace_mrays_s']:.3f} / {cpu['trace_mrays_s_single']:.4f} Mrays/s")
+    dominant = roof_trace if plain_ms >= out_avg else roof_mlp
     out = {
         "metric": "LBVH build Mprims/s + closest-hit Mrays/s, plain vs learned-proposal, 1 GPU",
         "value": N_TRIS / (build_ms * 1e3),
@@ -283,6 +324,7 @@ def scene_workload(args):
         # The roofline of the kernel that takes the most time in the step.
         "roofline": dominant,
         "roofline_build": roof_build,
+        "roofline_trace": roof_trace,
         "roofline_sort_pass": roof_sort,
         "roofline_mlp": roof_mlp,
         "cpu_baseline": cpu,
@@ -304,12 +346,14 @@ def batched_workload(args):
     sizes = scenes.batch_sizes(args.meshes, seed=3)
     total = int(sizes.sum())
     t0 = time.time()
-    accs = []
+    accs, vbs, ibs = [], [], []
     for i, n in enumerate(sizes):
         vb, ib = ctx.device_buffer(nbytes=36 * int(n)), ctx.device_buffer(nbytes=12 * int(n))
         one = np.array([n], dtype=np.int64)
         lib.gprtSynthBlobMeshes(ctx.h, one.ctypes.data_as(C.c_void_p), 1, 1000 + i, 0.002, vb.h, ib.h)
         accs.append(api.TriangleAccel.from_buffers(ctx, vb, ib, 3 * int(n), int(n)))
+        vbs.append(vb)
+        ibs.append(ib)
     ctx.sync()
     log(f"{len(accs)} meshes, {total} triangles generated on the device in {time.time() - t0:.1f}s")
     ev = (C.c_void_p * len(accs))(*[a.h for a in accs])
@@ -353,6 +397,29 @@ def batched_workload(args):
     lib.gprtKernelTimerEnable(ctx.h, False)
     graph_ms = g_ms / max(g_n, 1)
     ach = total * LBVH_BYTES_PER_PRIM / (graph_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as orc
+        T = _cpu_threads()
+
+        def host_meshes(sel):
+            v = np.concatenate([vbs[i].read(np.float32) for i in sel])
+            ix = np.concatenate([ibs[i].read(np.uint32) for i in sel])
+            return v, ix, sizes[sel]
+
+        # All cores: every 8th mesh, one oracle build per mesh spread over the threads; one thread:
+        # every 128th mesh, one after the other.
+        vm, im, sm = host_meshes(np.arange(0, len(sizes), 8))
+        multi = _median_time(lambda: orc.lbvh_build_many(vm, im, sm, threads=T))
+        v1, i1, s1 = host_meshes(np.arange(0, len(sizes), 128))
+        single = _median_time(lambda: orc.lbvh_build_many(v1, i1, s1, threads=1))
+        cpu = _cpu_baseline(int(sm.sum()) / multi / 1e6, "Mprims/s", T,
+                            f"oracle LBVH builds of every 8th mesh ({len(sm)} meshes, {int(sm.sum())} triangles), "
+                            f"meshes over {T} threads", int(s1.sum()) / single / 1e6,
+                            f"oracle LBVH builds of every 128th mesh ({len(s1)} meshes, {int(s1.sum())} triangles), "
+                            f"1 thread")
+        del vm, im, v1, i1
+        log(f"cpu: {cpu['value']:.2f} Mprims/s ({T} threads), {cpu['single_thread']['value']:.3f} (1 thread)")
     out = {
         "metric": "batched LBVH build Mprims/s (4,096 uneven meshes, one hipGraph launch), 1 GPU",
         "value": total / (graph_ms * 1e3),
@@ -377,7 +444,7 @@ def batched_workload(args):
                      "traffic": TRAFFIC_BYTES["batched_build_per_prim"] * total,
                      "bytes_per_prim": LBVH_BYTES_PER_PRIM},
         "autotune": tune,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     for a in accs:
@@ -386,65 +453,40 @@ def batched_workload(args):
 
 
 def dynamic_workload(args):
-    """Dynamic scene: 64 blob meshes of 65,536 triangles (mesh i: seed 5000 + i) under one instance
-    accel. Every frame f: all vertices = base + U(-1e-3, 1e-3) (seed 7000 + f), the 64 triangle
-    accels are rebuilt in one graph-captured batch and the instance accel on top of them, then
-    1024^2 pinhole primary rays and 1024^2 bounce rays from their hits (2,097,152 rays) are traced
-    plainly and with the learned proposal (classes = the instance accel's 4,096 top-level leaves).
-    Every 16 frames the proposal is fine-tuned on that frame's plain-traversal labels. Frame times
+    """Dynamic scene (gprt_amd.dynamic.DynamicScene): 64 blob meshes of 65,536 triangles (mesh i:
+    seed 5000 + i) under one instance accel. Every frame f: all vertices = base + U(-1e-3, 1e-3)
+    (seed 7000 + f), the 64 triangle accels are rebuilt in one graph-captured batch and the
+    instance accel on top of them, then 1024^2 pinhole primary rays and 1024^2 bounce rays from
+    their hits (2,097,152 rays) are traced plainly and with the learned proposal (classes = the
+    instance accel's 4,096 top-level leaf groups). Every 16 frames the proposal is fine-tuned on
+    that frame's rays with the plain traversal's labels (all 2,097,152 by default). Frame times
     come from the library's per-kernel HIP events: plain frame = perturb + builds + bounce
-    generation + plain traces; learned frame = the same with inference + proposal-first traces
-    (+ the fine-tune in the frames that have one)."""
-    from gprt_amd import api, scenes
+    generation + plain traces; learned frame = the same with inference + proposal-first traces;
+    the fine-tune is timed on its own (wall clock)."""
+    from gprt_amd import api
+    from gprt_amd.dynamic import DynamicScene
 
     lib = api.load()
     ctx = api.Context(0)
-    meshes, per = 64, 65536
-    sizes = np.full(meshes, per, dtype=np.int64)
-    base, ibuf = api.synth_blob_meshes(ctx, sizes, 5000, 0.002)
-    cur = ctx.device_buffer(nbytes=base.nbytes)
-    floats = base.nbytes // 4
-    lib.gprtSynthPerturb(ctx.h, base.h, cur.h, floats, 7000, 1e-3)
-    blas = api.mesh_accels(ctx, cur, ibuf, sizes)
-    api.build_batch(ctx, blas)
-    tlas = api.InstanceAccel(ctx, api.instance_records(blas))
-    tlas.build()
-    side = 1024
-    prim = scenes.pinhole_rays(side, side)
-    n = len(prim)
-    pb = ctx.device_buffer(prim)
-    bb = ctx.device_buffer(nbytes=32 * n)
-    hp, hb, hp2, hb2 = (ctx.device_buffer(nbytes=16 * n) for _ in range(4))
-    ip, ib_, lp, lb = (ctx.device_buffer(nbytes=4 * n) for _ in range(4))
-    prop = api.Proposal(ctx, max_batch=args.batch, seed=1)
-    classes = prop.bind(tlas)
-    leaves_per_class = int(lib.gprtAccelGetPrimCount(tlas.h)) // classes
-    tune_n = min(args.tune_rays, n)
-    log(f"dynamic scene: {meshes} x {per} triangles, {2 * n} rays/frame, {classes} proposal classes")
+    ds = DynamicScene(ctx, max_batch=args.batch)
+    n = ds.n
+    log(f"dynamic scene: {ds.meshes} x {ds.per} triangles, {ds.rays_per_frame} rays/frame, "
+        f"{ds.classes} proposal classes")
+    timing = {"on": False, "tune_s": 0.0}
 
-    def frame(f, timed_frames):
-        lib.gprtSynthPerturb(ctx.h, base.h, cur.h, floats, 7000 + f, 1e-3)
-        api.build_batch(ctx, blas)
-        tlas.build()
-        api.trace_detail(ctx, tlas, pb, hp, n, ip, lp)
-        api.bounce_rays(ctx, tlas, pb, hp, ip, n, bb, n, seed=7000 + f)
-        api.trace_detail(ctx, tlas, bb, hb, n, ib_, lb)
-        tuned = None
-        if f % 16 == 0:
-            # Fine-tune on this frame's plain labels (a sample of the primary and bounce rays).
+    def on_tune(before):
+        if before:
             lib.gprtKernelTimerEnable(ctx.h, False)
             ctx.sync()
-            k = tune_n // 2
-            l1 = prop.train_labels(pb, lp, k, epochs=args.tune_epochs, lr=args.lr)
-            l2 = prop.train_labels(bb, lb, k, epochs=args.tune_epochs, lr=args.lr)
-            tuned = (l1, l2)
-            lib.gprtKernelTimerEnable(ctx.h, timed_frames)
-        api.trace_detail(ctx, tlas, pb, hp2, n, proposal=prop)
-        api.trace_detail(ctx, tlas, bb, hb2, n, proposal=prop)
-        return tuned
+            timing["t0"] = time.perf_counter()
+        else:
+            ctx.sync()
+            timing["tune_s"] = time.perf_counter() - timing["t0"]
+            lib.gprtKernelTimerEnable(ctx.h, timing["on"])
 
+    tune_rays = args.tune_rays if args.tune_rays > 0 else None
     for f in range(args.warmup):
-        frame(f, False)
+        ds.frame(f, tune_rays=tune_rays, epochs=args.tune_epochs, lr=args.lr, on_tune=on_tune)
     ctx.sync()
     frames = args.frames
     plain_ms, learned_ms, build_ms, tplain, tlearn, tune_ms = [], [], [], [], [], []
@@ -454,11 +496,11 @@ def dynamic_workload(args):
     for f in range(frames):
         lib.gprtKernelTimerReset(ctx.h)
         lib.gprtKernelTimerEnable(ctx.h, True)
-        ts = time.perf_counter()
-        tuned = frame(f, True)
+        timing["on"] = True
+        tuned = ds.frame(f, tune_rays=tune_rays, epochs=args.tune_epochs, lr=args.lr, on_tune=on_tune)
         ctx.sync()
-        tune_wall = 0.0
         lib.gprtKernelTimerEnable(ctx.h, False)
+        timing["on"] = False
         pk = kernel_report(lib, ctx)
         common = ms_of(pk, "synth_")[0] + ms_of(pk, "lbvh_")[0] + ms_of(pk, "tlas_")[0] + ms_of(pk, "fill_u32")[0] \
             + ms_of(pk, "scan_")[0]
@@ -470,30 +512,20 @@ def dynamic_workload(args):
         tplain.append(tp)
         tlearn.append(tl)
         if tuned is not None:
-            tune_ms.append((time.perf_counter() - ts) * 1e3)
+            tune_ms.append(timing["tune_s"] * 1e3)
         if f % 16 == 0 or f == frames - 1:
-            a1, a2 = hp.read(api.HIT_DTYPE, n), hp2.read(api.HIT_DTYPE, n)
-            b1, b2 = hb.read(api.HIT_DTYPE, n), hb2.read(api.HIT_DTYPE, n)
-            mism += int((a1.view(np.uint32).reshape(n, 4) != a2.view(np.uint32).reshape(n, 4)).any(1).sum())
-            mism += int((b1.view(np.uint32).reshape(n, 4) != b2.view(np.uint32).reshape(n, 4)).any(1).sum())
-            # Proposal hit rate: the proposed top-level leaf holds the closest hit.
-            cls = ctx.device_buffer(nbytes=4 * n)
-            lib.gprtProposalInfer(ctx.h, prop.h, pb.h, n, cls.h)
-            lab = lp.read(np.int32, n)
-            c = cls.read(np.int32, n)
-            m = lab >= 0
-            hit_rates.append(float((c[m] == lab[m] // leaves_per_class).mean()) if m.any() else 0.0)
-            cls.destroy()
+            mism += ds.mismatches()
+            hit_rates.append(ds.proposal_hit_rate())
     wall = time.perf_counter() - t0
     pct = lambda a, q: float(np.percentile(np.asarray(a), q))
-    rays = 2 * n
+    rays = ds.rays_per_frame
     # Node fetches per ray of the last frame's plain traces (untimed), for the traversal's
     # algorithmic bytes: 32 B ray + 16 B hit + 64 B per inner node (top and instance levels);
     # leaf records are not counted, so this is a lower bound.
     vis = ctx.device_buffer(nbytes=4 * n)
-    api.trace_detail(ctx, tlas, pb, hp, n, visits_buf=vis)
+    api.trace_detail(ctx, ds.tlas, ds.pb, ds.hp, n, visits_buf=vis)
     v1 = float(vis.read(np.uint32).mean())
-    api.trace_detail(ctx, tlas, bb, hb, n, visits_buf=vis)
+    api.trace_detail(ctx, ds.tlas, ds.bb, ds.hb, n, visits_buf=vis)
     v2 = float(vis.read(np.uint32).mean())
     vis.destroy()
     visits = (v1 + v2) / 2
@@ -501,6 +533,36 @@ def dynamic_workload(args):
     roof = _roof_hbm("trace_closest_instances", n * (48 + 64 * visits), launch_ms, bytes_per_ray=48 + 64 * visits,
                      nodes_per_ray=visits, note="issue- and L2-bound like the scene traversal; leaf records "
                                                "not counted")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as orc
+        T = _cpu_threads()
+        verts = ds.cur.read(np.float32)
+        idx = ds.ibuf.read(np.uint32)
+        sizes = ds.sizes
+        # The last frame's geometry rebuilt on the CPU (one oracle LBVH per mesh) and a sample of
+        # its rays traced through the 64 meshes (identity instances); the frame-equivalent rate is
+        # the frame's rays over (rebuild + their traversal at the sample's rate).
+        multi_build = _median_time(lambda: orc.lbvh_build_many(verts, idx, sizes, threads=T))
+        _, nodes, aabbs = orc.lbvh_build_many(verts, idx, sizes, threads=T)
+        sample = np.concatenate([ds.pb.read(api.RAY_DTYPE, n)[::64], ds.bb.read(api.RAY_DTYPE, n)[::64]])
+        multi_trace = _median_time(lambda: orc.trace_bvh_multi(sample, verts, idx, sizes, nodes, aabbs, threads=T))
+        k1 = 8
+        single_build = _median_time(lambda: orc.lbvh_build_many(verts[: 9 * k1 * ds.per], idx[: 3 * k1 * ds.per],
+                                                                 sizes[:k1], threads=1), reps=3)
+        s1 = sample[:: 16]
+        single_trace = _median_time(lambda: orc.trace_bvh_multi(s1, verts, idx, sizes, nodes, aabbs, threads=1),
+                                    reps=3)
+        frame_multi = multi_build + multi_trace / len(sample) * rays
+        frame_single = single_build * ds.meshes / k1 + single_trace / len(s1) * rays
+        cpu = _cpu_baseline(rays / frame_multi / 1e6, "Mrays/s (whole frame)", T,
+                            f"oracle rebuild of the 64 meshes (meshes over {T} threads) + OpenMP traversal of "
+                            f"{len(sample)} of the frame's rays through them, scaled to {rays} rays",
+                            rays / frame_single / 1e6,
+                            f"oracle rebuild of 8 of the 64 meshes and traversal of {len(s1)} rays, 1 thread, "
+                            f"scaled to the whole frame (median of 3)",
+                            frame_ms=frame_multi * 1e3, frame_ms_single=frame_single * 1e3)
+        log(f"cpu: frame {cpu['frame_ms']:.0f} ms ({T} threads), {cpu['frame_ms_single']:.0f} ms (1 thread)")
     out = {
         "metric": "dynamic scene frame time (64 x 65,536 tris rebuilt per frame + 2,097,152 rays), plain vs "
                   "learned proposal, 1 GPU",
@@ -510,31 +572,25 @@ def dynamic_workload(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (geometry, traversal); "
         "bf16 MFMA with f32 accumulate (proposal MLP)",
         "data": "synthetic (device-generated blob meshes, seeded per-frame perturbation)",
-        "config": {"workload": f"{meshes} meshes x {per} triangles, {rays} rays/frame (1024^2 primary + 1024^2 "
-                               f"bounce), {frames} frames, fine-tune every 16 frames on {tune_n} labelled rays",
-                   "meshes": meshes, "triangles_per_mesh": per, "rays_per_frame": rays, "frames": frames,
-                   "proposal_classes": classes},
+        "config": {"workload": f"{ds.meshes} meshes x {ds.per} triangles, {rays} rays/frame (1024^2 primary + "
+                               f"1024^2 bounce), {frames} frames, fine-tune every 16 frames on "
+                               f"{tune_rays or rays} labelled rays x {args.tune_epochs} epochs",
+                   "meshes": ds.meshes, "triangles_per_mesh": ds.per, "rays_per_frame": rays, "frames": frames,
+                   "proposal_classes": ds.classes},
         "components": {
             "frame_ms_plain_p50": pct(plain_ms, 50), "frame_ms_plain_p99": pct(plain_ms, 99),
             "frame_ms_learned_p50": pct(learned_ms, 50), "frame_ms_learned_p99": pct(learned_ms, 99),
-            "build_ms_p50": pct(build_ms, 50), "build_mprims_s_p50": meshes * per / (pct(build_ms, 50) * 1e3),
+            "build_ms_p50": pct(build_ms, 50), "build_mprims_s_p50": ds.meshes * ds.per / (pct(build_ms, 50) * 1e3),
             "trace_ms_plain_p50": pct(tplain, 50), "trace_mrays_s_plain_p50": rays / (pct(tplain, 50) * 1e3),
             "trace_ms_learned_p50": pct(tlearn, 50), "trace_mrays_s_learned_p50": rays / (pct(tlearn, 50) * 1e3),
             "learned_vs_plain_trace": pct(tplain, 50) / pct(tlearn, 50),
-            "fine_tune_frame_wall_ms_mean": float(np.mean(tune_ms)) if tune_ms else None,
+            "fine_tune_wall_ms_mean": float(np.mean(tune_ms)) if tune_ms else None,
             "proposal_hit_rate_by_check": hit_rates, "hit_mismatches_learned_vs_plain": mism},
         "roofline": roof,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    for b in (pb, bb, hp, hb, hp2, hb2, ip, ib_, lp, lb, cur):
-        b.destroy()
-    prop.destroy()
-    tlas.destroy()
-    for a in blas:
-        a.destroy()
-    base.destroy()
-    ibuf.destroy()
+    ds.destroy()
     ctx.close()
 
 
@@ -630,13 +686,15 @@ def refit_workload(args):
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc
+        T = _cpu_threads()
         moved = acc.vbuf.read(np.float32).reshape(-1, 3)
         _, nodes, _ = acc.export()
-        t0 = time.perf_counter()
-        orc.refit(moved, idx, nodes)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": N_TRIS / cpu_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
-               "sample": f"oracle refit of the whole {N_TRIS}-triangle scene over the same topology, single thread"}
+        single = _median_time(lambda: orc.refit(moved, idx, nodes))
+        multi = _median_time(lambda: orc.par_refit(moved, idx, nodes, threads=T))
+        cpu = _cpu_baseline(N_TRIS / multi / 1e6, "Mprims/s", T,
+                            f"oracle atomic-flag refit of the whole {N_TRIS}-triangle scene over the same topology, "
+                            f"{T} threads", N_TRIS / single / 1e6,
+                            f"oracle refit (every leaf into every ancestor) of the whole scene, 1 thread")
     out = {
         "metric": "refit-only update Mprims/s (gprtAccelUpdate), 1 GPU", "value": N_TRIS / (refit_ms * 1e3),
         "unit": "Mprims/s", "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": wall * 1e3 / K,
@@ -905,15 +963,23 @@ def sort_workload(args):
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc  # the CPU restatement, timed as the baseline only
-        m = 1 << 24
-        ts = []
-        for _ in range(3):
-            t1 = time.perf_counter()
-            orc.stable_sort(keys[:m], vals[:m])
-            ts.append(time.perf_counter() - t1)
-        cpu = {"value": m / float(np.median(ts)) / 1e6, "unit": "Mkeys/s", "cores": 1, "kind": "port",
-               "sample": f"oracle std::stable_sort of the first {m} key/payload pairs (u32 + u32), median of 3, "
-                         f"single thread"}
+        T = _cpu_threads()
+        L = orc.lib()
+        kw, vw = np.empty_like(keys), np.empty_like(vals)
+
+        def restore(m):
+            np.copyto(kw[:m], keys[:m])
+            np.copyto(vw[:m], vals[:m])
+
+        m1 = 1 << 22
+        single = _median_time(lambda: L.orc_stable_sort(orc._p(kw), orc._p(vw), m1, 32, 32),
+                              setup=lambda: restore(m1))
+        multi = _median_time(lambda: L.orc_par_radix_sort(orc._p(kw), orc._p(vw), n, 32, 32, T),
+                             setup=lambda: restore(n))
+        cpu = _cpu_baseline(n / multi / 1e6, "Mkeys/s", T,
+                            f"oracle OpenMP LSD radix sort (8-bit digits) of all {n} key/payload pairs, {T} threads",
+                            m1 / single / 1e6,
+                            f"oracle std::stable_sort of the first {m1} key/payload pairs (u32 + u32), 1 thread")
     out = {
         "metric": "radix sort Mkeys/s (2^26 u32 keys + u32 payload, gprtBufferSortKV), 1 GPU",
         "value": n / (sort_ms * 1e3), "unit": "Mkeys/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -973,13 +1039,14 @@ def scan_workload(args):
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc  # the CPU restatement, timed as the baseline only
-        ts = []
-        for _ in range(3):
-            t1 = time.perf_counter()
-            orc.exclusive_sum(x)
-            ts.append(time.perf_counter() - t1)
-        cpu = {"value": n / float(np.median(ts)) / 1e9, "unit": "Gelems/s", "cores": 1, "kind": "port",
-               "sample": f"oracle exclusive scan of all {n} elements, median of 3, single thread"}
+        T = _cpu_threads()
+        L = orc.lib()
+        out = np.empty_like(x)
+        single = _median_time(lambda: L.orc_exclusive_sum(orc._p(x), orc._p(out), n))
+        multi = _median_time(lambda: L.orc_par_exclusive_sum(orc._p(x), orc._p(out), n, T))
+        cpu = _cpu_baseline(n / multi / 1e9, "Gelems/s", T,
+                            f"oracle two-pass blocked exclusive scan of all {n} elements, {T} threads",
+                            n / single / 1e9, f"oracle std::exclusive_scan of all {n} elements, 1 thread")
     out = {
         "metric": "exclusive scan Gelems/s (2^28 u32, gprtBufferExclusiveSum), 1 GPU",
         "value": n / (scan_ms * 1e-3) / 1e9, "unit": "Gelems/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -1004,11 +1071,11 @@ def main():
     ap.add_argument("--workload", default="scene",
                     choices=["scene", "batched", "dynamic", "copy", "refit", "boxes", "compute", "sort", "scan"])
     ap.add_argument("--frames", type=int, default=256, help="dynamic: timed frames")
-    ap.add_argument("--tune-rays", type=int, default=262144, help="dynamic: labelled rays per fine-tune")
+    ap.add_argument("--tune-rays", type=int, default=0, help="dynamic: labelled rays per fine-tune (0: the whole frame)")
     ap.add_argument("--tune-epochs", type=int, default=2, help="dynamic: epochs per fine-tune")
-    ap.add_argument("--cpu-rays", type=int, default=262144, help="rays of the CPU traversal sample")
-    ap.add_argument("--cpu-build-tris", type=int, default=N_TRIS, help="primitives of the CPU build sample")
-    ap.add_argument("--cpu-reps", type=int, default=3, help="CPU builds timed (median)")
+    ap.add_argument("--cpu-rays", type=int, default=262144, help="rays of the all-cores CPU traversal sample")
+    ap.add_argument("--cpu-rays-single", type=int, default=32768, help="rays of the 1-thread CPU traversal sample")
+    ap.add_argument("--cpu-build-tris", type=int, default=1 << 20, help="triangles of the 1-thread CPU build sample")
     ap.add_argument("--train-rays", type=int, default=16 << 20, help="proposal training rays (16,777,216: C4)")
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)

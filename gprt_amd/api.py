@@ -113,6 +113,7 @@ _SIGS = {
     "gprtAccelGetSize": (C.c_size_t, [_P, C.c_int]),
     "gprtAccelGetDeviceAddress": (_P, [_P, C.c_int]),
     "gprtAccelGetPrimCount": (C.c_uint32, [_P]),
+    "gprtAccelGetBounds": (None, [_P, _P, _P]),
     "gprtAccelTraceClosestHit": (None, [_P, _P, _P, _P, C.c_uint32]),
     "gprtAccelExportLBVH": (None, [_P, _P, _P, _P, _P]),
     "gprtProposalCreate": (_P, [_P, C.c_uint32, C.c_uint64]),
@@ -308,6 +309,10 @@ class TriangleAccel:
     def trace(self, rays_buf: Buffer, hits_buf: Buffer, count: int):
         self.lib.gprtAccelTraceClosestHit(self.ctx.h, self.h, rays_buf.h, hits_buf.h, count)
 
+    def bounds(self) -> np.ndarray:
+        """gprtAccelGetBounds: the root box {min.xyz, max.xyz}."""
+        return accel_bounds(self.ctx, self)
+
     def serialize(self) -> bytes:
         """gprtAccelSerialize: the built accel as bytes."""
         n = int(self.lib.gprtAccelSerialize(self.ctx.h, self.h, None, 0))
@@ -385,6 +390,13 @@ def mesh_accels(ctx: Context, vbuf: Buffer, ibuf: Buffer, sizes):
 
 def _h(b):
     return b.h if b is not None else None
+
+
+def accel_bounds(ctx: Context, accel) -> np.ndarray:
+    """gprtAccelGetBounds of any built accel: float32[6] {min.xyz, max.xyz}."""
+    out = np.zeros(6, dtype=np.float32)
+    ctx.lib.gprtAccelGetBounds(ctx.h, accel.h, out.ctypes.data_as(C.c_void_p))
+    return out
 
 
 class InstanceAccel:

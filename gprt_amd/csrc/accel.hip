@@ -186,6 +186,12 @@ static bool validate_geom(Geom *g, const char *fn) {
       fail("%s: %u boxes exceeds the 2^30 limit", fn, g->icount);
       return false;
     }
+    // The buffer may have been resized since gprtAABBsSetPositions checked it.
+    if ((size_t)g->voffset + (size_t)(g->icount - 1) * g->vstride + 24 > g->vbuf->size) {
+      fail("%s: %u boxes (stride %u, offset %u) exceed the box buffer (%zu B)", fn, g->icount, g->vstride,
+           g->voffset, g->vbuf->size);
+      return false;
+    }
     return true;
   }
   if (!g->vbuf || !g->ibuf) {
@@ -200,8 +206,17 @@ static bool validate_geom(Geom *g, const char *fn) {
     fail("%s: %u triangles exceeds the 2^30 limit", fn, g->icount);
     return false;
   }
-  if (g->vbuf->kind == BufKind::Host || g->ibuf->kind == BufKind::Host) {
-    // Host buffers are device-visible (mapped pinned memory); allowed but slow.
+  // The buffers may have been resized since gprtTrianglesSetVertices / SetIndices checked them;
+  // the kernels read every index triple and clamp vertex indices to vcount.
+  if ((size_t)g->voffset + (size_t)(g->vcount - 1) * g->vstride + 12 > g->vbuf->size) {
+    fail("%s: %u vertices (stride %u, offset %u) exceed the vertex buffer (%zu B)", fn, g->vcount, g->vstride,
+         g->voffset, g->vbuf->size);
+    return false;
+  }
+  if ((size_t)g->ioffset + (size_t)(g->icount - 1) * g->istride + 12 > g->ibuf->size) {
+    fail("%s: %u triangles (stride %u, offset %u) exceed the index buffer (%zu B)", fn, g->icount, g->istride,
+         g->ioffset, g->ibuf->size);
+    return false;
   }
   return true;
 }
@@ -378,6 +393,30 @@ GPRT_API bool gprtAccelDeserialize(GPRTContext _ctx, GPRTAccel _a, const void *i
   return true;
 }
 
+// The accels of a batch: each non-null, a triangle or box accel, named once (a batch store counts
+// one reference per accel), over valid geometry.
+static bool collect_batch(GPRTAccel *_accels, uint32_t count, std::vector<Accel *> &accels, const char *fn) {
+  accels.resize(count);
+  std::unordered_set<Accel *> seen;
+  for (uint32_t i = 0; i < count; ++i) {
+    accels[i] = (Accel *)_accels[i];
+    if (!accels[i]) {
+      fail("%s: accel %u is null", fn, i);
+      return false;
+    }
+    if (!seen.insert(accels[i]).second) {
+      fail("%s: accel %u appears twice", fn, i);
+      return false;
+    }
+    if (accels[i]->kind == AccelKind::Instances) {
+      fail("%s: accel %u is an instance accel", fn, i);
+      return false;
+    }
+    if (!validate_geom(accels[i]->geom, fn)) return false;
+  }
+  return true;
+}
+
 GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t count, GPRTBuildParams params) {
   const StageRange range("gprtAccelBuildBatch");
   Context *ctx = (Context *)_ctx;
@@ -386,24 +425,8 @@ GPRT_API void gprtAccelBuildBatch(GPRTContext _ctx, GPRTAccel *_accels, uint32_t
     return;
   }
   if (count == 0) return;
-  std::vector<Accel *> accels(count);
-  std::unordered_set<Accel *> seen;
-  for (uint32_t i = 0; i < count; ++i) {
-    accels[i] = (Accel *)_accels[i];
-    if (!accels[i]) {
-      fail("gprtAccelBuildBatch: accel %u is null", i);
-      return;
-    }
-    if (!seen.insert(accels[i]).second) {
-      fail("gprtAccelBuildBatch: accel %u appears twice", i);
-      return;
-    }
-    if (accels[i]->kind == AccelKind::Instances) {
-      fail("gprtAccelBuildBatch: accel %u is an instance accel", i);
-      return;
-    }
-    if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatch")) return;
-  }
+  std::vector<Accel *> accels;
+  if (!collect_batch(_accels, count, accels, "gprtAccelBuildBatch")) return;
   if (params.buildMode == GPRT_BUILD_MODE_UNINITIALIZED) params.buildMode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
   for (auto *a : accels) a->mode = params.buildMode;
   lbvh_build_batch(ctx, ctx->stream, accels.data(), count, true);
@@ -459,6 +482,27 @@ GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel _a, int) {
 GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel _a) {
   Accel *a = (Accel *)_a;
   return a ? a->n : 0;
+}
+
+// The root box lives in the accel's 16-float header (floats 8..13), written by the build's
+// hierarchy pass (or by the top level of an instance accel's build).
+GPRT_API void gprtAccelGetBounds(GPRTContext _ctx, GPRTAccel _a, float *bounds) {
+  Context *ctx = (Context *)_ctx;
+  Accel *a = (Accel *)_a;
+  if (!ctx || !a || !bounds) {
+    fail("gprtAccelGetBounds: null argument");
+    return;
+  }
+  if (!a->built || !a->header) {
+    fail("gprtAccelGetBounds: accel has not been built");
+    return;
+  }
+  float b[6];
+  if (!hip_ok(hipMemcpyAsync(b, a->header + 8, sizeof(b), hipMemcpyDeviceToHost, ctx->stream),
+              "hipMemcpyAsync(root box)"))
+    return;
+  if (!stream_wait(ctx, ctx->stream, "gprtAccelGetBounds")) return;
+  std::memcpy(bounds, b, sizeof(b));
 }
 
 GPRT_API void gprtAccelTraceClosestHit(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _rays, GPRTBuffer _hits,
@@ -529,15 +573,11 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
     return 0;
   }
   if (reps == 0) reps = 3;
-  std::vector<Accel *> accels(count);
-  for (uint32_t i = 0; i < count; ++i) {
-    accels[i] = (Accel *)_accels[i];
-    if (!accels[i] || accels[i]->kind == AccelKind::Instances) {
-      fail("gprtAccelBuildBatchAutotune: accel %u is null or an instance accel", i);
-      return 0;
-    }
-    if (!validate_geom(accels[i]->geom, "gprtAccelBuildBatchAutotune")) return 0;
-  }
+  std::vector<Accel *> accels;
+  if (!collect_batch(_accels, count, accels, "gprtAccelBuildBatchAutotune")) return 0;
+  // An accel keeps the build mode of its last build (default parameters if it has none).
+  for (auto *a : accels)
+    if (a->mode == GPRT_BUILD_MODE_UNINITIALIZED) a->mode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
   const int bits[3] = {4, 6, 8}, tiles[3] = {2048, 4096, 8192}, waves[3] = {0, 8, 16};
   const bool timer = ctx->timer.enabled;
   ctx->timer.enabled = false;

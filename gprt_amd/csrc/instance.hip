@@ -69,6 +69,7 @@ struct Tlas {
   std::vector<uint32_t> hostFirst;
   std::vector<unsigned char> hostRecords;
   uint64_t builtEpoch = ~0ull;  // the context's descriptor epoch when the table was last built
+  uint64_t builtContentEpoch = ~0ull;  // the context's triangle-accel content epoch at that build
   void *block = nullptr;
   size_t blockBytes = 0;
   TlasInst *insts = nullptr;
@@ -732,6 +733,7 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
   a->tris = top->tris;
   a->header = top->header;
   t->builtEpoch = ctx->blasEpoch;
+  t->builtContentEpoch = ctx->blasContentEpoch;
   return true;
 }
 
@@ -743,6 +745,10 @@ bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *h
   if (!work) return false;
   const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
   fill_u32(ctx, s, work, 0u, 1);
+  // The proposal step walks the instances' own triangle accels, the top-level walk the copies
+  // taken at this accel's build: once any triangle accel has changed since then, the proposal is
+  // skipped so that hits stay those of the plain trace.
+  if (t->builtContentEpoch != ctx->blasContentEpoch) proposal = nullptr;
   const uint32_t nc = proposal ? std::min(numClasses, t->classes) : 0u;
   // The unchecked kernel trusts the instance table: no triangle accel of this context was
   // destroyed or moved since the instance accel was built.

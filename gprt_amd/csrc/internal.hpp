@@ -85,6 +85,10 @@ struct Context {
   std::vector<uint32_t> blasFree;
   uint32_t blasNext = 0;
   uint64_t blasEpoch = 0;  // bumped whenever a descriptor is retired or rewritten
+  // Bumped whenever a triangle accel's nodes or leaf records change (build, refit, load): an
+  // instance accel traces copies taken at its own build, so a proposal step that walks a
+  // triangle accel's live arrays is valid only while this is unchanged since that build.
+  uint64_t blasContentEpoch = 0;
   std::vector<struct Accel *> slotAccel;  // host view: pool slot -> live triangle accel
 };
 

@@ -1431,6 +1431,24 @@ bool store_layout(BVHStore *st, const std::vector<MeshDesc> &desc, const std::ve
   return true;
 }
 
+// The geometry fields of a mesh descriptor (where the primitives are read from).
+void geom_fields(const Accel *acc, MeshDesc &d) {
+  const Geom *g = acc->geom;
+  d.verts = (const char *)g->vbuf->d + g->voffset;
+  d.vstride = g->vstride;
+  d.nverts = g->vcount;
+  d.ntri = g->icount;
+  if (acc->kind == AccelKind::AABBs) {
+    d.boxes = 1u;  // {lo.xyz, hi.xyz} per primitive at verts + i * vstride
+    d.idx = nullptr;
+    d.istride = 0;
+  } else {
+    d.boxes = 0u;
+    d.idx = (const char *)g->ibuf->d + g->ioffset;
+    d.istride = g->istride;
+  }
+}
+
 void set_views(Accel *a) {
   BVHStore *st = a->store;
   const MeshDesc &d = st->desc[a->mesh];
@@ -1471,16 +1489,7 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
     Geom *g = accels[m]->geom;
     MeshDesc &d = desc[m];
     std::memset(&d, 0, sizeof(d));
-    d.verts = (const char *)g->vbuf->d + g->voffset;
-    d.vstride = g->vstride;
-    d.nverts = g->vcount;
-    d.ntri = g->icount;
-    if (accels[m]->kind == AccelKind::AABBs) {
-      d.boxes = 1u;  // {lo.xyz, hi.xyz} per primitive at verts + i * vstride
-    } else {
-      d.idx = (const char *)g->ibuf->d + g->ioffset;
-      d.istride = g->istride;
-    }
+    geom_fields(accels[m], d);
     d.primOffset = (uint32_t)prims;
     d.innerOffset = (uint32_t)(prims - m);
     d.tileOffset = (uint32_t)tiles;
@@ -1528,6 +1537,7 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
   for (uint32_t m = 0; m < count; ++m) {
     set_views(accels[m]);
     if (!blas_desc_refresh(ctx, s, accels[m])) return false;
+    if (accels[m]->slot != 0xFFFFFFFFu) ++ctx->blasContentEpoch;
   }
   if (layoutOnly) return true;  // the caller fills the arrays (a deserialised accel)
 
@@ -1573,8 +1583,21 @@ bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t coun
 
 bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   BVHStore *st = acc->store;
-  const MeshDesc &d = st->desc[acc->mesh];
+  MeshDesc &d = st->desc[acc->mesh];
+  // The geometry may sit in another buffer, or the same buffer at a new address (a resize),
+  // since the build: the refit reads it where it is now.
+  MeshDesc now = d;
+  geom_fields(acc, now);
+  if (std::memcmp(&now, &d, sizeof(d)) != 0) {
+    d = now;
+    if (!hip_ok(hipMemcpyAsync(st->dDesc + acc->mesh, &st->desc[acc->mesh], sizeof(MeshDesc), hipMemcpyHostToDevice,
+                               s),
+                "hipMemcpyAsync(mesh descriptor)"))
+      return false;
+    if (!blas_desc_refresh(ctx, s, acc)) return false;
+  }
   const BuildArgs a = args_of(st);
+  if (acc->slot != 0xFFFFFFFFu) ++ctx->blasContentEpoch;
   launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u, false);
   return true;
 }

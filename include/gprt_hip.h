@@ -422,6 +422,11 @@ GPRT_API void gprtAccelTraceClosestHitDetail(GPRTContext context, GPRTAccel acce
 GPRT_API void gprtAccelExportLBVH(GPRTContext context, GPRTAccel accel, GPRTBuffer nodes, GPRTBuffer aabbs,
                                   GPRTBuffer keys);
 GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel accel);
+/* Root box {minX, minY, minZ, maxX, maxY, maxZ} of a built accel (for an instance accel, of its
+ * top level in world space): the union of every primitive's box, as the build computed it. Waits
+ * for the context's stream. The reference exposes no bounds query; this lets a caller check or
+ * place a built accel without exporting its tree. */
+GPRT_API void gprtAccelGetBounds(GPRTContext context, GPRTAccel accel, float *bounds);
 
 /* Per-kernel device timing. While enabled, every kernel the library launches is bracketed by
  * HIP events on the stream it runs on; gprtKernelTimerQuery returns the summed milliseconds

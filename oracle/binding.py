@@ -49,6 +49,14 @@ def lib() -> C.CDLL:
             "orc_plane_t_fp64": (C.c_double, [P, P, P, P]),
             "orc_trace_bruteforce": (None, [P, sz, P, P, sz, P, C.c_int]),
             "orc_trace_bvh": (None, [P, sz, P, P, P, P, sz, P, P, C.c_int]),
+            "orc_trace_bvh_ray": (None, [P, P, P, P, P, sz, P]),
+            "orc_num_procs": (C.c_int, []),
+            "orc_par_radix_sort": (None, [P, P, sz, C.c_int, C.c_int, C.c_int]),
+            "orc_par_exclusive_sum": (C.c_uint32, [P, P, sz, C.c_int]),
+            "orc_par_lbvh_build": (None, [P, P, sz, P, P, P, C.c_int]),
+            "orc_par_refit": (None, [P, P, P, sz, P, C.c_int]),
+            "orc_par_lbvh_build_many": (None, [P, P, P, sz, P, P, P, C.c_int]),
+            "orc_trace_bvh_multi": (None, [P, sz, P, P, P, sz, P, P, P, P, C.c_int]),
             "orc_max_threads": (C.c_int, []),
             "orc_box_bounds": (None, [P, sz, P]),
             "orc_box_keys": (None, [P, sz, P, P]),
@@ -246,3 +254,87 @@ def trace_bvh(rays, verts, idx, nodes, aabbs, threads: int = 1):
     lib().orc_trace_bvh(_p(r), len(r), _p(verts), _p(idx), _p(nodes), _p(aabbs), n, _p(hits), _p(counters),
                         threads)
     return hits, counters
+
+
+# ------------------------------------------------------------------------------------------------
+# All-cores forms (baseline_omp.cpp): the multi-threaded CPU baseline. threads=0: all of
+# omp_get_max_threads().
+# ------------------------------------------------------------------------------------------------
+def par_radix_sort(keys: np.ndarray, vals: np.ndarray | None = None, threads: int = 0):
+    k = np.ascontiguousarray(keys).copy()
+    v = None if vals is None else np.ascontiguousarray(vals).copy()
+    lib().orc_par_radix_sort(_p(k), None if v is None else _p(v), len(k), k.itemsize * 8,
+                             0 if v is None else v.itemsize * 8, threads)
+    return k, v
+
+
+def par_exclusive_sum(x: np.ndarray, threads: int = 0):
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.empty_like(x)
+    total = lib().orc_par_exclusive_sum(_p(x), _p(out), len(x), threads)
+    return out, int(total)
+
+
+def par_lbvh_build(verts, idx, threads: int = 0):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    n = len(idx)
+    keys = np.zeros(n, dtype=np.uint64)
+    nodes = np.zeros((2 * n - 1, 4), dtype=np.int32)
+    aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
+    lib().orc_par_lbvh_build(_p(verts), _p(idx), n, _p(keys), _p(nodes), _p(aabbs), threads)
+    return keys, nodes, aabbs
+
+
+def par_refit(verts, idx, nodes, threads: int = 0):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    n = (len(nodes) + 1) // 2
+    aabbs = np.zeros((len(nodes), 6), dtype=np.float32)
+    lib().orc_par_refit(_p(verts), _p(idx), _p(nodes), n, _p(aabbs), threads)
+    return aabbs
+
+
+def lbvh_build_many(verts, idx, sizes, threads: int = 0):
+    """Concatenated meshes (3 vertices per triangle, indices local to each mesh): one oracle build
+    per mesh, meshes spread over the threads. Outputs are concatenated (mesh m's nodes at
+    2 * first - m)."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+    total, m = int(sizes.sum()), len(sizes)
+    keys = np.zeros(total, dtype=np.uint64)
+    nodes = np.zeros((2 * total - m, 4), dtype=np.int32)
+    aabbs = np.zeros((2 * total - m, 6), dtype=np.float32)
+    lib().orc_par_lbvh_build_many(_p(verts), _p(idx), _p(sizes), m, _p(keys), _p(nodes), _p(aabbs), threads)
+    return keys, nodes, aabbs
+
+
+def trace_bvh_multi(rays, verts, idx, sizes, nodes, aabbs, threads: int = 0):
+    """Closest hit over the meshes of lbvh_build_many (identity instances): hits and the mesh of
+    each hit (-1 on a miss), the minimum of (t, mesh, primID)."""
+    r = _rays_f32(rays)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+    hits = np.zeros(len(r), dtype=HIT_DTYPE)
+    inst = np.zeros(len(r), dtype=np.int32)
+    lib().orc_trace_bvh_multi(_p(r), len(r), _p(np.ascontiguousarray(verts, dtype=np.float32)),
+                              _p(np.ascontiguousarray(idx, dtype=np.uint32)), _p(sizes), len(sizes),
+                              _p(np.ascontiguousarray(nodes, dtype=np.int32)),
+                              _p(np.ascontiguousarray(aabbs, dtype=np.float32)), _p(hits), _p(inst), threads)
+    return hits, inst
+
+
+def host_info() -> dict:
+    """The CPU the baseline runs on, read at run time."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": int(lib().orc_num_procs()), "omp_max_threads": int(lib().orc_max_threads()),
+            "os_cpu_count": os.cpu_count(), "cpu_model": model}

@@ -469,6 +469,58 @@ static bool slab(const Pre &r, const float *inv, const float *b, float tbest, fl
   return std::fmax(tn - slack, r.tmin) <= std::fmin(tf * (1.0f + 2.0f * g3) + slack, tbest);
 }
 
+// Closest hit of one ray (8 floats) over the reference-layout tree; adds the inner nodes visited
+// and the triangles tested to the counters.
+static void trace_ray(const float *ray, const float *verts, const uint32_t *idx, const int32_t *nodes,
+                      const float *aabbs, long n, float *hit, uint64_t &visited, uint64_t &tested) {
+  const Pre r = pre_of(ray);
+  float inv[3];
+  for (int a = 0; a < 3; ++a) {
+    const float d = std::fabs(r.d[a]) < 1e-30f ? std::copysign(1e-30f, r.d[a]) : r.d[a];
+    inv[a] = 1.0f / d;
+  }
+  float tbest = r.tmax, bu = 0.f, bv = 0.f;
+  int32_t best = -1;
+  if (n == 1) {
+    consider(r, verts, idx, (uint32_t)nodes[3], tbest, best, bu, bv);
+    ++tested;
+  } else {
+    long stack[128];
+    int sp = 0;
+    long node = 0;
+    while (true) {
+      ++visited;
+      const long c[2] = {nodes[4 * node + 0], nodes[4 * node + 1]};
+      float tn[2];
+      bool h[2];
+      for (int s = 0; s < 2; ++s) h[s] = slab(r, inv, &aabbs[6 * c[s]], tbest, &tn[s]);
+      for (int s = 0; s < 2; ++s) {
+        if (h[s] && c[s] >= n - 1) {
+          consider(r, verts, idx, (uint32_t)nodes[4 * c[s] + 3], tbest, best, bu, bv);
+          ++tested;
+          h[s] = false;
+        }
+      }
+      if (h[0] && h[1]) {
+        const bool nearFirst = tn[0] <= tn[1];
+        stack[sp++] = nearFirst ? c[1] : c[0];
+        node = nearFirst ? c[0] : c[1];
+      } else if (h[0]) {
+        node = c[0];
+      } else if (h[1]) {
+        node = c[1];
+      } else {
+        if (sp == 0) break;
+        node = stack[--sp];
+      }
+    }
+  }
+  hit[0] = tbest;
+  std::memcpy(&hit[1], &best, 4);
+  hit[2] = bu;
+  hit[3] = bv;
+}
+
 // Serial (threads=1) or per-ray parallel BVH traversal over the reference-layout tree.
 // counters (optional): [0] inner nodes visited, [1] triangles tested, summed over rays.
 void orc_trace_bvh(const float *rays, size_t nrays, const float *verts, const uint32_t *idx, const int32_t *nodes,
@@ -476,58 +528,18 @@ void orc_trace_bvh(const float *rays, size_t nrays, const float *verts, const ui
   const long n = (long)n_;
   uint64_t visited = 0, tested = 0;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1) reduction(+ : visited, tested)
-  for (long i = 0; i < (long)nrays; ++i) {
-    const Pre r = pre_of(rays + 8 * i);
-    float inv[3];
-    for (int a = 0; a < 3; ++a) {
-      const float d = std::fabs(r.d[a]) < 1e-30f ? std::copysign(1e-30f, r.d[a]) : r.d[a];
-      inv[a] = 1.0f / d;
-    }
-    float tbest = r.tmax, bu = 0.f, bv = 0.f;
-    int32_t best = -1;
-    if (n == 1) {
-      consider(r, verts, idx, (uint32_t)nodes[3], tbest, best, bu, bv);
-      ++tested;
-    } else {
-      long stack[128];
-      int sp = 0;
-      long node = 0;
-      while (true) {
-        ++visited;
-        const long c[2] = {nodes[4 * node + 0], nodes[4 * node + 1]};
-        float tn[2];
-        bool h[2];
-        for (int s = 0; s < 2; ++s) h[s] = slab(r, inv, &aabbs[6 * c[s]], tbest, &tn[s]);
-        for (int s = 0; s < 2; ++s) {
-          if (h[s] && c[s] >= n - 1) {
-            consider(r, verts, idx, (uint32_t)nodes[4 * c[s] + 3], tbest, best, bu, bv);
-            ++tested;
-            h[s] = false;
-          }
-        }
-        if (h[0] && h[1]) {
-          const bool nearFirst = tn[0] <= tn[1];
-          stack[sp++] = nearFirst ? c[1] : c[0];
-          node = nearFirst ? c[0] : c[1];
-        } else if (h[0]) {
-          node = c[0];
-        } else if (h[1]) {
-          node = c[1];
-        } else {
-          if (sp == 0) break;
-          node = stack[--sp];
-        }
-      }
-    }
-    hits[4 * i + 0] = tbest;
-    std::memcpy(&hits[4 * i + 1], &best, 4);
-    hits[4 * i + 2] = bu;
-    hits[4 * i + 3] = bv;
-  }
+  for (long i = 0; i < (long)nrays; ++i) trace_ray(rays + 8 * i, verts, idx, nodes, aabbs, n, hits + 4 * i, visited, tested);
   if (counters) {
     counters[0] = visited;
     counters[1] = tested;
   }
+}
+
+// One ray, no threading (for callers that parallelise over rays themselves).
+void orc_trace_bvh_ray(const float *ray, const float *verts, const uint32_t *idx, const int32_t *nodes,
+                       const float *aabbs, size_t n, float *hit) {
+  uint64_t visited = 0, tested = 0;
+  trace_ray(ray, verts, idx, nodes, aabbs, (long)n, hit, visited, tested);
 }
 
 // ------------------------------------------------------------------------------------------------

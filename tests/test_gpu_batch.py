@@ -91,3 +91,75 @@ def test_batch_rejects_duplicates(ctx):
     assert "twice" in api.last_error()
     ctx.lib.gprtClearLastError()
     a.destroy()
+
+
+def _shrink(ctx, buf, nbytes):
+    ctx.lib.gprtBufferResize(ctx.h, buf.h, 1, nbytes, True, 0)
+
+
+@pytest.mark.parametrize("which", ["vertices", "indices"])
+def test_build_rejects_buffers_shrunk_after_set(ctx, which):
+    """A vertex or index buffer resized below what gprtTrianglesSet* declared fails the build, the
+    batch build and the update before any kernel reads it."""
+    v, idx = scenes.uniform_scene(1000, seed=3)
+    a = api.TriangleAccel(ctx, v, idx)
+    a.build(api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    assert api.last_error() is None
+    buf = a.vbuf if which == "vertices" else a.ibuf
+    _shrink(ctx, buf, buf.nbytes // 2)
+    assert api.last_error() is None
+    for call in (lambda: a.update(), lambda: a.build(), lambda: api.build_batch(ctx, [a])):
+        call()
+        err = api.last_error()
+        assert err is not None and f"exceed the {'vertex' if which == 'vertices' else 'index'} buffer" in err, err
+        ctx.lib.gprtClearLastError()
+    a.destroy()
+
+
+def test_box_build_rejects_buffer_shrunk_after_set(ctx):
+    rng = np.random.default_rng(3)
+    c = rng.random((500, 3), dtype=np.float32)
+    boxes = np.concatenate([c - 0.01, c + 0.01], 1).astype(np.float32)
+    a = api.AABBAccel(ctx, boxes)
+    _shrink(ctx, a.vbuf, 24 * 499)
+    a.build()
+    assert "exceed the box buffer" in (api.last_error() or "")
+    ctx.lib.gprtClearLastError()
+    a.destroy()
+
+
+def test_autotune_small_batch(ctx, orc):
+    """gprtAccelBuildBatchAutotune: every configuration of the sweep is reported, the returned index
+    names the fastest one, gprtGetSortConfig reads it back, and the accels it leaves built equal the
+    oracle's LBVH of each mesh. A batch naming an accel twice is rejected."""
+    import json
+    import ctypes as C
+    meshes = _meshes(4000)[:9]
+    accs = [api.TriangleAccel(ctx, v, idx) for v, idx in meshes]
+    arr = (C.c_void_p * len(accs))(*[a.h for a in accs])
+    rep = C.create_string_buffer(1 << 14)
+    try:
+        best = ctx.lib.gprtAccelBuildBatchAutotune(ctx.h, arr, len(accs), 1, rep, len(rep))
+        assert api.last_error() is None, api.last_error()
+        r = json.loads(rep.value.decode())
+        cfgs = r["configs"]
+        assert len(cfgs) == 27
+        assert {(c["digitBits"], c["tileKeys"], c["wavesPerCU"]) for c in cfgs} == {
+            (b, t, w) for b in (4, 6, 8) for t in (2048, 4096, 8192) for w in (0, 8, 16)}
+        ms = [c["ms"] for c in cfgs]
+        assert best == int(np.argmin(ms))
+        for k in ("digitBits", "tileKeys", "wavesPerCU"):
+            assert r["best"][k] == cfgs[best][k]
+        got = [C.c_uint32(0) for _ in range(3)]
+        ctx.lib.gprtGetSortConfig(ctx.h, *[C.byref(g) for g in got])
+        assert [g.value for g in got] == [r["best"]["digitBits"], r["best"]["tileKeys"], r["best"]["wavesPerCU"]]
+        for acc, (v, idx) in zip(accs, meshes):
+            _check(acc, orc, v, idx)
+        dup = (C.c_void_p * 2)(accs[0].h, accs[0].h)
+        ctx.lib.gprtAccelBuildBatchAutotune(ctx.h, dup, 2, 1, rep, len(rep))
+        assert "twice" in (api.last_error() or "")
+        ctx.lib.gprtClearLastError()
+    finally:
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+        for a in accs:
+            a.destroy()

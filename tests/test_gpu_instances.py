@@ -269,3 +269,65 @@ def test_bounce_rays_single_accel_match_numpy(ctx):
     for b in (rb, hb, ob):
         b.destroy()
     acc.destroy()
+
+
+@pytest.mark.parametrize("change", ["refit", "rebuild"])
+def test_proposal_skipped_after_triangle_accel_changes(ctx, orc, change):
+    """An instance accel traces copies of its instances' triangles taken at its own build. When a
+    triangle accel is refit or rebuilt (same size, same storage) afterwards without rebuilding the
+    instance accel, the proposal step must not walk the changed triangles: traces with and
+    without a proposal stay identical, and both equal the brute force over the geometry the
+    instance accel was built from."""
+    sizes = [20000, 20000, 20000]
+    meshes = _meshes(sizes, 2600, blob=True)
+    blas = [api.TriangleAccel(ctx, v, idx) for v, idx in meshes]
+    for a in blas:
+        a.build(api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    tlas = api.InstanceAccel(ctx, api.instance_records(blas))
+    tlas.build()
+    rays = _rays(96, 16)
+    n = len(rays)
+    rb = ctx.device_buffer(rays)
+    hb, ib, lb = ctx.device_buffer(nbytes=16 * n), ctx.device_buffer(nbytes=4 * n), ctx.device_buffer(nbytes=4 * n)
+    api.trace_detail(ctx, tlas, rb, hb, n, ib, lb)
+    hits0, inst0, leaf0 = hb.read(api.HIT_DTYPE, n), ib.read(np.int32, n), lb.read(np.int32, n)
+    assert _check(hits0, inst0, _oracle(orc, rays, meshes)) > 0.05
+    prop = api.Proposal(ctx, max_batch=1024, seed=5)
+    prop.bind(tlas)
+    prop.train_labels(rb, lb, n, epochs=4, lr=2e-3)
+    # Move every triangle of instance 0 towards the camera: the moved geometry would be hit first.
+    v0, idx0 = meshes[0]
+    moved = v0.copy()
+    moved[:, 2] -= np.float32(0.3)
+    blas[0].vbuf.write(moved)
+    if change == "refit":
+        blas[0].update()
+    else:
+        blas[0].build(api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    assert api.last_error() is None, api.last_error()
+    h1, i1, l1 = (ctx.device_buffer(nbytes=16 * n), ctx.device_buffer(nbytes=4 * n),
+                  ctx.device_buffer(nbytes=4 * n))
+    print(f"proposal after a {change} of a triangle accel: {n} rays", flush=True)
+    api.trace_detail(ctx, tlas, rb, hb, n, ib, lb)
+    api.trace_detail(ctx, tlas, rb, h1, n, i1, l1, proposal=prop)
+    ctx.lib.gprtAccelTraceClosestHitEx(ctx.h, tlas.h, prop.h, rb.h, h1.h, n, None)
+    assert api.last_error() is None, api.last_error()
+    plain = hb.read(api.HIT_DTYPE, n)
+    np.testing.assert_array_equal(plain.view(np.uint32), hits0.view(np.uint32))
+    np.testing.assert_array_equal(ib.read(np.int32, n), inst0)
+    np.testing.assert_array_equal(h1.read(api.HIT_DTYPE, n).view(np.uint32), plain.view(np.uint32))
+    np.testing.assert_array_equal(i1.read(np.int32, n), inst0)
+    np.testing.assert_array_equal(l1.read(np.int32, n), leaf0)
+    # After the instance accel is rebuilt it traces the moved triangles, with or without proposal.
+    tlas.build()
+    api.trace_detail(ctx, tlas, rb, hb, n, ib, lb)
+    api.trace_detail(ctx, tlas, rb, h1, n, i1, l1, proposal=prop)
+    now = [(moved, idx0)] + meshes[1:]
+    _check(hb.read(api.HIT_DTYPE, n), ib.read(np.int32, n), _oracle(orc, rays, now))
+    np.testing.assert_array_equal(h1.read(api.HIT_DTYPE, n).view(np.uint32), hb.read(api.HIT_DTYPE, n).view(np.uint32))
+    for b in (rb, hb, ib, lb, h1, i1, l1):
+        b.destroy()
+    prop.destroy()
+    tlas.destroy()
+    for a in blas:
+        a.destroy()

@@ -203,3 +203,28 @@ def test_trace_single_triangle_and_misses(ctx, orc):
     assert g["primID"].tolist() == [0, -1, -1]
     assert g["t"][0] == 2.0
     acc.destroy()
+
+
+def test_refit_follows_moved_vertex_buffer(ctx, orc):
+    """An update reads the geometry where it is now: after the vertex buffer is resized (the
+    allocation moves) or the geometry is pointed at another buffer, the refit boxes equal the
+    oracle's refit of the new vertices over the kept topology."""
+    verts, idx = scenes.blob_scene(70_000, seed=8)
+    acc = _build(ctx, verts, idx, api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    _, nodes0, _ = acc.export()
+    moved = (verts + np.float32(1e-3)).astype(np.float32)
+    ctx.lib.gprtBufferResize(ctx.h, acc.vbuf.h, 1, 2 * verts.nbytes, False, 0)
+    acc.vbuf.write(moved)
+    acc.update()
+    assert api.last_error() is None, api.last_error()
+    _, nodes1, aabbs1 = acc.export()
+    np.testing.assert_array_equal(nodes1, nodes0)
+    np.testing.assert_array_equal(aabbs1.view(np.uint32), orc.refit(moved, idx, nodes0).view(np.uint32))
+    other = ctx.device_buffer((verts - np.float32(2e-3)).astype(np.float32))
+    ctx.lib.gprtTrianglesSetVertices(acc.geom, other.h, len(verts), 12, 0)
+    acc.update()
+    _, _, aabbs2 = acc.export()
+    ref = orc.refit((verts - np.float32(2e-3)).astype(np.float32), idx, nodes0)
+    np.testing.assert_array_equal(aabbs2.view(np.uint32), ref.view(np.uint32))
+    acc.destroy()
+    other.destroy()
