@@ -39,6 +39,17 @@ class GPRTBuildParams(C.Structure):
         super().__init__(buildMode, False, False, False, False, False)
 
 
+GPRT_LBVH_POINTS = 1
+GPRT_LBVH_EDGES = 2
+GPRT_LBVH_TRIANGLES = 4
+
+
+class GPRTLBVH(C.Structure):
+    """gprt/development/gprt_lbvh.h:64-85: the LBVH and its result buffers."""
+    _fields_ = [("type", C.c_int), ("numPrims", C.c_uint32), ("mortonCodes", C.c_void_p), ("ids", C.c_void_p),
+                ("nodes", C.c_void_p), ("aabbs", C.c_void_p), ("impl", C.c_void_p)]
+
+
 class GPRTInstance(C.Structure):
     """gprt::Instance (gprt/gprt_shared.h:92-101): 3x4 row-major transform [A | b], then
     instanceCustomIndex:24 | mask:8, geometryIndex:24 | flags:8, accelAddress (64 bytes)."""
@@ -114,6 +125,11 @@ _SIGS = {
     "gprtAccelGetDeviceAddress": (_P, [_P, C.c_int]),
     "gprtAccelGetPrimCount": (C.c_uint32, [_P]),
     "gprtAccelGetBounds": (None, [_P, _P, _P]),
+    "gprtPointsLBVHCreate": (GPRTLBVH, [_P, _P, C.c_uint32]),
+    "gprtEdgesLBVHCreate": (GPRTLBVH, [_P, _P, _P, C.c_uint32]),
+    "gprtTriangleLBVHCreate": (GPRTLBVH, [_P, _P, _P, C.c_uint32]),
+    "gprtLBVHBuild": (None, [_P, C.POINTER(GPRTLBVH)]),
+    "gprtLBVHDestroy": (None, [C.POINTER(GPRTLBVH)]),
     "gprtAccelTraceClosestHit": (None, [_P, _P, _P, _P, C.c_uint32]),
     "gprtAccelExportLBVH": (None, [_P, _P, _P, _P, _P]),
     "gprtProposalCreate": (_P, [_P, C.c_uint32, C.c_uint64]),
@@ -131,6 +147,9 @@ _SIGS = {
     "gprtProposalLogits": (None, [_P, _P, _P, C.c_uint32, _P]),
     "gprtProposalGradients": (None, [_P, _P, _P, _P, C.c_uint32, _P, C.c_size_t]),
     "gprtProposalExportTreelets": (None, [_P, _P, _P, _P]),
+    "gprtProposalSetGate": (None, [_P, _P, C.c_float]),
+    "gprtProposalGetGate": (C.c_float, [_P]),
+    "gprtProposalCalibrateGate": (C.c_float, [_P, _P, _P, _P, C.c_uint32]),
     "gprtAccelTraceClosestHitEx": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P]),
     "gprtSynthBlobMeshes": (None, [_P, _P, C.c_uint32, C.c_uint64, C.c_float, _P, _P]),
     "gprtSynthPerturb": (None, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_float]),
@@ -380,6 +399,8 @@ def mesh_accels(ctx: Context, vbuf: Buffer, ibuf: Buffer, sizes):
     """One TriangleAccel per mesh of concatenated geometry (3 vertices per triangle, as written
     by synth_blob_meshes), referencing the shared buffers at the mesh's offsets."""
     accs, t0 = [], 0
+    if 36 * int(np.sum(sizes)) > 0xFFFFFFFF:
+        raise ValueError("geometry offsets are 32-bit (gprtTrianglesSetVertices): use one buffer per mesh beyond 4 GiB")
     for n in sizes:
         n = int(n)
         accs.append(TriangleAccel(ctx, buffers=(vbuf, ibuf, 3 * n, n), offsets=(36 * t0, 12 * t0),
@@ -390,6 +411,47 @@ def mesh_accels(ctx: Context, vbuf: Buffer, ibuf: Buffer, sizes):
 
 def _h(b):
     return b.h if b is not None else None
+
+
+class LBVH:
+    """gprtPointsLBVHCreate / gprtEdgesLBVHCreate / gprtTriangleLBVHCreate + gprtLBVHBuild: the
+    development LBVH over float3 vertices and (for edges / triangles) uint2 / uint3 indices."""
+
+    def __init__(self, ctx: Context, verts: np.ndarray, prims: np.ndarray | None = None, kind: str = "triangles"):
+        self.ctx, self.lib = ctx, ctx.lib
+        verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        self.vbuf = ctx.device_buffer(verts)
+        self.pbuf = None
+        if kind == "points":
+            self.h = self.lib.gprtPointsLBVHCreate(ctx.h, self.vbuf.h, len(verts) if prims is None else int(prims))
+        else:
+            width = 2 if kind == "edges" else 3
+            p = np.ascontiguousarray(prims, dtype=np.uint32).reshape(-1, width)
+            self.pbuf = ctx.device_buffer(p)
+            fn = self.lib.gprtEdgesLBVHCreate if kind == "edges" else self.lib.gprtTriangleLBVHCreate
+            self.h = fn(ctx.h, self.vbuf.h, self.pbuf.h, len(p))
+        if not self.h.impl:
+            raise RuntimeError("LBVH create failed: %s" % last_error())
+        self.n = int(self.h.numPrims)
+
+    def build(self):
+        self.lib.gprtLBVHBuild(self.ctx.h, C.byref(self.h))
+
+    def read(self):
+        """(mortonCodes u32[n], ids u32[n], nodes int32[2n-1, 4], aabbs float32[2n-1, 6])."""
+        n = self.n
+        out = []
+        for h, dt, cnt in ((self.h.mortonCodes, np.uint32, n), (self.h.ids, np.uint32, n),
+                           (self.h.nodes, np.int32, 4 * (2 * n - 1)), (self.h.aabbs, np.float32, 6 * (2 * n - 1))):
+            out.append(Buffer(self.ctx, h).read(dt, cnt))
+        return out[0], out[1], out[2].reshape(-1, 4), out[3].reshape(-1, 6)
+
+    def destroy(self):
+        if self.h.impl:
+            self.lib.gprtLBVHDestroy(C.byref(self.h))
+        self.vbuf.destroy()
+        if self.pbuf is not None:
+            self.pbuf.destroy()
 
 
 def accel_bounds(ctx: Context, accel) -> np.ndarray:

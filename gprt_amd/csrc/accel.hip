@@ -120,8 +120,8 @@ GPRT_API void gprtAABBsSetPositions(GPRTGeom _g, GPRTBuffer _p, uint32_t count, 
 
 // An accel over box primitives (gprt/gprt_host.h:1282). It is built by the same LBVH builder; its
 // closest hit is the box surface (boxprim_hit), standing in for the user intersection program
-// that the reference's AABB geometry runs. It has no descriptor slot, so instance accels can not
-// reference it.
+// that the reference's AABB geometry runs. Its descriptor slot (what device programs trace
+// through) carries the box magic word, which instance records do not accept.
 GPRT_API GPRTAccel gprtAABBAccelCreate(GPRTContext _ctx, GPRTGeom _g, unsigned int flags) {
   Context *ctx = (Context *)_ctx;
   Geom *g = (Geom *)_g;
@@ -138,6 +138,10 @@ GPRT_API GPRTAccel gprtAABBAccelCreate(GPRTContext _ctx, GPRTGeom _g, unsigned i
   a->kind = AccelKind::AABBs;
   a->geom = g;
   a->flags = flags;
+  if (!blas_slot_acquire(ctx, a)) {
+    delete a;
+    return nullptr;
+  }
   return (GPRTAccel)a;
 }
 
@@ -470,12 +474,13 @@ GPRT_API size_t gprtAccelGetSize(GPRTAccel _a, int) {
                          : 0;
 }
 
-// A triangle accel's address is its descriptor slot (what gprt::Instance::accelAddress names); it
-// stays the same for the accel's lifetime. An instance accel's is its top-level node array.
+// A triangle or AABB accel's address is its descriptor slot (what gprt::Instance::accelAddress
+// names, and what device programs trace through); it stays the same for the accel's lifetime. An
+// instance accel's is its top-level node array.
 GPRT_API const void *gprtAccelGetDeviceAddress(GPRTAccel _a, int) {
   Accel *a = (Accel *)_a;
   if (!a) return nullptr;
-  if (a->kind != AccelKind::Triangles || a->slot == 0xFFFFFFFFu) return a->nodes;
+  if (a->kind == AccelKind::Instances || a->slot == 0xFFFFFFFFu) return a->nodes;
   return a->ctx->blasPool ? (const void *)(a->ctx->blasPool + a->slot) : nullptr;
 }
 

@@ -549,7 +549,7 @@ bool blas_slot_acquire(Context *ctx, Accel *a) {
   } else if (ctx->blasNext < kBlasPoolSlots) {
     a->slot = ctx->blasNext++;
   } else {
-    fail("gprtTriangleAccelCreate: more than %u live triangle accels in one context", kBlasPoolSlots);
+    fail("accel create: more than %u live triangle and AABB accels in one context", kBlasPoolSlots);
     return false;
   }
   if (ctx->slotAccel.size() <= a->slot) ctx->slotAccel.resize((size_t)a->slot + 1, nullptr);
@@ -579,13 +579,14 @@ bool blas_desc_refresh(Context *ctx, hipStream_t s, Accel *a) {
     d.nodes = a->nodes;
     d.tris = a->tris;
     d.header = a->header;
+    const bool boxes = a->kind == AccelKind::AABBs;
     d.verts = (const char *)g->vbuf->d + g->voffset;
-    d.idx = (const char *)g->ibuf->d + g->ioffset;
+    d.idx = boxes ? nullptr : (const char *)g->ibuf->d + g->ioffset;
     d.n = a->n;
     d.vstride = g->vstride;
-    d.istride = g->istride;
+    d.istride = boxes ? 0u : g->istride;
     d.nverts = g->vcount;
-    d.magic = kBlasMagic;
+    d.magic = boxes ? kBoxAccelMagic : kBlasMagic;  // instance records accept triangle accels only
     d.slot = a->slot;
   }
   if (std::memcmp(&d, &a->hdesc, sizeof(d)) == 0) return true;

@@ -13,8 +13,15 @@
 #include <vector>
 
 #include "../../include/gprt_hip.h"
+#include "../../include/gprt_hip_device.h"
 
 namespace gprt {
+
+// Descriptor of a triangle or AABB accel in the context's pool (gprtAccelGetDeviceAddress): the
+// layout device programs read (include/gprt_hip_device.h).
+using BlasDesc = gprt_device::AccelDescriptor;
+constexpr uint32_t kBlasMagic = gprt_device::kTriangleAccelMagic;
+constexpr uint32_t kBoxAccelMagic = gprt_device::kBoxAccelMagic;
 
 // ------------------------------------------------------------------------------------------------
 // Errors (gprt/gprt.cpp:205-251: print "#gprt error:" then raise(SIGINT)).
@@ -81,7 +88,7 @@ struct Context {
   // Pool of triangle-accel descriptors (device, fixed capacity so addresses stay valid): the
   // address an instance record names (gprt::Instance::accelAddress) is a slot of this pool, which
   // lets the instance-accel build check every address on the device before reading through it.
-  struct BlasDesc *blasPool = nullptr;
+  BlasDesc *blasPool = nullptr;
   std::vector<uint32_t> blasFree;
   uint32_t blasNext = 0;
   uint64_t blasEpoch = 0;  // bumped whenever a descriptor is retired or rewritten
@@ -89,7 +96,7 @@ struct Context {
   // instance accel traces copies taken at its own build, so a proposal step that walks a
   // triangle accel's live arrays is valid only while this is unchanged since that build.
   uint64_t blasContentEpoch = 0;
-  std::vector<struct Accel *> slotAccel;  // host view: pool slot -> live triangle accel
+  std::vector<struct Accel *> slotAccel;  // host view: pool slot -> live triangle or AABB accel
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -136,8 +143,13 @@ struct MeshDesc {
   uint32_t innerOffset;  // first inner node of this mesh (= primOffset - mesh index)
   uint32_t tileOffset;   // first tile of this mesh
   uint32_t ntiles;       // ceil(ntri / tileKeys)
-  uint32_t boxes;        // 1: the primitives are boxes {lo.xyz, hi.xyz} at verts + i * vstride (idx unused)
+  uint32_t kind;         // PrimKind: what a primitive is and where it is read from
 };
+
+// Primitive kinds of the LBVH builder. Triangles: uint3 indices at idx + i * istride into float3
+// vertices. Boxes: {lo.xyz, hi.xyz} at verts + i * vstride (idx unused). Edges: uint2 indices at
+// idx + i * istride. Points: the float3 vertex i itself.
+enum PrimKind : uint32_t { kPrimTriangles = 0, kPrimBoxes = 1, kPrimEdges = 2, kPrimPoints = 3 };
 
 // Device storage for the LBVHs of one or more meshes built together (one accel = one mesh).
 // Every kernel of the build is launched over the tile table, so a batch of thousands of uneven
@@ -183,22 +195,11 @@ struct BVHStore {
 
 // Device descriptor of a built triangle accel (64 B, one pool slot per accel, rewritten after
 // every build): what an instance accel needs to traverse it and to read its triangles.
-struct BlasDesc {
-  const void *nodes;  // BVHNode[n - 1]
-  const void *tris;   // TriRecord[n]
-  const float *header;
-  const char *verts;
-  const char *idx;
-  uint32_t n, vstride, istride, nverts;
-  uint32_t magic;  // kBlasMagic while the accel is built and alive
-  uint32_t slot;
-};
-constexpr uint32_t kBlasMagic = 0x53414C42u;
 constexpr uint32_t kBlasPoolSlots = 65536;
 
 // Triangles and AABBs are primitive accels (built by the LBVH builder); Instances is the two-level
 // accel over triangle accels.
-enum class AccelKind { Triangles, Instances, AABBs };
+enum class AccelKind { Triangles, Instances, AABBs, Edges, Points };
 struct Tlas;
 
 struct Accel {

@@ -107,13 +107,23 @@ __device__ __forceinline__ void load_triangle(const MeshDesc &d, uint32_t prim, 
 }
 
 // Primitive `prim` as three points whose bounding box is the primitive's box: a triangle's
-// vertices, or a box primitive's two corners {lo.xyz, hi.xyz} (the first repeated).
+// vertices, a box primitive's two corners {lo.xyz, hi.xyz}, an edge's two end points (the first
+// repeated), or a point three times.
 __device__ __forceinline__ void load_prim(const MeshDesc &d, uint32_t prim, float3 &a, float3 &b, float3 &c) {
-  if (d.boxes) {
-    prim = prim < d.ntri ? prim : d.ntri - 1;
+  prim = prim < d.ntri ? prim : d.ntri - 1;
+  if (d.kind == kPrimBoxes) {
     const auto *p = global_ptr<float>(d.verts + (size_t)prim * d.vstride);
     a = make_float3(p[0], p[1], p[2]);
     b = make_float3(p[3], p[4], p[5]);
+    c = a;
+  } else if (d.kind == kPrimEdges) {
+    const auto *e = global_ptr<uint32_t>(d.idx + (size_t)prim * d.istride);
+    a = load_vertex(d, e[0]);
+    b = load_vertex(d, e[1]);
+    c = a;
+  } else if (d.kind == kPrimPoints) {
+    a = load_vertex(d, prim);
+    b = a;
     c = a;
   } else {
     load_triangle(d, prim, a, b, c);
@@ -156,13 +166,18 @@ __global__ void __launch_bounds__(256) k_init(uint32_t *__restrict__ zero, size_
   for (; i < words; i += stride) zero[i] = 0u;
 }
 
-// Triangle centroid, rounded exactly as the key computation of gprt_lbvh.hlsl:139-146 (no
-// contraction); a box primitive's centre is (lo + hi) * 0.5.
+// The point a primitive's key is computed from, rounded exactly as the key computations of
+// gprt_lbvh.hlsl (no contraction): a triangle's centroid (p1 + p2 + p3) / 3 (:139-146), an edge's
+// midpoint (p1 + p2) / 2 (:131-137), a point itself (:228-236); a box primitive's centre is
+// (lo + hi) * 0.5.
 __device__ __forceinline__ float3 centroid(float3 p, float3 q, float3 s) {
   return make_float3((p.x + q.x + s.x) / 3.0f, (p.y + q.y + s.y) / 3.0f, (p.z + q.z + s.z) / 3.0f);
 }
 __device__ __forceinline__ float3 prim_centroid(const MeshDesc &d, float3 p, float3 q, float3 s) {
-  return d.boxes ? make_float3((p.x + q.x) * 0.5f, (p.y + q.y) * 0.5f, (p.z + q.z) * 0.5f) : centroid(p, q, s);
+  if (d.kind == kPrimBoxes) return make_float3((p.x + q.x) * 0.5f, (p.y + q.y) * 0.5f, (p.z + q.z) * 0.5f);
+  if (d.kind == kPrimEdges) return make_float3((p.x + q.x) / 2.0f, (p.y + q.y) / 2.0f, (p.z + q.z) / 2.0f);
+  if (d.kind == kPrimPoints) return p;
+  return centroid(p, q, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -186,7 +201,7 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
   constexpr int kBoundsBatch = 4;
   for (uint32_t i0 = r.lo + threadIdx.x; i0 < r.hi; i0 += kBoundsThreads * kBoundsBatch) {
     float3 p[kBoundsBatch], q[kBoundsBatch], s[kBoundsBatch];
-    if (d.boxes) {
+    if (d.kind != kPrimTriangles) {
 #pragma unroll
       for (int u = 0; u < kBoundsBatch; ++u) {
         const uint32_t i = min(i0 + u * kBoundsThreads, r.hi - 1);  // past the end: repeats a valid one
@@ -242,7 +257,7 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
     }
   }
   const int allIdent = __syncthreads_and(ident ? 1 : 0);
-  if (threadIdx.x == 6 && !allIdent && !d.boxes)
+  if (threadIdx.x == 6 && !allIdent && d.kind == kPrimTriangles)
     atomicAnd((uint32_t *)a.headers + (size_t)m * 16 + kIdentityWord, 0u);
   if (threadIdx.x < 6) {
     const int c = threadIdx.x;
@@ -653,7 +668,7 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
   s_split[tid] = 0xFFFFu;
   s_out[tid] = kNoOutput;
   float box[6];
-  const bool ident = identOk && !d.boxes && ((const uint32_t *)header)[kIdentityWord] != 0u;
+  const bool ident = identOk && d.kind == kPrimTriangles && ((const uint32_t *)header)[kIdentityWord] != 0u;
   if (k < hi) {
     float3 p, q, s;
     if (ident) {
@@ -671,12 +686,16 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
     box[3] = fmax_ord(p.x, fmax_ord(q.x, s.x));
     box[4] = fmax_ord(p.y, fmax_ord(q.y, s.y));
     box[5] = fmax_ord(p.z, fmax_ord(q.z, s.z));
-    // Leaf record: a triangle {v0, primID} {v1} {v2}, or a box primitive's ordered corners
-    // {lo, primID} {hi} {0}.
+    // Leaf record: a triangle {v0, primID} {v1} {v2}, an edge {p1, primID} {p2} {0}, or a box
+    // primitive's (a point's) ordered corners {lo, primID} {hi} {0}.
     TriRecord rec;
-    if (d.boxes) {
+    if (d.kind == kPrimBoxes || d.kind == kPrimPoints) {
       rec.v0 = make_float4(box[0], box[1], box[2], __uint_as_float(prim));
       rec.v1 = make_float4(box[3], box[4], box[5], 0.0f);
+      rec.v2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else if (d.kind == kPrimEdges) {
+      rec.v0 = make_float4(p.x, p.y, p.z, __uint_as_float(prim));
+      rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
       rec.v2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else {
       rec.v0 = make_float4(p.x, p.y, p.z, __uint_as_float(prim));
@@ -1183,7 +1202,7 @@ __global__ void __launch_bounds__(256) k_export(const BVHNode *__restrict__ node
         for (int q = 0; q < 6; ++q) b[q] = __uint_as_float(0x7FC00000u);
       }
     }
-    outKeys[k] = ((uint64_t)codes[k] << 32) | ids[k];
+    if (outKeys) outKeys[k] = ((uint64_t)codes[k] << 32) | ids[k];
   }
   for (int q = 0; q < 6; ++q) outBoxes[(size_t)x * 6 + q] = b[q];
 }
@@ -1438,12 +1457,12 @@ void geom_fields(const Accel *acc, MeshDesc &d) {
   d.vstride = g->vstride;
   d.nverts = g->vcount;
   d.ntri = g->icount;
-  if (acc->kind == AccelKind::AABBs) {
-    d.boxes = 1u;  // {lo.xyz, hi.xyz} per primitive at verts + i * vstride
+  if (acc->kind == AccelKind::AABBs || acc->kind == AccelKind::Points) {
+    d.kind = acc->kind == AccelKind::AABBs ? kPrimBoxes : kPrimPoints;  // read at verts + i * vstride
     d.idx = nullptr;
     d.istride = 0;
   } else {
-    d.boxes = 0u;
+    d.kind = acc->kind == AccelKind::Edges ? kPrimEdges : kPrimTriangles;
     d.idx = (const char *)g->ibuf->d + g->ioffset;
     d.istride = g->istride;
   }

@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(64) k_infer_hidden(const GPRTRay *__restrict__
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_infer_top1(uint32_t count, const uint16_t *__restrict__ W4t, const float *__restrict__ bias,
                  const uint16_t *__restrict__ h3, int32_t *__restrict__ outClass, float *__restrict__ outLogit,
-                 uint32_t numClasses) {
+                 uint32_t numClasses, float gate) {
   extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const uint32_t ray0 = blockIdx.x * kInferRays;
@@ -602,6 +602,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       for (int o = 1; o < 16; o <<= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
       int c = 4095 - (int)(__float_as_uint(v) & 0xFFFu);
       if ((uint32_t)c >= numClasses) c = 0;  // only without any finite logit
+      if (v < gate) c = -1;                  // below the confidence gate: no proposal
       const uint32_t ri = ray0 + w * kInferRaysPerWave + i * 16 + 4 * (l >> 4) + q;
       if ((l & 15) == 0 && ri < count) {
         outClass[ri] = c;
@@ -954,7 +955,8 @@ void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint
          (const int32_t *)p->primClass, p->treePrims, labels);
 }
 
-void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit) {
+void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit,
+               float gate) {
   const uint32_t classes = p->numClasses ? p->numClasses : (uint32_t)kMlpOut;
   // Two kernels through a bf16 activation buffer of at most 2^22 rays (1 GiB) per round.
   const uint32_t chunk = std::min<uint32_t>(n, 1u << 22);
@@ -977,7 +979,7 @@ void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint3
            (uint16_t *)p->h3);
     launch(ctx, s, "mlp_infer_top1", k_infer_top1, dim3(blocks_for(m, kInferRays)), dim3(256), ldsOut, m,
            mlp_wbf(p, 3), (const float *)p->bias, (const uint16_t *)p->h3, cls + r0, logit ? logit + r0 : nullptr,
-           classes);
+           classes, gate);
   }
 }
 

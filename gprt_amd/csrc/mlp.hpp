@@ -1,6 +1,7 @@
 // Treelet proposal network: sizes, parameter layout and host-side state.
 #pragma once
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 
@@ -56,6 +57,9 @@ struct Proposal {
   // Per-ray scratch for proposal-first traces.
   void *rayScratch = nullptr;
   size_t rayScratchBytes = 0;
+  // Confidence gate of proposal-first traces: rays whose top logit is below it are traced without
+  // their proposal (-INFINITY: every ray uses its proposal).
+  float gate = -INFINITY;
 };
 
 size_t mlp_param_count();
@@ -70,7 +74,10 @@ void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint
 // Classes of top-level leaves (instance accels): labels[i] = leafClass[leaves[i]], or -1.
 void mlp_labels_of_leaves(Context *ctx, hipStream_t s, Proposal *p, const int32_t *leaves, uint32_t n,
                           int32_t *labels);
-void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit);
+// Top-1 class per ray (and its top logit when `logit` is set); a ray whose top logit is below
+// `gate` gets class -1 (-INFINITY: every ray keeps its class).
+void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit,
+               float gate);
 bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth);
 
 }  // namespace gprt

@@ -1,6 +1,7 @@
 // C-ABI of the learned treelet proposal: create / bind to an accel (treelet cut) / train on
 // plain-traversal hits / infer / proposal-first trace, plus parameter and gradient access for
 // tests. None of these has a reference counterpart (see the extension block of gprt_hip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -450,7 +451,7 @@ GPRT_API void gprtProposalInfer(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _r
     return;
   }
   if (count == 0) return;
-  mlp_infer(ctx, ctx->stream, p, rays->d, count, (int32_t *)cls->d, nullptr);
+  mlp_infer(ctx, ctx->stream, p, rays->d, count, (int32_t *)cls->d, nullptr, -INFINITY);
 }
 
 GPRT_API void gprtProposalExportTreelets(GPRTContext _ctx, GPRTProposal _p, GPRTBuffer _classRoot,
@@ -508,7 +509,7 @@ void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *r
   if (p) {
     if (!ensure_ray_scratch(p, (size_t)count * 4, s)) return;
     cls = (int32_t *)p->rayScratch;
-    mlp_infer(ctx, s, p, rays->d, count, cls, nullptr);
+    mlp_infer(ctx, s, p, rays->d, count, cls, nullptr, p->gate);
   }
   uint32_t *vis = visits ? (uint32_t *)visits->d : nullptr;
   if (a->kind == AccelKind::Instances) {
@@ -526,6 +527,91 @@ void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *r
 }
 
 }  // namespace
+
+GPRT_API void gprtProposalSetGate(GPRTContext _ctx, GPRTProposal _p, float threshold) {
+  Proposal *p = (Proposal *)_p;
+  if (!_ctx || !p) {
+    fail("gprtProposalSetGate: null argument");
+    return;
+  }
+  if (std::isnan(threshold)) {
+    fail("gprtProposalSetGate: the threshold is NaN");
+    return;
+  }
+  p->gate = threshold;
+}
+
+GPRT_API float gprtProposalGetGate(GPRTProposal _p) {
+  Proposal *p = (Proposal *)_p;
+  return p ? p->gate : NAN;
+}
+
+// The gate from measured traversal cost: for every ray of the sample the inner nodes fetched
+// plainly and proposal-first (ungated) are counted; proposing to the rays whose top logit is at
+// least t costs sum over those rays of (proposal-first - plain) fetches, and the threshold with
+// the smallest total is kept (none proposed costs 0, so the gate never makes the sample dearer).
+GPRT_API float gprtProposalCalibrateGate(GPRTContext _ctx, GPRTProposal _p, GPRTAccel _a, GPRTBuffer _rays,
+                                         uint32_t count) {
+  const StageRange range("gprtProposalCalibrateGate");
+  const char *fn = "gprtProposalCalibrateGate";
+  Context *ctx = (Context *)_ctx;
+  Proposal *p = (Proposal *)_p;
+  Accel *a = (Accel *)_a;
+  Buffer *rays = (Buffer *)_rays;
+  if (!ctx || !p || !a || !rays) {
+    fail("%s: null argument", fn);
+    return NAN;
+  }
+  if (!a->built || p->accel != a || p->treePrims != a->n) {
+    fail("%s: the proposal is not bound to this built accel (bind it after the build)", fn);
+    return NAN;
+  }
+  if ((size_t)count * sizeof(GPRTRay) > rays->size) {
+    fail("%s: %u rays exceed the ray buffer", fn, count);
+    return NAN;
+  }
+  if (count == 0) return 0.f;
+  hipStream_t s = ctx->stream;
+  const size_t n = count;
+  char *blk = nullptr;
+  const size_t bytes = n * (4 + 4 + 4 + 4 + sizeof(GPRTHit));
+  if (!hip_ok(hipMalloc((void **)&blk, bytes), "hipMalloc(calibration)")) return NAN;
+  int32_t *cls = (int32_t *)blk;
+  float *logit = (float *)(blk + 4 * n);
+  uint32_t *v0 = (uint32_t *)(blk + 8 * n), *v1 = (uint32_t *)(blk + 12 * n);
+  void *hits = blk + 16 * n;
+  mlp_infer(ctx, s, p, rays->d, count, cls, logit, -INFINITY);
+  if (a->kind == AccelKind::Instances) {
+    tlas_trace(ctx, s, a, rays->d, hits, count, nullptr, 0, nullptr, nullptr, v0);
+    tlas_trace(ctx, s, a, rays->d, hits, count, cls, p->numClasses, nullptr, nullptr, v1);
+  } else {
+    trace_closest_ex(ctx, s, a, rays->d, hits, count, nullptr, nullptr, 0, v0);
+    trace_closest_ex(ctx, s, a, rays->d, hits, count, cls, p->classRoot, p->numClasses, v1);
+  }
+  std::vector<float> lg(n);
+  std::vector<uint32_t> h0(n), h1(n);
+  bool ok = hip_ok(hipMemcpyAsync(lg.data(), logit, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
+            hip_ok(hipMemcpyAsync(h0.data(), v0, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
+            hip_ok(hipMemcpyAsync(h1.data(), v1, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
+            stream_wait(ctx, s, fn);
+  hipFree(blk);
+  if (!ok) return NAN;
+  std::vector<uint32_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = (uint32_t)i;
+  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return lg[x] > lg[y]; });
+  int64_t cum = 0, best = 0;
+  size_t keep = 0;
+  for (size_t k = 0; k < n; ++k) {
+    cum += (int64_t)h1[order[k]] - (int64_t)h0[order[k]];
+    // A cut between equal logits cannot be expressed by a threshold.
+    if (cum < best && (k + 1 == n || lg[order[k + 1]] < lg[order[k]])) {
+      best = cum;
+      keep = k + 1;
+    }
+  }
+  p->gate = keep == 0 ? INFINITY : (keep == n ? -INFINITY : lg[order[keep - 1]]);
+  return (float)keep / (float)n;
+}
 
 GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext _ctx, GPRTAccel _a, GPRTProposal _p, GPRTBuffer _rays,
                                          GPRTBuffer _hits, uint32_t count, GPRTBuffer _visits) {

@@ -177,21 +177,21 @@ size_t scan_scratch_bytes(size_t n) {
 }
 
 void scan_run(Context *ctx, hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, int mode, bool selectPositive,
-              void *scratch, uint32_t *totalDev) {
-  (void)totalDev;  // state[0] is the total; the caller reads it from the scratch base
+              void *state, uint32_t *totalDev) {
+  (void)totalDev;  // state[0] is the total; the caller reads it from the state's base
   const uint32_t tileItems = mode == 0 ? kScanTileSum : kScanTileSel;
   const uint32_t tiles = (uint32_t)((n + tileItems - 1) / tileItems);
-  fill_u32(ctx, s, scratch, 0u, ((size_t)tiles + 2) * 2);
+  fill_u32(ctx, s, state, 0u, ((size_t)tiles + 2) * 2);
   const int sel = selectPositive ? 1 : 0;
   if (mode == 0)
     launch(ctx, s, "scan_lookback", k_scan<kScanWavesSum, kScanKSum, 0>, dim3(tiles), dim3(kScanWavesSum * 64), 0, in,
-           out, (uint32_t)n, sel, (uint64_t *)scratch, tiles);
+           out, (uint32_t)n, sel, (uint64_t *)state, tiles);
   else if (mode == 1)
     launch(ctx, s, "scan_lookback", k_scan<kScanWavesSel, kScanKSel, 1>, dim3(tiles), dim3(kScanWavesSel * 64), 0, in,
-           out, (uint32_t)n, sel, (uint64_t *)scratch, tiles);
+           out, (uint32_t)n, sel, (uint64_t *)state, tiles);
   else
     launch(ctx, s, "scan_lookback", k_scan<kScanWavesSel, kScanKSel, 2>, dim3(tiles), dim3(kScanWavesSel * 64), 0, in,
-           out, (uint32_t)n, sel, (uint64_t *)scratch, tiles);
+           out, (uint32_t)n, sel, (uint64_t *)state, tiles);
 }
 
 }  // namespace gprt

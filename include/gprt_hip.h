@@ -381,6 +381,19 @@ GPRT_API void gprtProposalGradients(GPRTContext context, GPRTProposal proposal, 
 /* classRoot[c]: treelet root (inner node index, or ~leaf position); primClass[prim]: its treelet. */
 GPRT_API void gprtProposalExportTreelets(GPRTContext context, GPRTProposal proposal, GPRTBuffer classRoot,
                                          GPRTBuffer primClass);
+/* Confidence gate of proposal-first traces: a ray whose top logit is below `threshold` is traced
+ * without its proposal (for such a ray the treelet walk more likely costs than saves node fetches).
+ * -INFINITY (the default) keeps the proposal of every ray, +INFINITY drops all. Hits never depend
+ * on it. gprtProposalInfer is not gated. */
+GPRT_API void gprtProposalSetGate(GPRTContext context, GPRTProposal proposal, float threshold);
+GPRT_API float gprtProposalGetGate(GPRTProposal proposal);
+/* Sets the gate from measured cost on a sample of rays (use rays of the workload's distribution,
+ * not the ones to be timed): each ray is traced plainly and proposal-first with its inner node
+ * fetches counted, and the threshold that minimises the sample's total fetches is kept (the empty
+ * choice costs nothing, so the gate never makes the sample dearer than the plain trace). Returns
+ * the share of the sample that keeps its proposal. */
+GPRT_API float gprtProposalCalibrateGate(GPRTContext context, GPRTProposal proposal, GPRTAccel accel, GPRTBuffer rays,
+                                         uint32_t count);
 /* Closest hit with an optional proposal (may be NULL) and optional per-ray count of inner nodes
  * fetched (visits, uint32, may be NULL). */
 GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext context, GPRTAccel accel, GPRTProposal proposal,
@@ -422,6 +435,40 @@ GPRT_API void gprtAccelTraceClosestHitDetail(GPRTContext context, GPRTAccel acce
 GPRT_API void gprtAccelExportLBVH(GPRTContext context, GPRTAccel accel, GPRTBuffer nodes, GPRTBuffer aabbs,
                                   GPRTBuffer keys);
 GPRT_API uint32_t gprtAccelGetPrimCount(GPRTAccel accel);
+
+/* ---------------------------------------------------------------------------------------------
+ * Development LBVH (gprt/development/gprt_lbvh.h:28-211, the reference's software LBVH over points,
+ * edges or triangles, whose results are plain buffers). After gprtLBVHBuild:
+ *   mortonCodes uint32[numPrims]   30-bit Morton codes of the key points, sorted (ties by primID)
+ *   ids         uint32[numPrims]   primIDs in that order
+ *   nodes       int4[2 numPrims-1] {left, right, parent, leaf primID or -1}; inner nodes [0, n-2]
+ *                                  with the root at 0, leaf i at n-1+i (gprt_lbvh.h:41-51)
+ *   aabbs       float3[2 (2 numPrims-1)]  {min, max} per node
+ * The key point and box of a primitive follow gprt/development/gprt_lbvh.hlsl: a point itself
+ * (:105-110, :228-243); an edge (uint2 into the vertices) its two end points and their midpoint
+ * (p1 + p2) / 2 (:112-119, :131-137, :245-260); a triangle (uint3) its vertices and centroid
+ * (p1 + p2 + p3) / 3 (:121-129, :139-146, :262-277). Vertices are float3 (12 B); the topology is
+ * the reference's Karras tree over (code, primID). Create allocates the four output buffers (the
+ * caller may read them with gprtBufferMap); Build enqueues the build on the context's stream;
+ * Destroy frees them. The reference defines GPRT_LBVH_EDGES and the edge kernels but no create
+ * function for them; gprtEdgesLBVHCreate is that missing one.
+ * ------------------------------------------------------------------------------------------- */
+typedef enum { GPRT_LBVH_POINTS = 1, GPRT_LBVH_EDGES = 2, GPRT_LBVH_TRIANGLES = 4 } GPRTLBVHType; /* :58-62 */
+typedef struct GPRTLBVH {                                                                         /* :64-85 */
+  GPRTLBVHType type;
+  uint32_t numPrims;
+  GPRTBuffer mortonCodes;
+  GPRTBuffer ids;
+  GPRTBuffer nodes;
+  GPRTBuffer aabbs;
+  void *impl; /* library state */
+} GPRTLBVH;
+GPRT_API GPRTLBVH gprtPointsLBVHCreate(GPRTContext context, GPRTBuffer vertices, uint32_t count); /* :88 */
+GPRT_API GPRTLBVH gprtEdgesLBVHCreate(GPRTContext context, GPRTBuffer vertices, GPRTBuffer edges, uint32_t count);
+GPRT_API GPRTLBVH gprtTriangleLBVHCreate(GPRTContext context, GPRTBuffer vertices, GPRTBuffer indices,
+                                         uint32_t count);                                         /* :125 */
+GPRT_API void gprtLBVHBuild(GPRTContext context, GPRTLBVH *lbvh);                                 /* :163 */
+GPRT_API void gprtLBVHDestroy(GPRTLBVH *lbvh);                                                    /* :197 */
 /* Root box {minX, minY, minZ, maxX, maxY, maxZ} of a built accel (for an instance accel, of its
  * top level in world space): the union of every primitive's box, as the build computed it. Waits
  * for the context's stream. The reference exposes no bounds query; this lets a caller check or

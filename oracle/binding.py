@@ -45,6 +45,7 @@ def lib() -> C.CDLL:
             "orc_refit": (None, [P, P, P, sz, P]),
             "orc_refit_bruteforce": (None, [P, P, P, sz, P]),
             "orc_lbvh_build": (None, [P, P, sz, P, P, P]),
+            "orc_lbvh_build_pe": (None, [P, P, sz, P, P, P]),
             "orc_intersect": (C.c_int, [P, P, P, P, P, P, P]),
             "orc_plane_t_fp64": (C.c_double, [P, P, P, P]),
             "orc_trace_bruteforce": (None, [P, sz, P, P, sz, P, C.c_int]),
@@ -156,6 +157,29 @@ def lbvh_build(verts, idx):
     nodes = np.zeros((2 * n - 1, 4), dtype=np.int32)
     aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
     lib().orc_lbvh_build(_p(verts), _p(idx), n, _p(keys), _p(nodes), _p(aabbs))
+    return keys, nodes, aabbs
+
+
+def lbvh_build_points(verts):
+    """Point LBVH: keys (morton << 32 | primID, sorted), nodes int32[2n-1, 4], aabbs float32[2n-1, 6]."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    n = len(verts)
+    keys = np.zeros(n, dtype=np.uint64)
+    nodes = np.zeros((2 * n - 1, 4), dtype=np.int32)
+    aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
+    lib().orc_lbvh_build_pe(_p(verts), None, n, _p(keys), _p(nodes), _p(aabbs))
+    return keys, nodes, aabbs
+
+
+def lbvh_build_edges(verts, edges):
+    """Edge LBVH over uint2 edges into float3 vertices (same outputs as lbvh_build_points)."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+    n = len(edges)
+    keys = np.zeros(n, dtype=np.uint64)
+    nodes = np.zeros((2 * n - 1, 4), dtype=np.int32)
+    aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
+    lib().orc_lbvh_build_pe(_p(verts), _p(edges), n, _p(keys), _p(nodes), _p(aabbs))
     return keys, nodes, aabbs
 
 

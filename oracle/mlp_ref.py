@@ -80,8 +80,17 @@ def forward(params: np.ndarray, X: np.ndarray, bf16: bool = True):
     return logits, acts, pres
 
 
-def loss_and_grad(params: np.ndarray, X: np.ndarray, labels: np.ndarray, bf16: bool = True):
-    """Mean softmax cross-entropy over the rows and its gradient in the flat layout (FP64)."""
+def loss_and_grad(params: np.ndarray, X: np.ndarray, labels: np.ndarray, bf16: bool = True,
+                  bf16_grads: bool = False, with_cond: bool = False):
+    """Mean softmax cross-entropy over the rows and its gradient in the flat layout (FP64).
+
+    bf16_grads: also round the backpropagated quantities where the device stores them in bf16
+    (the output-layer gradient dZ after scaling by 1/n, and each hidden layer's masked dA), so the
+    result differs from the device's only by FP32 accumulation order and the rare bf16 rounding
+    flips that order causes. with_cond: also return the elementwise condition sums of the same
+    products with every term taken in absolute value (|dZ|^T |H| for a weight gradient, column sums
+    of |dZ| for a bias gradient): an accumulation-order or rounding error of relative size e in
+    the terms changes an element by at most e times its condition sum."""
     params = np.asarray(params, dtype=np.float64)
     logits, acts, pres = forward(params, X, bf16)
     n = len(X)
@@ -94,20 +103,36 @@ def loss_and_grad(params: np.ndarray, X: np.ndarray, labels: np.ndarray, bf16: b
     dz /= n
     layers = unpack(params)
     rnd = bf16_round if bf16 else (lambda a: a)
+    grnd = bf16_round if bf16_grads else (lambda a: a)
     grad = np.zeros(PARAMS)
+    cond = np.zeros(PARAMS) if with_cond else None
     offs = offsets()
     # output layer
+    dz = grnd(dz)
     w0, shape, b0, bn = offs[3]
     grad[w0:w0 + shape[0] * shape[1]] = (dz.T @ acts[3]).ravel()
     grad[b0:b0 + bn] = dz.sum(0)
-    da = (dz @ rnd(layers[3][0])) * (pres[2] > 0)
+    if with_cond:
+        cond[w0:w0 + shape[0] * shape[1]] = (np.abs(dz).T @ np.abs(acts[3])).ravel()
+        cond[b0:b0 + bn] = np.abs(dz).sum(0)
+    da = grnd((dz @ rnd(layers[3][0])) * (pres[2] > 0))
     for layer in (2, 1, 0):
         w0, shape, b0, bn = offs[layer]
         grad[w0:w0 + shape[0] * shape[1]] = (da.T @ acts[layer]).ravel()
         grad[b0:b0 + bn] = da.sum(0)
+        if with_cond:
+            cond[w0:w0 + shape[0] * shape[1]] = (np.abs(da).T @ np.abs(acts[layer])).ravel()
+            cond[b0:b0 + bn] = np.abs(da).sum(0)
         if layer > 0:
-            da = (da @ rnd(layers[layer][0])) * (pres[layer - 1] > 0)
-    return loss, grad
+            da = grnd((da @ rnd(layers[layer][0])) * (pres[layer - 1] > 0))
+    return (loss, grad, cond) if with_cond else (loss, grad)
+
+
+def logit_cond(params: np.ndarray, acts) -> np.ndarray:
+    """Elementwise condition sums of the output layer, |H3| |W4|^T + |b4| (bf16 weights): an
+    error of relative size e in its terms changes a logit by at most e times this."""
+    W4, b4 = unpack(np.asarray(params, dtype=np.float64))[3]
+    return np.abs(acts[3]) @ np.abs(bf16_round(W4)).T + np.abs(b4)
 
 
 def adam_step(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):

@@ -17,6 +17,8 @@
 //   refit     gprt/development/gprt_lbvh.hlsl:400-428 (every leaf folds its box into every
 //             ancestor; order-free min/max).
 //   layout    gprt/development/gprt_lbvh.h:28-55 (int4 nodes, float3 pairs).
+//   points / edges  gprt/development/gprt_lbvh.hlsl:105-119, :131-137, :188-213, :228-260 (their
+//             boxes and key points), built by the same stages as triangles.
 //   trace     closest hit with the open interval tmin < t (gprt/gprt_fallbacks.slang:108,139);
 //             triangle test = watertight ray/triangle intersection (Woop, Benthin, Wald, JCGT
 //             2013), restated here independently of the device code.
@@ -343,6 +345,69 @@ void orc_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint64_t 
   orc_stable_sort(keysOut, nullptr, n, 64, 0);
   orc_karras(keysOut, n, nodes);
   orc_refit(verts, idx, nodes, n, aabbs);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Point and edge LBVHs (gprt/development/gprt_lbvh.h:58-62, :163-195 with GPRT_LBVH_POINTS /
+// GPRT_LBVH_EDGES). A point's box is the point and its key point the point itself
+// (gprt_lbvh.hlsl:105-110, :228-243); an edge (uint2 into the vertices) has the box of its two end
+// points and the key point (p1 + p2) / 2 (:112-119, :131-137, :245-260). Scene box over every
+// primitive box (:188-213), quantisation as orc_morton_keys, then the stages of orc_lbvh_build.
+// ------------------------------------------------------------------------------------------------
+static uint64_t key_of_point(const float c[3], const float *bounds, uint32_t prim) {
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    float pt = (c[a] - bounds[a]) / (bounds[3 + a] - bounds[a]);
+    pt = std::fmin(std::fmax(pt * 1024.f, 0.f), 1023.f);
+    q[a] = (uint32_t)pt;
+  }
+  return ((uint64_t)orc_morton3(q[0], q[1], q[2]) << 32) | (uint64_t)prim;
+}
+
+static void fold_point(float *b, const float *p) {
+  for (int a = 0; a < 3; ++a) {
+    b[a] = omin(b[a], p[a]);
+    b[3 + a] = omax(b[3 + a], p[a]);
+  }
+}
+
+// edges == nullptr: points (primitive i is vertex i).
+void orc_lbvh_build_pe(const float *verts, const uint32_t *edges, size_t n, uint64_t *keys, int32_t *nodes,
+                       float *aabbs) {
+  auto primBox = [&](uint32_t prim, float *b) {
+    for (int a = 0; a < 3; ++a) {
+      b[a] = INFINITY;
+      b[3 + a] = -INFINITY;
+    }
+    if (edges) {
+      fold_point(b, verts + 3 * (size_t)edges[2 * prim]);
+      fold_point(b, verts + 3 * (size_t)edges[2 * prim + 1]);
+    } else {
+      fold_point(b, verts + 3 * (size_t)prim);
+    }
+  };
+  float bounds[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = 0; i < n; ++i) {
+    float b[6];
+    primBox((uint32_t)i, b);
+    for (int a = 0; a < 3; ++a) {
+      bounds[a] = omin(bounds[a], b[a]);
+      bounds[3 + a] = omax(bounds[3 + a], b[3 + a]);
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    float c[3];
+    if (edges) {
+      const float *p1 = verts + 3 * (size_t)edges[2 * i], *p2 = verts + 3 * (size_t)edges[2 * i + 1];
+      for (int a = 0; a < 3; ++a) c[a] = (p1[a] + p2[a]) / 2.f;
+    } else {
+      for (int a = 0; a < 3; ++a) c[a] = verts[3 * i + a];
+    }
+    keys[i] = key_of_point(c, bounds, (uint32_t)i);
+  }
+  orc_stable_sort(keys, nullptr, n, 64, 0);
+  orc_karras(keys, n, nodes);
+  refit_with(nodes, n, aabbs, primBox);
 }
 
 // ------------------------------------------------------------------------------------------------

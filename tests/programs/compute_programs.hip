@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "gprt_hip_device.h"
+
 struct Float3 {
   float x, y, z;
 };
@@ -77,4 +79,49 @@ struct Full {
 extern "C" __global__ void Echo(Full f) {
   const uint32_t i = threadIdx.x;
   if (i < 62) f.out[i] = f.words[i];
+}
+
+// Closest hits of a ray batch traced inside the program against the accel whose
+// gprtAccelGetDeviceAddress is `accel` (a triangle or AABB accel).
+extern "C" __global__ void TraceRays(const void *accel, const GPRTRay *rays, GPRTHit *hits, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  hits[i] = gprtTraceClosestHit(accel, rays[i]);
+}
+
+// The ray-generation program of the reference's compute-bounds sample
+// (samples/s5-computePrograms/s5-0-computeAABBs/deviceCode.slang:30-55): a pinhole camera
+// {pos, dir_00, dir_du, dir_dv} (sharedCode.h), one ray per pixel through the pixel centre,
+// tmin 0, tmax 10000, traced against record.world. The sample turns the hit into a colour; this
+// one stores the hit and the ray it traced.
+struct Camera {
+  Float3 pos, dir_00, dir_du, dir_dv;
+};
+struct RayGenData {
+  const void *world;
+  GPRTHit *frameBuffer;
+  GPRTRay *rays;
+};
+
+extern "C" __global__ void SimpleRayGen(RayGenData record, Camera cam, uint32_t w, uint32_t h) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= w || y >= h) return;
+  const float sx = ((float)x + 0.5f) / (float)w, sy = ((float)y + 0.5f) / (float)h;
+  float d[3] = {cam.dir_00.x + sx * cam.dir_du.x + sy * cam.dir_dv.x,
+                cam.dir_00.y + sx * cam.dir_du.y + sy * cam.dir_dv.y,
+                cam.dir_00.z + sx * cam.dir_du.z + sy * cam.dir_dv.z};
+  const float inv = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  GPRTRay ray;
+  ray.ox = cam.pos.x;
+  ray.oy = cam.pos.y;
+  ray.oz = cam.pos.z;
+  ray.tmin = 0.0f;
+  ray.dx = d[0] * inv;
+  ray.dy = d[1] * inv;
+  ray.dz = d[2] * inv;
+  ray.tmax = 10000.0f;
+  const uint32_t px = y * w + x;
+  record.rays[px] = ray;
+  record.frameBuffer[px] = gprtTraceClosestHit(record.world, ray);
 }

@@ -28,16 +28,31 @@ def _check(acc, orc, v, idx):
     np.testing.assert_array_equal(aabbs.view(np.uint32), ra.view(np.uint32))
 
 
-@pytest.mark.parametrize("bits,tile", [(8, 4096), (6, 2048), (4, 8192), (8, 8192)])
-def test_batch_equals_oracle_per_mesh(ctx, orc, bits, tile):
-    ctx.lib.gprtSetSortConfig(ctx.h, bits, tile, 0)
+@pytest.fixture(scope="module")
+def batch_meshes(orc):
+    """The meshes with their oracle LBVHs (the same for every configuration)."""
+    return [(v, idx, orc.lbvh_build(v, idx)) for v, idx in _meshes(1000)]
+
+
+# Every configuration the autotuner may pick: digit width x tile x sort-pass occupancy cap.
+@pytest.mark.parametrize("bits", [8, 6, 4])
+@pytest.mark.parametrize("tile", [2048, 4096, 8192])
+@pytest.mark.parametrize("waves", [0, 8, 16])
+def test_batch_equals_oracle_per_mesh(ctx, orc, batch_meshes, bits, tile, waves):
+    ctx.lib.gprtSetSortConfig(ctx.h, bits, tile, waves)
     try:
-        meshes = _meshes(1000)
-        accs = [api.TriangleAccel(ctx, v, idx) for v, idx in meshes]
+        accs = [api.TriangleAccel(ctx, v, idx) for v, idx, _ in batch_meshes]
+        print(f"batch: {len(accs)} meshes, digits {bits}, tile {tile}, waves/CU {waves}", flush=True)
         api.build_batch(ctx, accs)
         assert api.last_error() is None, api.last_error()
-        for acc, (v, idx) in zip(accs, meshes):
-            _check(acc, orc, v, idx)
+        for acc, (v, idx, (rk, rn, ra)) in zip(accs, batch_meshes):
+            k, nodes, aabbs = acc.export()
+            np.testing.assert_array_equal(k, rk)
+            np.testing.assert_array_equal(nodes, rn)
+            np.testing.assert_array_equal(aabbs.view(np.uint32), ra.view(np.uint32))
+            lo = v[idx.ravel()].min(0)
+            hi = v[idx.ravel()].max(0)
+            np.testing.assert_array_equal(acc.bounds(), np.concatenate([lo, hi]))
         for a in accs:
             a.destroy()
     finally:

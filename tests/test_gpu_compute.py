@@ -191,3 +191,100 @@ def test_errors(ctx, module):
     p2.destroy()
     prog.destroy()
     out.destroy()
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Camera(C.Structure):  # samples/s5-computePrograms/s5-0-computeAABBs/sharedCode.h (camera)
+    _fields_ = [("pos", Float3), ("dir_00", Float3), ("dir_du", Float3), ("dir_dv", Float3)]
+
+
+class RayGenData(C.Structure):
+    _fields_ = [("world", C.c_void_p), ("frameBuffer", C.c_void_p), ("rays", C.c_void_p)]
+
+
+def _bits(h):
+    return h.view(np.uint32).reshape(-1, 4)
+
+
+def test_program_traces_aabb_accel_like_the_library(ctx, orc, module):
+    """The reference sample's whole flow inside programs: ComputeBounds writes the sphere boxes,
+    the AABB accel is built from them, and the sample's ray-generation program traces its camera
+    rays against the accel it names by gprtAccelGetDeviceAddress (include/gprt_hip_device.h).
+    Hits equal gprtAccelTraceClosestHit on the same rays bit for bit, and the oracle's brute
+    force over the boxes."""
+    n = 5000
+    rng = np.random.default_rng(7)
+    centres = rng.random((n, 3), dtype=np.float32)
+    radii = (rng.random(n, dtype=np.float32) * np.float32(0.02) + np.float32(2e-3)).astype(np.float32)
+    vb, rb = ctx.device_buffer(centres), ctx.device_buffer(radii)
+    acc = api.AABBAccel(ctx, np.zeros((n, 6), np.float32))
+    bounds = module.compute("ComputeBounds")
+    bounds.launch(((n + 63) // 64,), (64,), AABBGeomData(_dptr(vb), _dptr(rb), _dptr(acc.vbuf)), C.c_uint32(n))
+    acc.build()
+    world = ctx.lib.gprtAccelGetDeviceAddress(acc.h, 0)
+    w, h = 80, 60
+    fb, rays_buf = ctx.device_buffer(nbytes=16 * w * h), ctx.device_buffer(nbytes=32 * w * h)
+    cam = Camera(Float3(0.5, 0.5, -1.5), Float3(-0.4, -0.3, 1.0), Float3(0.8, 0.0, 0.0), Float3(0.0, 0.6, 0.0))
+    raygen = module.compute("SimpleRayGen")
+    print(f"device trace: {w}x{h} camera rays against {n} boxes", flush=True)
+    raygen.launch(((w + 15) // 16, (h + 15) // 16), (16, 16), RayGenData(world, _dptr(fb), _dptr(rays_buf)), cam,
+                  C.c_uint32(w), C.c_uint32(h))
+    assert api.last_error() is None, api.last_error()
+    got = fb.read(api.HIT_DTYPE)
+    lib_hits = ctx.device_buffer(nbytes=16 * w * h)
+    acc.trace(rays_buf, lib_hits, w * h)
+    want = lib_hits.read(api.HIT_DTYPE)
+    np.testing.assert_array_equal(_bits(got), _bits(want))
+    boxes = np.concatenate([centres - radii[:, None], centres + radii[:, None]], 1).astype(np.float32)
+    ref = orc.trace_boxes_bruteforce(rays_buf.read(scenes.RAY_DTYPE), boxes)
+    np.testing.assert_array_equal(got["primID"], ref["primID"])
+    np.testing.assert_array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
+    assert (got["primID"] >= 0).mean() > 0.2
+    for b in (vb, rb, fb, rays_buf, lib_hits):
+        b.destroy()
+    bounds.destroy()
+    raygen.destroy()
+    acc.destroy()
+
+
+def test_program_traces_triangle_accel_like_the_library(ctx, orc, module):
+    """A ray batch traced inside a program against a triangle accel equals the library's trace
+    bit for bit (t, primID, u, v); an accel not built yet, a destroyed accel and a null or
+    misaligned address give misses."""
+    verts, idx = scenes.blob_scene(200_000, seed=12)
+    acc = api.TriangleAccel(ctx, verts, idx)
+    rays = np.concatenate([scenes.pinhole_rays(128, 128), scenes.random_rays(30_000, 5)])
+    n = len(rays)
+    rb = ctx.device_buffer(rays)
+    hb, hb2 = ctx.device_buffer(nbytes=16 * n), ctx.device_buffer(nbytes=16 * n)
+    trace = module.compute("TraceRays")
+    addr = ctx.lib.gprtAccelGetDeviceAddress(acc.h, 0)
+
+    def program_trace(address):
+        trace.launch(((n + 255) // 256,), (256,), C.c_void_p(address), _dptr(rb), _dptr(hb), C.c_uint32(n))
+        return hb.read(api.HIT_DTYPE)
+
+    def all_miss(hits):
+        return (hits["primID"] == -1).all() and np.array_equal(hits["t"], rays["tmax"]) and (hits["u"] == 0).all()
+
+    assert all_miss(program_trace(addr))  # not built yet
+    acc.build()
+    print(f"device trace: {n} rays against {len(idx)} triangles", flush=True)
+    got = program_trace(addr)
+    acc.trace(rb, hb2, n)
+    want = hb2.read(api.HIT_DTYPE)
+    assert api.last_error() is None, api.last_error()
+    np.testing.assert_array_equal(_bits(got), _bits(want))
+    assert (got["primID"] >= 0).mean() > 0.02
+    sel = np.random.default_rng(1).choice(n, 256, replace=False)
+    ref = orc.trace_bruteforce(rays[sel], verts, idx)
+    np.testing.assert_array_equal(_bits(got[sel]), _bits(ref))
+    assert all_miss(program_trace(0)) and all_miss(program_trace(addr + 16))
+    acc.destroy()
+    assert all_miss(program_trace(addr))  # the slot outlives the accel, cleared
+    for b in (rb, hb, hb2):
+        b.destroy()
+    trace.destroy()

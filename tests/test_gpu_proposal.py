@@ -32,16 +32,73 @@ def _logits(ctx, prop, rays):
     return z
 
 
+# Bounds for results computed with bf16 operands and FP32 accumulation, against the reference
+# that rounds to bf16 at the same points (oracle/mlp_ref.py):
+#  * per element, 2^-6 of its condition sum (the same sum with every term in absolute value):
+#    FP32 accumulation order moves an element by at most ~K 2^-24 of it (K <= 4096: 2.4e-4), and
+#    a bf16 rounding flip that the order causes (an FP32 value within its error of a bf16
+#    rounding boundary) moves one stored term by at most one bf16 ulp, 2^-7 of itself; the bound
+#    leaves room for two such flips in one element;
+#  * per tensor, the relative Frobenius error at most 2^-12: flips are rare and unsigned, so they
+#    do not add up; a misplaced bias or operand column moves the whole tensor;
+#  * per output class, the mean error over the rays at most 2^-10 of the mean condition sum: a
+#    wrong bias or weight column shifts every ray the same way, flips average out.
+# Each bound is shown to reject the mistakes it is there for (planted in the reference, on the
+# CPU, below). A float32 emulation of the device's arithmetic on the CPU stays below 1e-7 (logits)
+# and 6e-3 (worst gradient element, 3e-5 relative norm) of these sums.
+ELEM_TOL = 2.0 ** -6
+NORM_TOL = 2.0 ** -12
+MEAN_TOL = 2.0 ** -10
+
+
+def _worst(got, ref, cond):
+    return float((np.abs(got.astype(np.float64) - ref) / (cond + 1e-30)).max())
+
+
+def _relnorm(got, ref):
+    return float(np.linalg.norm(got.astype(np.float64) - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def _colmean(got, ref, cond):
+    return float((np.abs((got.astype(np.float64) - ref).mean(0)) / (cond.mean(0) + 1e-30)).max())
+
+
+def _logits_ok(z, ref, cond):
+    return _worst(z, ref, cond) <= ELEM_TOL and _relnorm(z, ref) <= NORM_TOL and _colmean(z, ref, cond) <= MEAN_TOL
+
+
 def test_forward_matches_cpu(ctx):
     prop = api.Proposal(ctx, max_batch=512, seed=1)
     p = M.he_init(7).astype(np.float32)
     prop.set_params(p)
     rays = np.concatenate([scenes.random_rays(200, seed=3), scenes.pinhole_rays(8, 7)])
     z = _logits(ctx, prop, rays)
-    ref, _, _ = M.forward(p.astype(np.float64), M.encode(rays), bf16=True)
-    err = np.abs(z - ref).max()
-    assert err <= 0.02 * np.abs(ref).max() + 1e-3, (err, np.abs(ref).max())
+    p64 = p.astype(np.float64)
+    X = M.encode(rays)
+    ref, acts, _ = M.forward(p64, X, bf16=True)
+    cond = M.logit_cond(p64, acts)
+    print(f"logits: worst element {_worst(z, ref, cond):.3e} of its condition sum, relative norm "
+          f"{_relnorm(z, ref):.3e}, worst class mean {_colmean(z, ref, cond):.3e}", flush=True)
+    assert _worst(z, ref, cond) <= ELEM_TOL
+    assert _relnorm(z, ref) <= NORM_TOL
+    assert _colmean(z, ref, cond) <= MEAN_TOL
     assert np.mean(np.argmax(z, 1) == np.argmax(ref, 1)) > 0.95
+    # Mistakes planted in the reference fail the same checks: one output bias off by 0.05 ...
+    bad = ref.copy()
+    bad[:, 1234] += 0.05
+    assert not _logits_ok(z, bad, cond)
+    # ... two 16-class fragment columns of the output layer exchanged (rows 16..31 and 32..47 of
+    # W4t) ...
+    w0, shape, _, _ = M.offsets()[3]
+    q = p64.copy()
+    W4 = q[w0:w0 + shape[0] * shape[1]].reshape(shape)
+    W4[16:32], W4[32:48] = W4[32:48].copy(), W4[16:32].copy()
+    assert not _logits_ok(z, M.forward(q, X, bf16=True)[0], cond)
+    # ... and one hidden bias moved to the next lane.
+    q = p64.copy()
+    hb0 = M.offsets()[1][2]
+    q[hb0 + 5], q[hb0 + 6] = q[hb0 + 6], q[hb0 + 5]
+    assert not _logits_ok(z, M.forward(q, X, bf16=True)[0], cond)
     prop.destroy()
 
 
@@ -81,12 +138,26 @@ def test_gradients_match_cpu(ctx):
     g = np.zeros(prop.n_params, dtype=np.float32)
     ctx.lib.gprtProposalGradients(ctx.h, prop.h, rb.h, lb.h, len(rays), g.ctypes.data_as(C.c_void_p), len(g))
     assert api.last_error() is None, api.last_error()
-    _, ref = M.loss_and_grad(p.astype(np.float64), M.encode(rays), labels, bf16=True)
+    X = M.encode(rays)
+    _, ref, cond = M.loss_and_grad(p.astype(np.float64), X, labels, bf16=True, bf16_grads=True, with_cond=True)
+    tensors = []
     for layer, (w0, shape, b0, bn) in enumerate(M.offsets()):
-        for lo, hi in ((w0, w0 + shape[0] * shape[1]), (b0, b0 + bn)):
-            gg, rr = g[lo:hi].astype(np.float64), ref[lo:hi]
-            rel = np.linalg.norm(gg - rr) / max(np.linalg.norm(rr), 1e-12)
-            assert rel < 3e-2, (layer, lo, rel)
+        tensors += [(f"W{layer + 1}", slice(w0, w0 + shape[0] * shape[1])), (f"b{layer + 1}", slice(b0, b0 + bn))]
+    for name, sl in tensors:
+        worst, rel = _worst(g[sl], ref[sl], cond[sl]), _relnorm(g[sl], ref[sl])
+        print(f"{name}: worst element {worst:.3e} of its condition sum, relative norm {rel:.3e}", flush=True)
+        assert worst <= ELEM_TOL and rel <= NORM_TOL, (name, worst, rel)
+    # Mistakes planted in the reference fail: the output-layer bias gradient shifted by one lane,
+    # and a hidden layer's weight gradient with two 16-column fragments exchanged.
+    w0, shape, b0, bn = M.offsets()[3]
+    sb = slice(b0, b0 + bn)
+    bad = np.roll(ref[sb], 1)
+    assert _worst(g[sb], bad, cond[sb]) > ELEM_TOL or _relnorm(g[sb], bad) > NORM_TOL
+    w0, shape, _, _ = M.offsets()[1]
+    sw = slice(w0, w0 + shape[0] * shape[1])
+    bad = ref[sw].reshape(shape).copy()
+    bad[:, 0:16], bad[:, 16:32] = bad[:, 16:32].copy(), bad[:, 0:16].copy()
+    assert _worst(g[sw], bad.ravel(), cond[sw]) > ELEM_TOL or _relnorm(g[sw], bad.ravel()) > NORM_TOL
     rb.destroy()
     lb.destroy()
     prop.destroy()
@@ -280,3 +351,53 @@ def test_checkpoint_round_trip(ctx, scene):
         buf.destroy()
     a.destroy()
     b.destroy()
+
+
+def test_confidence_gate(ctx, scene):
+    """The confidence gate: with +inf no ray keeps its proposal (node fetches equal the plain
+    trace's), with -inf every ray does, and a gate calibrated on training rays keeps the sample's
+    total node fetches at most the plain trace's; hits are identical to the plain trace in every
+    case, and gprtProposalInfer is not gated."""
+    verts, idx, acc = scene
+    prop = api.Proposal(ctx, max_batch=1024, seed=41)
+    prop.bind(acc, 12)
+    tr = scenes.jittered_pinhole_rays(30000, seed=42)
+    th, _ = _trace(ctx, acc, tr)
+    tr = np.concatenate([tr, scenes.bounce_rays(tr, th, verts, idx, 15000, seed=43)])
+    th, _ = _trace(ctx, acc, tr)
+    rb, hb = ctx.device_buffer(tr), ctx.device_buffer(th)
+    prop.train(rb, hb, len(tr), epochs=6, lr=2e-3)
+    test = scenes.pinhole_rays(96, 96)
+    plain, v_plain = _trace(ctx, acc, test)
+    test = np.concatenate([test, scenes.bounce_rays(test, plain, verts, idx, 6000, seed=44)])
+    plain, v_plain = _trace(ctx, acc, test)
+    lib = ctx.lib
+    assert lib.gprtProposalGetGate(prop.h) == -np.inf
+    h_all, v_all = _trace(ctx, acc, test, prop)
+    _same(h_all, plain)
+    lib.gprtProposalSetGate(ctx.h, prop.h, np.inf)
+    h_none, v_none = _trace(ctx, acc, test, prop)
+    _same(h_none, plain)
+    np.testing.assert_array_equal(v_none, v_plain)
+    # gprtProposalInfer ignores the gate.
+    tb, cb = ctx.device_buffer(test), ctx.device_buffer(nbytes=4 * len(test))
+    lib.gprtProposalInfer(ctx.h, prop.h, tb.h, len(test), cb.h)
+    assert (cb.read(np.int32) >= 0).all()
+    # Calibrated on (a sample of) the training rays.
+    n_cal = 20000
+    keep = lib.gprtProposalCalibrateGate(ctx.h, prop.h, acc.h, rb.h, n_cal)
+    assert api.last_error() is None, api.last_error()
+    gate = lib.gprtProposalGetGate(prop.h)
+    assert 0.0 <= keep <= 1.0 and not np.isnan(gate)
+    _, vc_plain = _trace(ctx, acc, tr[:n_cal])
+    _, vc_gated = _trace(ctx, acc, tr[:n_cal], prop)
+    print(f"gate {gate:.3f} keeps {keep:.3f} of the calibration rays: node fetches plain {vc_plain.sum()} gated "
+          f"{vc_gated.sum()}; test rays: plain {v_plain.sum()}, all proposed {v_all.sum()}", flush=True)
+    # A ray's fetch count depends a little on the rays that share its wave (leaf tests are batched
+    # across the wave), so the gated total is compared with a 1% margin.
+    assert vc_gated.sum() <= 1.01 * vc_plain.sum()
+    h_gated, _ = _trace(ctx, acc, test, prop)
+    _same(h_gated, plain)
+    for b in (rb, hb, tb, cb):
+        b.destroy()
+    prop.destroy()

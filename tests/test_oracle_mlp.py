@@ -51,3 +51,23 @@ def test_adam_first_step_closed_form():
     np.testing.assert_allclose(w1, w - 0.01 * np.sign(g), rtol=0, atol=1e-7)
     np.testing.assert_allclose(m, 0.1 * g)
     np.testing.assert_allclose(v, 0.001 * g * g)
+
+
+def test_bf16_gradient_emulation_and_condition_sums():
+    """The bf16-rounded backward (the device's storage points) differs from the unrounded FP64
+    backward by a small but non-zero share of every tensor, and the condition sums bound
+    |gradient| and |logits|."""
+    p = M.he_init(4)
+    rays = scenes.random_rays(64, seed=3)
+    X = M.encode(rays)
+    labels = np.random.default_rng(2).integers(0, M.OUT, len(X))
+    _, exact = M.loss_and_grad(p, X, labels, bf16=True)
+    _, g, cond = M.loss_and_grad(p, X, labels, bf16=True, bf16_grads=True, with_cond=True)
+    assert np.all(np.abs(g) <= cond * (1 + 1e-12) + 1e-300)
+    # Rounding dZ and dA changes every gradient tensor, by about 0.1-0.3% of its norm.
+    for w0, shape, b0, bn in M.offsets():
+        for sl in (slice(w0, w0 + shape[0] * shape[1]), slice(b0, b0 + bn)):
+            rel = np.linalg.norm(g[sl] - exact[sl]) / np.linalg.norm(exact[sl])
+            assert 1e-4 < rel < 1e-2, rel
+    z, acts, _ = M.forward(p, X, bf16=True)
+    assert np.all(np.abs(z) <= M.logit_cond(p, acts) * (1 + 1e-12))
