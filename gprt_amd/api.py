@@ -99,6 +99,7 @@ _SIGS = {
     "gprtBufferResize": (None, [_P, _P, C.c_size_t, C.c_size_t, C.c_bool, C.c_int]),
     "gprtBufferCopy": (None, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int]),
     "gprtBufferExclusiveSum": (C.c_uint32, [_P, _P, _P, _P]),
+    "gprtBufferExclusiveSumNoAlloc": (C.c_uint32, [_P, _P, _P]),
     "gprtBufferPartition": (C.c_uint32, [_P, _P, C.c_bool, _P, _P]),
     "gprtBufferSelect": (C.c_uint32, [_P, _P, C.c_bool, _P, _P]),
     "gprtBufferSort": (None, [_P, _P, _P]),

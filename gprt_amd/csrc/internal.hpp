@@ -248,6 +248,11 @@ inline void launch(Context *ctx, hipStream_t s, const char *name, K kernel, dim3
 // ------------------------------------------------------------------------------------------------
 // Scan family: mode 0 = exclusive sum, 1 = partition, 2 = select.
 size_t scan_scratch_bytes(size_t n);
+// The library's static look-back state for gprtBufferExclusiveSumNoAlloc: looked up once per
+// device (with that device current) by gprtContextCreate; its device address per device.
+bool scan_noalloc_init(int device);
+void *scan_noalloc_state(int device);
+size_t scan_noalloc_max_items();
 void scan_run(Context *ctx, hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, int mode, bool selectPositive,
               void *scratch, uint32_t *totalDev);
 

@@ -254,6 +254,7 @@ GPRT_API GPRTContext gprtContextCreate(int32_t *requestedDeviceIDs, int numDevic
     return nullptr;
   }
   if (!hip_ok(hipSetDevice(dev), "hipSetDevice")) return nullptr;
+  if (!scan_noalloc_init(dev)) return nullptr;
   Context *ctx = new Context();
   ctx->device = dev;
   hipDeviceProp_t prop;
@@ -434,6 +435,41 @@ static uint32_t buffer_scan(GPRTContext _ctx, GPRTBuffer _in, GPRTBuffer _out, G
   uint32_t *totalDev = (uint32_t *)sp;
   scan_run(ctx, ctx->stream, (const uint32_t *)in->d, (uint32_t *)out->d, n, mode, selectPositive, sp, totalDev);
   if (!hip_ok(hipMemcpyAsync(ctx->pinnedWord, totalDev, 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync"))
+    return (uint32_t)-1;
+  if (!stream_wait(ctx, ctx->stream, fn)) return (uint32_t)-1;
+  return *ctx->pinnedWord;
+}
+
+// gprtBufferExclusiveSum with the library's static state of the context's device instead of a
+// scratch buffer: no allocation of any kind (the total comes back through the context's pinned
+// word, allocated with the context). The state is cleared by a fill kernel on every call.
+GPRT_API uint32_t gprtBufferExclusiveSumNoAlloc(GPRTContext _ctx, GPRTBuffer _in, GPRTBuffer _out) {
+  const StageRange range("gprtBufferExclusiveSumNoAlloc");
+  const char *fn = "gprtBufferExclusiveSumNoAlloc";
+  Context *ctx = (Context *)_ctx;
+  Buffer *in = (Buffer *)_in, *out = (Buffer *)_out;
+  if (!ctx || !in || !out) {
+    fail("%s: null argument", fn);
+    return (uint32_t)-1;
+  }
+  const size_t n = in->size / 4;
+  if (n > scan_noalloc_max_items()) {
+    fail("%s: %zu items exceed the %zu-item limit of the library's static scan state (use gprtBufferExclusiveSum)",
+         fn, n, scan_noalloc_max_items());
+    return (uint32_t)-1;
+  }
+  if (out->size < n * 4) {
+    fail("%s: output holds %zu bytes, needs %zu", fn, out->size, n * 4);
+    return (uint32_t)-1;
+  }
+  if (n == 0) return 0;
+  void *state = scan_noalloc_state(ctx->device);
+  if (!state) {
+    fail("%s: no static scan state on device %d", fn, ctx->device);
+    return (uint32_t)-1;
+  }
+  scan_run(ctx, ctx->stream, (const uint32_t *)in->d, (uint32_t *)out->d, n, 0, false, state, nullptr);
+  if (!hip_ok(hipMemcpyAsync(ctx->pinnedWord, state, 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync"))
     return (uint32_t)-1;
   if (!stream_wait(ctx, ctx->stream, fn)) return (uint32_t)-1;
   return *ctx->pinnedWord;

@@ -15,6 +15,9 @@
 //  * Status words are 64-bit {flag:32 | value:32}: values wrap modulo 2^32 with no 2^30 limit
 //    (the reference packs value<<2|flag into 32 bits, gprt/gprt_scan.slang:223-228).
 //  * State layout (uint64 words): [0] total (low 32 bits), [1] tile ticket, [2 + t] tile t status.
+#include <mutex>
+#include <vector>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -169,7 +172,39 @@ constexpr int kScanKSum = 8, kScanKSel = 4;
 constexpr uint32_t kScanTileSum = kScanWavesSum * 64 * kScanKSum * 4;
 constexpr uint32_t kScanTileSel = kScanWavesSel * 64 * kScanKSel * 4;
 
+// The look-back state of gprtBufferExclusiveSumNoAlloc: one device-resident array per GPU that
+// the library runs on, held until the process ends, so that the call allocates nothing. It holds
+// the state of one exclusive sum of up to kNoAllocMaxItems items: [0] total, [1] tile ticket,
+// [2 + t] the status of tile t. Host code never passes `scratch` itself to a kernel (in host code
+// the name stands for a host-side shadow); it uses the device address hipGetSymbolAddress returns
+// for the current device, kept per device in g_noAllocState.
+constexpr size_t kNoAllocMaxItems = 1ull << 31;
+constexpr size_t kNoAllocWords = kNoAllocMaxItems / kScanTileSum + 2;  // 1 MiB + 16 B
+
+std::mutex g_noAllocMutex;
+std::vector<void *> g_noAllocState;  // device address of `scratch` per device ordinal
+
 }  // namespace
+
+static __device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+bool scan_noalloc_init(int device) {
+  std::lock_guard<std::mutex> lock(g_noAllocMutex);
+  if (device < 0) return false;
+  if ((size_t)device < g_noAllocState.size() && g_noAllocState[device]) return true;
+  void *p = nullptr;
+  if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scan state)") || !p) return false;
+  if (g_noAllocState.size() <= (size_t)device) g_noAllocState.resize((size_t)device + 1, nullptr);
+  g_noAllocState[device] = p;
+  return true;
+}
+
+void *scan_noalloc_state(int device) {
+  std::lock_guard<std::mutex> lock(g_noAllocMutex);
+  return (device >= 0 && (size_t)device < g_noAllocState.size()) ? g_noAllocState[device] : nullptr;
+}
+
+size_t scan_noalloc_max_items() { return kNoAllocMaxItems; }
 
 size_t scan_scratch_bytes(size_t n) {
   size_t tiles = (n + kScanTileSel - 1) / kScanTileSel;  // the smaller tile: enough for either

@@ -284,6 +284,16 @@ GPRT_API void gprtClearLastError(void);
 GPRT_API void gprtBufferSortKV(GPRTContext context, GPRTBuffer keys, GPRTBuffer values, uint32_t count,
                                uint32_t keyBits, uint32_t valueBits, GPRTBuffer scratch);
 
+/* No counterpart in the reference: gprtBufferExclusiveSum for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). Same result
+ * and returned total as gprtBufferExclusiveSum(context, input, output, scratch); instead of a
+ * scratch buffer its look-back state is one static device array per GPU, held until the process
+ * ends (1 MiB), so the call makes no allocation of any kind. Input is at most 2^31 items
+ * (8 GiB); larger inputs fail through the error path. Calls on one GPU share that array: calls
+ * that overlap on one GPU (two contexts' streams, or two host threads) race and must be ordered
+ * by the caller. Waits for the context's stream to return the total, as the original does. */
+GPRT_API uint32_t gprtBufferExclusiveSumNoAlloc(GPRTContext context, GPRTBuffer input, GPRTBuffer output);
+
 /* Radix sort knobs used by every sort and accel build on this context (the autotuner's search
  * space): digitBits in {4, 6, 8}, tileKeys (keys per workgroup tile) in {2048, 4096, 8192},
  * wavesPerCU (0 = no cap) caps the resident waves of the accel builds' sort passes through LDS
