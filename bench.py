@@ -175,25 +175,53 @@ def scene_workload(args):
     t0 = time.time()
     loss, train_acc = prop.train(trb, thb, len(tr), epochs=args.epochs, lr=args.lr)
     train_s = time.time() - t0
+    # Confidence gate: calibrated on a sample of the training rays (not the timed ones) from the
+    # node fetches each ray costs plainly and proposal-first.
+    t0 = time.time()
+    gate_keep = float(lib.gprtProposalCalibrateGate(ctx.h, prop.h, acc.h, trb.h, min(args.gate_rays, len(tr))))
+    gate = float(lib.gprtProposalGetGate(prop.h))
+    gate_s = time.time() - t0
     trb.destroy()
     thb.destroy()
     log(f"proposal: {classes} treelets; trained on {len(tr)} rays x {args.epochs} epochs in {train_s:.1f}s "
-        f"(data {gen_s:.1f}s): loss {loss:.3f}, train top-1 {train_acc:.3f}")
+        f"(data {gen_s:.1f}s): loss {loss:.3f}, train top-1 {train_acc:.3f}; gate {gate:.3f} keeps "
+        f"{gate_keep:.3f} of {min(args.gate_rays, len(tr))} calibration rays ({gate_s:.1f}s)")
 
     rb = ctx.device_buffer(rays)
     hb = ctx.device_buffer(nbytes=n_rays * 16)
     hb2 = ctx.device_buffer(nbytes=n_rays * 16)
+
+    def learned_trace():
+        """The learned trace's device time (inference + proposal-first traversal, however they
+        overlap), between two events on the context's stream."""
+        lib.gprtBeginProfile(ctx.h)
+        lib.gprtAccelTraceClosestHitEx(ctx.h, acc.h, prop.h, rb.h, hb2.h, n_rays, None)
+        return float(lib.gprtEndProfile(ctx.h))
+
+    # Overlap of inference and traversal: a short sweep of chunk counts and traversal workgroups
+    # per CU (results do not depend on it); the fastest is used for the timed steps.
+    overlap_sweep = {}
+    for chunks, blocks in [(0, 0)] + [(c, b) for c in args.overlap_chunks for b in args.overlap_blocks]:
+        lib.gprtProposalSetOverlap(ctx.h, prop.h, chunks, blocks)
+        learned_trace()
+        overlap_sweep[f"{chunks}x{blocks}"] = min(learned_trace() for _ in range(2))
+    best_overlap = min(overlap_sweep, key=overlap_sweep.get)
+    oc, ob = (int(x) for x in best_overlap.split("x"))
+    lib.gprtProposalSetOverlap(ctx.h, prop.h, oc, ob)
+    log(f"overlap sweep (learned trace ms): {overlap_sweep}; using {best_overlap}")
+    learned_wall = []
 
     def step():
         # The rebuild is deterministic (same keys, same tree), so the proposal's treelet cut stays
         # valid; after a geometry change a stale cut would cost speed, never correctness.
         acc.build()
         acc.trace(rb, hb, n_rays)
-        lib.gprtAccelTraceClosestHitEx(ctx.h, acc.h, prop.h, rb.h, hb2.h, n_rays, None)
+        learned_wall.append(learned_trace())
 
     for _ in range(args.warmup):
         step()
     ctx.sync()
+    learned_wall.clear()
     lib.gprtKernelTimerReset(ctx.h)
     lib.gprtKernelTimerEnable(ctx.h, True)
     ctx.sync()
@@ -213,10 +241,12 @@ def scene_workload(args):
     trace_ms_tot, trace_n = ms_of(per_kernel, "trace_closest")
     plain_ms = (trace_ms_tot - ms_of(per_kernel, "trace_closest_proposal")[0]) / K
     prop_trace_ms = ms_of(per_kernel, "trace_closest_proposal")[0] / K
-    # Inference = the hidden-layer kernel + the fused output-layer / argmax kernel.
+    # Inference = the fused encoding / hidden / output-layer / argmax kernel (per step, summed over
+    # its chunks); the learned trace's time is its device wall time (inference and traversal may
+    # overlap).
     infer_ms_tot, _ = ms_of(per_kernel, "mlp_infer")
     infer_ms = infer_ms_tot / K
-    learned_ms = infer_ms + prop_trace_ms
+    learned_ms = float(np.median(learned_wall))
     h1, h2 = hb.read(api.HIT_DTYPE), hb2.read(api.HIT_DTYPE)
     mismatches = int((h1.view(np.uint32).reshape(-1, 4) != h2.view(np.uint32).reshape(-1, 4)).any(1).sum())
     hit_rate = float((h1["primID"] >= 0).mean())
@@ -233,15 +263,14 @@ def scene_workload(args):
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": LBVH_BYTES_PER_PRIM,
                   "traffic": TRAFFIC_BYTES["build_per_prim"] * N_TRIS}
     roof_build["frac"] = roof_build["achieved"] / HBM_PEAK_GBS
-    out_ms_tot, out_n = ms_of(per_kernel, "mlp_infer_top1")
+    out_ms_tot, out_n = ms_of(per_kernel, "mlp_infer_fused")
     out_avg = out_ms_tot / max(out_n, 1)
-    out_rays = n_rays * K / max(out_n, 1)  # rays per launch (large batches run in rounds)
-    roof_mlp = {"kernel": "mlp_infer_top1 (output layer + argmax)", "bound": "mfma",
-                "achieved": out_rays * mlp_out_flop_per_ray(classes) / (out_avg * 1e-3) / 1e12,
+    out_rays = n_rays * K / max(out_n, 1)  # rays per launch
+    roof_mlp = {"kernel": "mlp_infer_fused (encoding, hidden layers, output layer + argmax)", "bound": "mfma",
+                "achieved": out_rays * mlp_flop_per_ray(classes) / (out_avg * 1e-3) / 1e12,
                 "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "avg_launch_ms": out_avg,
-                "flop_per_ray": mlp_out_flop_per_ray(classes), "classes": classes,
-                "rays_per_launch": out_rays, "traffic": TRAFFIC_BYTES["mlp_infer_top1"],
-                "whole_inference_tflops": n_rays * mlp_flop_per_ray(classes) / (infer_ms * 1e-3) / 1e12}
+                "flop_per_ray": mlp_flop_per_ray(classes), "classes": classes,
+                "rays_per_launch": out_rays, "traffic": None}
     roof_mlp["frac"] = roof_mlp["achieved"] / BF16_PEAK_TFLOPS
 
     from oracle import binding as orc
@@ -318,9 +347,12 @@ def scene_workload(args):
                        "trace_ms_plain": plain_ms, "trace_mrays_s_plain": n_rays / (plain_ms * 1e3),
                        "trace_ms_learned": learned_ms, "trace_mrays_s_learned": n_rays / (learned_ms * 1e3),
                        "learned_infer_ms": infer_ms, "learned_traverse_ms": prop_trace_ms,
+                       "learned_serial_sum_ms": infer_ms + prop_trace_ms, "overlap": best_overlap,
+                       "overlap_sweep_ms": overlap_sweep,
                        "learned_vs_plain": plain_ms / learned_ms, "treelet_cut_ms": cut_ms,
                        "hit_mismatches_learned_vs_plain": mismatches, "hit_rate": hit_rate,
-                       "train_s": train_s, "train_loss": loss, "train_top1": train_acc},
+                       "train_s": train_s, "train_loss": loss, "train_top1": train_acc,
+                       "gate": gate, "gate_keep_fraction": gate_keep},
         # The roofline of the kernel that takes the most time in the step.
         "roofline": dominant,
         "roofline_build": roof_build,
@@ -520,8 +552,8 @@ def dynamic_workload(args):
     pct = lambda a, q: float(np.percentile(np.asarray(a), q))
     rays = ds.rays_per_frame
     # Node fetches per ray of the last frame's plain traces (untimed), for the traversal's
-    # algorithmic bytes: 32 B ray + 16 B hit + 64 B per inner node (top and instance levels);
-    # leaf records are not counted, so this is a lower bound.
+    # algorithmic bytes: 32 B ray + 8 B hit + 32 B per inner node visited (top and instance
+    # levels); leaf tests are not counted, so this is a lower bound.
     vis = ctx.device_buffer(nbytes=4 * n)
     api.trace_detail(ctx, ds.tlas, ds.pb, ds.hp, n, visits_buf=vis)
     v1 = float(vis.read(np.uint32).mean())
@@ -530,9 +562,9 @@ def dynamic_workload(args):
     vis.destroy()
     visits = (v1 + v2) / 2
     launch_ms = pct(tplain, 50) / 2  # two traversal launches per frame (primary, bounce)
-    roof = _roof_hbm("trace_closest_instances", n * (48 + 64 * visits), launch_ms, bytes_per_ray=48 + 64 * visits,
-                     nodes_per_ray=visits, note="issue- and L2-bound like the scene traversal; leaf records "
-                                               "not counted")
+    roof = _roof_hbm("trace_closest_instances", n * (40 + 32 * visits), launch_ms, bytes_per_ray=40 + 32 * visits,
+                     nodes_per_ray=visits, fetched_bytes_per_ray=48 + 64 * visits,
+                     note="issue- and L2-bound like the scene traversal; leaf tests not counted")
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc
@@ -1078,6 +1110,11 @@ def main():
     ap.add_argument("--cpu-build-tris", type=int, default=1 << 20, help="triangles of the 1-thread CPU build sample")
     ap.add_argument("--train-rays", type=int, default=16 << 20, help="proposal training rays (16,777,216: C4)")
     ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--gate-rays", type=int, default=262144, help="training rays the proposal gate is calibrated on")
+    ap.add_argument("--overlap-chunks", type=int, nargs="*", default=[4, 8, 16],
+                    help="scene: chunk counts of the overlapped learned trace to sweep")
+    ap.add_argument("--overlap-blocks", type=int, nargs="*", default=[6, 8, 10],
+                    help="scene: traversal workgroups per CU of the overlapped learned trace to sweep")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--meshes", type=int, default=4096)

@@ -149,6 +149,7 @@ _SIGS = {
     "gprtProposalGradients": (None, [_P, _P, _P, _P, C.c_uint32, _P, C.c_size_t]),
     "gprtProposalExportTreelets": (None, [_P, _P, _P, _P]),
     "gprtProposalSetGate": (None, [_P, _P, C.c_float]),
+    "gprtProposalSetOverlap": (None, [_P, _P, C.c_uint32, C.c_uint32]),
     "gprtProposalGetGate": (C.c_float, [_P]),
     "gprtProposalCalibrateGate": (C.c_float, [_P, _P, _P, _P, C.c_uint32]),
     "gprtAccelTraceClosestHitEx": (None, [_P, _P, _P, _P, _P, C.c_uint32, _P]),

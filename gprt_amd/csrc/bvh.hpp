@@ -38,6 +38,19 @@ using gprt_device::ray_setup;
 using gprt_device::RayPre;
 using gprt_device::tri_hit;
 
+// Rays and hits are read and written once per trace: streaming (non-temporal) accesses keep them
+// from displacing the tree, which is reused by every ray, from the caches.
+typedef float stream_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ GPRTRay load_ray_stream(const GPRTRay *p) {
+  const stream_f4 a = __builtin_nontemporal_load((const stream_f4 *)p);
+  const stream_f4 b = __builtin_nontemporal_load((const stream_f4 *)p + 1);
+  return GPRTRay{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+__device__ __forceinline__ void store_hit_stream(GPRTHit *p, const GPRTHit &h) {
+  const stream_f4 v = {h.t, __int_as_float(h.primID), h.u, h.v};
+  __builtin_nontemporal_store(v, (stream_f4 *)p);
+}
+
 // Header words (float): [0..5] scene box as ordered uints during the build, then floats;
 // [8..13] root box; [15] reserved.
 constexpr int kHeaderFloats = 16;

@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(kTraceBlock)
     if (base >= count) break;
     const uint32_t ri = base + (uint32_t)l;
     if (ri < count) {
-      const GPRTRay ray = rays[ri];
+      const GPRTRay ray = load_ray_stream(rays + ri);
       const RayPre rw = ray_setup(ray.ox, ray.oy, ray.oz, ray.tmin, ray.dx, ray.dy, ray.dz);
       HitState h;
       hit_init(h, ray.tmax);
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(kTraceBlock)
       o.primID = h.prim;
       o.u = h.u;
       o.v = h.v;
-      hits[ri] = o;
+      store_hit_stream(hits + ri, o);
       if (instOut) instOut[ri] = h.prim >= 0 ? (int32_t)h.inst : -1;
       if (leafOut) leafOut[ri] = h.prim >= 0 ? h.leaf : -1;
       if (visitsOut) visitsOut[ri] = visits;
@@ -739,12 +739,14 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
 }
 
 bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
-                const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits) {
+                const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits,
+                unsigned blocksPerCU) {
   if (count == 0) return true;
   Tlas *t = a->tlas;
   uint32_t *work = ensure_trace_scratch(ctx, s, kMaxStack - kLdsStack);
   if (!work) return false;
-  const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
+  const unsigned blocks = (unsigned)ctx->cuCount * (blocksPerCU ? std::min<unsigned>(blocksPerCU, kBlocksPerCU)
+                                                               : (unsigned)kBlocksPerCU);
   fill_u32(ctx, s, work, 0u, 1);
   // The proposal step walks the instances' own triangle accels, the top-level walk the copies
   // taken at this accel's build: once any triangle accel has changed since then, the proposal is

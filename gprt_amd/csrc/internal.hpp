@@ -97,6 +97,12 @@ struct Context {
   // triangle accel's live arrays is valid only while this is unchanged since that build.
   uint64_t blasContentEpoch = 0;
   std::vector<struct Accel *> slotAccel;  // host view: pool slot -> live triangle or AABB accel
+  // Second in-order stream (default priority) on which proposal inference runs ahead of the
+  // traversal it feeds, and the ordering events between the two (created on first use).
+  hipStream_t aux = nullptr;
+  std::vector<hipEvent_t> orderEvents;
+  hipStream_t aux_stream();
+  hipEvent_t order_event(size_t i);
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -282,7 +288,8 @@ uint32_t tlas_leaves(const Accel *a);
 uint32_t tlas_classes(const Accel *a);
 const int32_t *tlas_leaf_class(const Accel *a);
 bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
-                const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits);
+                const int32_t *proposal, uint32_t numClasses, int32_t *instOut, int32_t *leafOut, uint32_t *visits,
+                unsigned blocksPerCU = 0);
 
 // LBVH (segmented: one store may hold many meshes)
 void store_release(BVHStore *st);
@@ -300,7 +307,10 @@ void lbvh_parents(Context *ctx, hipStream_t s, Accel *a);
 // Trace
 bool trace_closest(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count);
 // proposal / classRoot / visits may be null. proposal[i] = treelet class of ray i (or -1).
+// blocksPerCU: persistent workgroups per CU (0: the default); fewer leave room for kernels of
+// another stream on the same CUs.
 bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
-                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits);
+                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits,
+                      unsigned blocksPerCU = 0);
 
 }  // namespace gprt

@@ -418,29 +418,31 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ w, co
 }
 
 // ------------------------------------------------------------------------------------------------
-// Inference in two kernels. k_infer_hidden: one wave per 64 rays encodes them and runs the three
-// 128x128 hidden layers (activations in 17 KB of LDS, weight fragments read from L2, so eight
-// waves share a CU), writing the bf16 layer-3 activations (256 B per ray). k_infer_top1: the
-// 128x4096 output layer with a running argmax over 256 rays per workgroup, never writing the
-// logits; 80 KB of LDS and at most 256 registers let two workgroups (two waves per SIMD) share a
-// CU and hide each other's stalls.
+// Inference in one kernel, k_infer_top1, 256 rays per workgroup (64 per wave). Each wave encodes
+// its rays and runs the three 128x128 hidden layers over a 64 x 128 activation tile in LDS (weight
+// fragments read from L2); the layer-3 activations then move to registers and the 128x4096 output
+// layer runs with a running argmax, W4t streamed through a two-chunk LDS ring that reuses the
+// activation tiles' space. Neither the activations nor the logits are written to memory: per ray
+// the kernel reads its 32-byte ray and writes a class (and optionally the winning logit). 80 KB of
+// LDS and at most 256 registers let two workgroups (two waves per SIMD) share a CU and hide each
+// other's stalls.
 // ------------------------------------------------------------------------------------------------
 constexpr int kInferRaysPerWave = 64;
 constexpr int kInferRays = 4 * kInferRaysPerWave;
 constexpr int kActStride = 136;  // bf16 per activation row in LDS (128 + 8 pad)
 
-__global__ void __launch_bounds__(64) k_infer_hidden(const GPRTRay *__restrict__ rays, uint32_t count,
-                                                     const uint16_t *__restrict__ W1t,
-                                                     const uint16_t *__restrict__ W2t,
-                                                     const uint16_t *__restrict__ W3t,
-                                                     const float *__restrict__ bias, uint16_t *__restrict__ h3) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[kInferRaysPerWave * kActStride];
-  const int l = threadIdx.x;
-  const uint32_t ri = blockIdx.x * kInferRaysPerWave + l;
+// One wave: encodes its 64 rays (rays [ray0, ray0 + 64), zero features past `count`) into its
+// 64 x 128 activation tile `act` in LDS and runs the three 128x128 hidden layers over it; the
+// bf16 layer-3 activations are left in `act`. Weight fragments are read from L2.
+__device__ __forceinline__ void hidden_layers(const GPRTRay *__restrict__ rays, uint32_t count, uint32_t ray0,
+                                              uint16_t *act, const uint16_t *__restrict__ W1t,
+                                              const uint16_t *__restrict__ W2t, const uint16_t *__restrict__ W3t,
+                                              const float *__restrict__ bias, int l) {
+  const uint32_t ri = ray0 + l;
   {
     uint32_t wds[kMlpInPad / 2];
     if (ri < count) {
-      encode_ray(rays[ri], wds);
+      encode_ray(load_ray_stream(rays + ri), wds);
     } else {
 #pragma unroll
       for (int i = 0; i < kMlpInPad / 2; ++i) wds[i] = 0u;
@@ -485,28 +487,31 @@ __global__ void __launch_bounds__(64) k_infer_hidden(const GPRTRay *__restrict__
     }
     __builtin_amdgcn_wave_barrier();
   }
-  if (ri < count) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) *(uint4 *)&h3[(size_t)ri * 128 + q * 8] = *(const uint4 *)&act[l * kActStride + q * 8];
-  }
 }
 
+// The hidden layers run first (each wave over its 64 rays, its activation tile in the LDS that the
+// W4t ring takes afterwards), so the layer-3 activations go from LDS straight to registers.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_infer_top1(uint32_t count, const uint16_t *__restrict__ W4t, const float *__restrict__ bias,
-                 const uint16_t *__restrict__ h3, int32_t *__restrict__ outClass, float *__restrict__ outLogit,
-                 uint32_t numClasses, float gate) {
+                 int32_t *__restrict__ outClass, float *__restrict__ outLogit, uint32_t numClasses, float gate,
+                 const GPRTRay *__restrict__ rays, const uint16_t *__restrict__ W1t,
+                 const uint16_t *__restrict__ W2t, const uint16_t *__restrict__ W3t) {
   extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const uint32_t ray0 = blockIdx.x * kInferRays;
   // A fragments (layer-3 activations of this wave's 64 rays) stay in registers.
   Frag a[4][4];
+  {
+    static_assert(4 * kInferRaysPerWave * kActStride * 2 <= 2 * 128 * 256 + kMlpOut * 4,
+                  "the activation tiles fit in the ring and bias area");
+    uint16_t *act = smem + (size_t)w * kInferRaysPerWave * kActStride;
+    hidden_layers(rays, count, ray0 + w * kInferRaysPerWave, act, W1t, W2t, W3t, bias, l);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint32_t ri = ray0 + w * kInferRaysPerWave + i * 16 + (l & 15);
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-      a[i][ks].u = ri < count ? *(const uint4 *)&h3[(size_t)ri * 128 + ks * 32 + 8 * (l >> 4)]
-                              : make_uint4(0u, 0u, 0u, 0u);
+      for (int ks = 0; ks < 4; ++ks)
+        a[i][ks].u = *(const uint4 *)&act[(i * 16 + (l & 15)) * kActStride + ks * 32 + 8 * (l >> 4)];
+    __syncthreads();  // every wave has its fragments: the ring and the bias take the LDS
   }
   float best[4][4];
 #pragma unroll
@@ -831,9 +836,7 @@ bool mlp_alloc(Context *ctx, Proposal *p, uint32_t maxBatch) {
 void mlp_release(Proposal *p) {
   if (p->block) hipFree(p->block);
   if (p->treeBlock) hipFree(p->treeBlock);
-  if (p->h3) hipFree(p->h3);
-  p->block = p->treeBlock = p->h3 = nullptr;
-  p->h3Bytes = 0;
+  p->block = p->treeBlock = nullptr;
 }
 
 // Copies fp32 parameters to the device and refreshes the bf16 weights and the bias vector.
@@ -958,29 +961,10 @@ void mlp_labels(Context *ctx, hipStream_t s, Proposal *p, const void *hits, uint
 void mlp_infer(Context *ctx, hipStream_t s, Proposal *p, const void *rays, uint32_t n, int32_t *cls, float *logit,
                float gate) {
   const uint32_t classes = p->numClasses ? p->numClasses : (uint32_t)kMlpOut;
-  // Two kernels through a bf16 activation buffer of at most 2^22 rays (1 GiB) per round.
-  const uint32_t chunk = std::min<uint32_t>(n, 1u << 22);
-  const size_t need = (size_t)chunk * 256;
-  if (need > p->h3Bytes) {
-    if (p->h3) {
-      hipStreamSynchronize(s);
-      hipFree(p->h3);
-      p->h3 = nullptr;
-      p->h3Bytes = 0;
-    }
-    if (!hip_ok(hipMalloc(&p->h3, need), "hipMalloc(proposal activations)")) return;
-    p->h3Bytes = need;
-  }
   const size_t ldsOut = 2 * 128 * 256 + (size_t)kMlpOut * 4;  // W4t ring + output bias
-  for (uint32_t r0 = 0; r0 < n; r0 += chunk) {
-    const uint32_t m = std::min(chunk, n - r0);
-    launch(ctx, s, "mlp_infer_hidden", k_infer_hidden, dim3(blocks_for(m, kInferRaysPerWave)), dim3(64), 0,
-           (const GPRTRay *)rays + r0, m, mlp_wbf(p, 0), mlp_wbf(p, 1), mlp_wbf(p, 2), (const float *)p->bias,
-           (uint16_t *)p->h3);
-    launch(ctx, s, "mlp_infer_top1", k_infer_top1, dim3(blocks_for(m, kInferRays)), dim3(256), ldsOut, m,
-           mlp_wbf(p, 3), (const float *)p->bias, (const uint16_t *)p->h3, cls + r0, logit ? logit + r0 : nullptr,
-           classes, gate);
-  }
+  launch(ctx, s, "mlp_infer_fused", k_infer_top1, dim3(blocks_for(n, kInferRays)), dim3(256), ldsOut, n,
+         mlp_wbf(p, 3), (const float *)p->bias, cls, logit, classes, gate,
+         (const GPRTRay *)rays, mlp_wbf(p, 0), mlp_wbf(p, 1), mlp_wbf(p, 2));
 }
 
 bool mlp_treelets(Context *ctx, hipStream_t s, Proposal *p, Accel *a, int depth) {

@@ -51,15 +51,18 @@ struct Proposal {
   uint32_t *numClassesDev = nullptr;
   Accel *accel = nullptr;
   bool instanceBound = false;  // bound to an instance accel: classes of its top-level leaves
-  // bf16 layer-3 activations between the two inference kernels.
-  void *h3 = nullptr;
-  size_t h3Bytes = 0;
   // Per-ray scratch for proposal-first traces.
   void *rayScratch = nullptr;
   size_t rayScratchBytes = 0;
   // Confidence gate of proposal-first traces: rays whose top logit is below it are traced without
   // their proposal (-INFINITY: every ray uses its proposal).
   float gate = -INFINITY;
+  // Overlap of proposal-first traces: the rays go in `overlapChunks` chunks, each inferred on the
+  // context's second stream while the previous chunk is traversed on its main stream with
+  // `overlapBlocks` traversal workgroups per CU (room for the inference on the same CUs).
+  // overlapChunks < 2: inference, then one traversal.
+  uint32_t overlapChunks = 0;
+  uint32_t overlapBlocks = 0;
 };
 
 size_t mlp_param_count();

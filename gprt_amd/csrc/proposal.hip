@@ -506,18 +506,48 @@ void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *r
   if (count == 0) return;
   hipStream_t s = ctx->stream;
   int32_t *cls = nullptr;
+  uint32_t *vis = visits ? (uint32_t *)visits->d : nullptr;
+  // One traversal of rays [r0, r0 + m) with their classes.
+  auto traverse_range = [&](uint32_t r0, uint32_t m, unsigned blocksPerCU) {
+    const GPRTRay *r = (const GPRTRay *)rays->d + r0;
+    GPRTHit *h = (GPRTHit *)hits->d + r0;
+    const int32_t *c = cls ? cls + r0 : nullptr;
+    uint32_t *v = vis ? vis + r0 : nullptr;
+    if (a->kind == AccelKind::Instances)
+      tlas_trace(ctx, s, a, r, h, m, c, p ? p->numClasses : 0, inst ? (int32_t *)inst->d + r0 : nullptr,
+                 leaf ? (int32_t *)leaf->d + r0 : nullptr, v, blocksPerCU);
+    else
+      trace_closest_ex(ctx, s, a, r, h, m, c, p ? p->classRoot : nullptr, p ? p->numClasses : 0, v, blocksPerCU);
+  };
+  const uint32_t chunks = p ? std::min<uint32_t>(p->overlapChunks, (count + 65535) / 65536) : 0;
   if (p) {
     if (!ensure_ray_scratch(p, (size_t)count * 4, s)) return;
     cls = (int32_t *)p->rayScratch;
-    mlp_infer(ctx, s, p, rays->d, count, cls, nullptr, p->gate);
   }
-  uint32_t *vis = visits ? (uint32_t *)visits->d : nullptr;
-  if (a->kind == AccelKind::Instances) {
-    tlas_trace(ctx, s, a, rays->d, hits->d, count, cls, p ? p->numClasses : 0, inst ? (int32_t *)inst->d : nullptr,
-               leaf ? (int32_t *)leaf->d : nullptr, vis);
-    return;
+  hipStream_t s2 = chunks >= 2 ? ctx->aux_stream() : nullptr;
+  if (chunks >= 2 && s2 && ctx->order_event(chunks)) {
+    // Chunk k is inferred on the second stream while chunk k - 1 is traversed on the main stream;
+    // the second stream starts after the work already enqueued on the main one, and the main
+    // stream waits for each chunk's classes before traversing it (so everything later on the main
+    // stream is ordered after all of it).
+    const uint32_t step = ((count + chunks - 1) / chunks + 255) / 256 * 256;
+    hipEvent_t start = ctx->order_event(chunks);
+    hip_ok(hipEventRecord(start, s), "hipEventRecord");
+    hip_ok(hipStreamWaitEvent(s2, start, 0), "hipStreamWaitEvent");
+    for (uint32_t k = 0, r0 = 0; r0 < count; ++k, r0 += step) {
+      const uint32_t m = std::min(step, count - r0);
+      mlp_infer(ctx, s2, p, (const GPRTRay *)rays->d + r0, m, cls + r0, nullptr, p->gate);
+      hip_ok(hipEventRecord(ctx->order_event(k), s2), "hipEventRecord");
+    }
+    for (uint32_t k = 0, r0 = 0; r0 < count; ++k, r0 += step) {
+      hip_ok(hipStreamWaitEvent(s, ctx->order_event(k), 0), "hipStreamWaitEvent");
+      traverse_range(r0, std::min(step, count - r0), p->overlapBlocks);
+    }
+  } else {
+    if (p) mlp_infer(ctx, s, p, rays->d, count, cls, nullptr, p->gate);
+    traverse_range(0, count, 0);
   }
-  trace_closest_ex(ctx, s, a, rays->d, hits->d, count, cls, p ? p->classRoot : nullptr, p ? p->numClasses : 0, vis);
+  if (a->kind == AccelKind::Instances) return;
   if (inst || leaf) {
     // Triangle accel: instance 0 on a hit, no top-level leaf.
     Buffer *hb = hits;
@@ -527,6 +557,20 @@ void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *r
 }
 
 }  // namespace
+
+GPRT_API void gprtProposalSetOverlap(GPRTContext _ctx, GPRTProposal _p, uint32_t chunks, uint32_t traceBlocksPerCU) {
+  Proposal *p = (Proposal *)_p;
+  if (!_ctx || !p) {
+    fail("gprtProposalSetOverlap: null argument");
+    return;
+  }
+  if (chunks > 64) {
+    fail("gprtProposalSetOverlap: %u chunks (at most 64)", chunks);
+    return;
+  }
+  p->overlapChunks = chunks;
+  p->overlapBlocks = traceBlocksPerCU;
+}
 
 GPRT_API void gprtProposalSetGate(GPRTContext _ctx, GPRTProposal _p, float threshold) {
   Proposal *p = (Proposal *)_p;

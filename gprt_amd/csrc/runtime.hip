@@ -269,10 +269,25 @@ GPRT_API GPRTContext gprtContextCreate(int32_t *requestedDeviceIDs, int numDevic
   return (GPRTContext)ctx;
 }
 
+hipStream_t Context::aux_stream() {
+  if (!aux && !hip_ok(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking), "hipStreamCreate(aux)")) aux = nullptr;
+  return aux;
+}
+
+hipEvent_t Context::order_event(size_t i) {
+  while (orderEvents.size() <= i) {
+    hipEvent_t e = nullptr;
+    if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate(order)")) return nullptr;
+    orderEvents.push_back(e);
+  }
+  return orderEvents[i];
+}
+
 GPRT_API void gprtContextDestroy(GPRTContext _ctx) {
   Context *ctx = (Context *)_ctx;
   if (!ctx) return;
   hipStreamSynchronize(ctx->stream);
+  if (ctx->aux) hipStreamSynchronize(ctx->aux);
   ctx->timer.drain();
   for (auto e : ctx->timer.pool) hipEventDestroy(e);
   if (ctx->arena) hipFree(ctx->arena);
@@ -281,6 +296,8 @@ GPRT_API void gprtContextDestroy(GPRTContext _ctx) {
   if (ctx->pinnedWord) hipHostFree(ctx->pinnedWord);
   hipEventDestroy(ctx->profBegin);
   hipEventDestroy(ctx->profEnd);
+  for (hipEvent_t e : ctx->orderEvents) hipEventDestroy(e);
+  if (ctx->aux) hipStreamDestroy(ctx->aux);
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -16,6 +16,8 @@
 // With a proposal, each ray first traverses the subtree of its proposed treelet (a real hit found
 // there becomes the starting (t, primID)), then the ordinary traversal from the root runs with
 // that tighter bound; the lexicographic (t, primID) rule makes the result identical.
+#include <algorithm>
+
 #include "bvh.hpp"
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -44,7 +46,7 @@ __global__ void __launch_bounds__(kTraceBlock)
     if (base >= count) break;
     const uint32_t ri = base + (uint32_t)l;
     if (ri < count) {
-      const GPRTRay ray = rays[ri];
+      const GPRTRay ray = load_ray_stream(rays + ri);
       const RayPre r = ray_setup(ray.ox, ray.oy, ray.oz, ray.tmin, ray.dx, ray.dy, ray.dz);
       HitState h;
       hit_init(h, ray.tmax);
@@ -67,7 +69,7 @@ __global__ void __launch_bounds__(kTraceBlock)
       o.primID = h.prim;
       o.u = h.u;
       o.v = h.v;
-      hits[ri] = o;
+      store_hit_stream(hits + ri, o);
       if (visitsOut) visitsOut[ri] = visits;
     }
   }
@@ -92,11 +94,13 @@ uint32_t *ensure_trace_scratch(Context *ctx, hipStream_t s, int spillWordsPerLan
 }
 
 bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *hits, uint32_t count,
-                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits) {
+                      const int32_t *proposal, const int32_t *classRoot, uint32_t numClasses, uint32_t *visits,
+                      unsigned blocksPerCU) {
   if (count == 0) return true;
   uint32_t *work = ensure_trace_scratch(ctx, s, kMaxStack - kLdsStack);
   if (!work) return false;
-  const unsigned blocks = (unsigned)ctx->cuCount * kBlocksPerCU;
+  const unsigned blocks = (unsigned)ctx->cuCount * (blocksPerCU ? std::min<unsigned>(blocksPerCU, kBlocksPerCU)
+                                                               : (unsigned)kBlocksPerCU);
   fill_u32(ctx, s, work, 0u, 1);
   const char *name = proposal ? "trace_closest_proposal" : "trace_closest";
   if (a->kind == AccelKind::AABBs)

@@ -404,6 +404,14 @@ GPRT_API float gprtProposalGetGate(GPRTProposal proposal);
  * the share of the sample that keeps its proposal. */
 GPRT_API float gprtProposalCalibrateGate(GPRTContext context, GPRTProposal proposal, GPRTAccel accel, GPRTBuffer rays,
                                          uint32_t count);
+/* Overlap of proposal-first traces (a tuning knob; results do not depend on it): with chunks >= 2
+ * the rays are split into that many chunks (of at least 65,536 rays), each chunk's inference runs
+ * on a second stream of the context while the previous chunk is traversed, and the traversal runs
+ * `traceBlocksPerCU` persistent workgroups per CU (0: as many as the plain trace; fewer leave room
+ * for the inference on the same CUs). chunks < 2 (the default): inference, then one traversal.
+ * Later calls on the context are ordered after the whole trace, as for any call. */
+GPRT_API void gprtProposalSetOverlap(GPRTContext context, GPRTProposal proposal, uint32_t chunks,
+                                     uint32_t traceBlocksPerCU);
 /* Closest hit with an optional proposal (may be NULL) and optional per-ray count of inner nodes
  * fetched (visits, uint32, may be NULL). */
 GPRT_API void gprtAccelTraceClosestHitEx(GPRTContext context, GPRTAccel accel, GPRTProposal proposal,

@@ -401,3 +401,33 @@ def test_confidence_gate(ctx, scene):
     for b in (rb, hb, tb, cb):
         b.destroy()
     prop.destroy()
+
+
+def test_overlapped_proposal_trace(ctx, scene):
+    """Proposal-first traces with inference overlapped on the context's second stream (chunks of
+    the rays, fewer traversal workgroups per CU): hits identical to the plain trace and node
+    fetches identical to the trace without overlap; a call right after it (a map) sees the whole
+    result."""
+    verts, idx, acc = scene
+    prop = api.Proposal(ctx, max_batch=1024, seed=51)
+    prop.bind(acc, 12)
+    tr = scenes.jittered_pinhole_rays(20000, seed=52)
+    th, _ = _trace(ctx, acc, tr)
+    rb, hb = ctx.device_buffer(tr), ctx.device_buffer(th)
+    prop.train(rb, hb, len(tr), epochs=4, lr=2e-3)
+    test = scenes.pinhole_rays(512, 512)
+    plain, _ = _trace(ctx, acc, test)
+    test = np.concatenate([test, scenes.bounce_rays(test, plain, verts, idx, 150_001, seed=53)])
+    plain, _ = _trace(ctx, acc, test)
+    serial, v_serial = _trace(ctx, acc, test, prop)
+    _same(serial, plain)
+    for chunks, blocks in ((4, 8), (7, 6), (64, 0), (2, 14)):
+        ctx.lib.gprtProposalSetOverlap(ctx.h, prop.h, chunks, blocks)
+        print(f"overlapped proposal trace: {len(test)} rays, {chunks} chunks, {blocks} workgroups/CU", flush=True)
+        got, v = _trace(ctx, acc, test, prop)
+        np.testing.assert_array_equal(got.view(np.uint32).reshape(-1, 4), plain.view(np.uint32).reshape(-1, 4))
+        np.testing.assert_array_equal(v, v_serial)
+    ctx.lib.gprtProposalSetOverlap(ctx.h, prop.h, 0, 0)
+    for b in (rb, hb):
+        b.destroy()
+    prop.destroy()
