@@ -1,4 +1,4 @@
-"""ctypes bindings of libgprt_hip.so (declared in include/gprt_hip.h).
+"""ctypes bindings of libgprt_hip.so (declared in include/gprt_hip.h and include/gprt_hip_ext.h).
 
 The names, argument order and meaning are GPRT's host API (gprt/gprt_host.h) for the
 acceleration-structure path; see the header for the reference line each one replaces. The
@@ -105,6 +105,10 @@ _SIGS = {
     "gprtBufferSort": (None, [_P, _P, _P]),
     "gprtBufferSortPayload": (None, [_P, _P, _P, _P]),
     "gprtBufferSortKV": (None, [_P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "gprtBufferSort32": (None, [_P, _P, _P]),
+    "gprtBufferSort64": (None, [_P, _P, _P]),
+    "gprtBufferSortPayload32": (None, [_P, _P, _P, _P]),
+    "gprtBufferSortPayload64": (None, [_P, _P, _P, _P]),
     "gprtSetSortConfig": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gprtGetSortConfig": (None, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gprtAccelBuildBatchAutotune": (C.c_uint32, [_P, C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_char_p,

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/gprt_hip.h"
+#include "../../include/gprt_hip_ext.h"
 #include "../../include/gprt_hip_device.h"
 
 namespace gprt {

@@ -613,6 +613,48 @@ GPRT_API void gprtBufferSortKV(GPRTContext _ctx, GPRTBuffer _keys, GPRTBuffer _v
               "gprtBufferSortKV");
 }
 
+// Width-explicit sorts: the count follows from the key buffer's size and the key width.
+static void sort_width(GPRTContext _ctx, GPRTBuffer _keys, GPRTBuffer _vals, bool payload, uint32_t bits,
+                       GPRTBuffer _scratch, const char *name) {
+  Context *ctx = (Context *)_ctx;
+  Buffer *k = (Buffer *)_keys, *v = (Buffer *)_vals;
+  if (!ctx || !k || (payload && !v)) {
+    fail("%s: null argument", name);
+    return;
+  }
+  const size_t count = k->size / (bits / 8);
+  if (payload && v->size < count * (bits / 8)) {
+    fail("%s: %zu values of %u bits exceed the value buffer (%zu B)", name, count, bits, v->size);
+    return;
+  }
+  if (k->kind == BufKind::Host || (payload && v->kind == BufKind::Host)) {
+    fail("%s: keys and values must be device buffers", name);
+    return;
+  }
+  buffer_sort(ctx, k, payload ? v : nullptr, count, (int)bits, payload ? (int)bits : 0,
+              (Buffer *)_scratch, name);
+}
+
+GPRT_API void gprtBufferSort32(GPRTContext ctx, GPRTBuffer keys, GPRTBuffer scratch) {
+  const StageRange range("gprtBufferSort32");
+  sort_width(ctx, keys, nullptr, false, 32, scratch, "gprtBufferSort32");
+}
+
+GPRT_API void gprtBufferSort64(GPRTContext ctx, GPRTBuffer keys, GPRTBuffer scratch) {
+  const StageRange range("gprtBufferSort64");
+  sort_width(ctx, keys, nullptr, false, 64, scratch, "gprtBufferSort64");
+}
+
+GPRT_API void gprtBufferSortPayload32(GPRTContext ctx, GPRTBuffer keys, GPRTBuffer values, GPRTBuffer scratch) {
+  const StageRange range("gprtBufferSortPayload32");
+  sort_width(ctx, keys, values, true, 32, scratch, "gprtBufferSortPayload32");
+}
+
+GPRT_API void gprtBufferSortPayload64(GPRTContext ctx, GPRTBuffer keys, GPRTBuffer values, GPRTBuffer scratch) {
+  const StageRange range("gprtBufferSortPayload64");
+  sort_width(ctx, keys, values, true, 64, scratch, "gprtBufferSortPayload64");
+}
+
 GPRT_API void gprtSetSortConfig(GPRTContext _ctx, uint32_t digitBits, uint32_t tileKeys, uint32_t wavesPerCU) {
   Context *ctx = (Context *)_ctx;
   if (!ctx) {

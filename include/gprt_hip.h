@@ -277,12 +277,8 @@ GPRT_API void gprtSetSyncTimeout(GPRTContext context, double seconds);
 GPRT_API const char *gprtGetLastError(void); /* NULL if no error since the last clear */
 GPRT_API void gprtClearLastError(void);
 
-/* Width-explicit sorts. The reference's templates advertise uint32_t buffers while its
- * implementation reads 64-bit keys (gprt/gprt_host.h:2390 vs gprt/gprt.cpp:8837); these make the
- * width explicit. keyBits = 32 or 64; valueBits = 0 (no payload), 32 or 64. count = element count
- * (keys need not fill the buffer). Stable, ascending, in place, enqueued. */
-GPRT_API void gprtBufferSortKV(GPRTContext context, GPRTBuffer keys, GPRTBuffer values, uint32_t count,
-                               uint32_t keyBits, uint32_t valueBits, GPRTBuffer scratch);
+/* The width-explicit sorts (gprtBufferSort32/64, gprtBufferSortPayload32/64, gprtBufferSortKV)
+ * are declared extensions and live in gprt_hip_ext.h. */
 
 /* No counterpart in the reference: gprtBufferExclusiveSum for callers that cannot allocate
  * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). Same result

@@ -1,5 +1,5 @@
-"""C-ABI checks that need no GPU: the library loads, exports every symbol include/gprt_hip.h
-declares, and validates arguments before any HIP call (reference error convention:
+"""C-ABI checks that need no GPU: the library loads, exports every symbol include/gprt_hip.h and
+include/gprt_hip_ext.h declare, and validates arguments before any HIP call (reference error convention:
 gprt/gprt.cpp:247-251 prints "#gprt error:" and raises SIGINT; tests use the recording mode)."""
 import ctypes as C
 import os
@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    text = open(os.path.join(ROOT, "include", "gprt_hip.h")).read()
+    text = "".join(open(os.path.join(ROOT, "include", h)).read() for h in ("gprt_hip.h", "gprt_hip_ext.h"))
     return sorted(set(re.findall(r"GPRT_API\s+[\w\s\*]+?\b(gprt\w+)\s*\(", text)))
 
 
@@ -102,6 +102,28 @@ def test_instance_record_layout(tmp_path):
     assert rec["accelAddress"] == 0x1122334455667788
     c = api.GPRTInstance.from_buffer_copy(raw)
     assert (c.instanceCustomIndex, c.mask, c.geometryIndex, c.flags) == (0x123456, 0xAB, 7, 3)
+
+
+def test_width_explicit_sorts_validate(rec):
+    for name in ("gprtBufferSort32", "gprtBufferSort64"):
+        getattr(rec, name)(None, None, None)
+        assert "null argument" in api.last_error()
+        rec.gprtClearLastError()
+    for name in ("gprtBufferSortPayload32", "gprtBufferSortPayload64"):
+        getattr(rec, name)(None, None, None, None)
+        assert "null argument" in api.last_error()
+        rec.gprtClearLastError()
+
+
+def test_extension_header_compiles_as_c(tmp_path):
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    src = tmp_path / "ext.c"
+    src.write_text('#include "gprt_hip_ext.h"\nvoid (*f)(GPRTContext, GPRTBuffer, GPRTBuffer) = gprtBufferSort32;\nint main(void) { return f != gprtBufferSort32; }\n')
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                    str(src)], check=True)
 
 
 def test_compute_entry_points_validate(rec):

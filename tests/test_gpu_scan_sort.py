@@ -232,3 +232,28 @@ def test_sort_full_size_microbench(ctx):
     order = np.argsort(keys, kind="stable")
     np.testing.assert_array_equal(k, keys[order])
     np.testing.assert_array_equal(v, order.astype(np.uint32))
+
+
+@pytest.mark.parametrize("bits,payload,n", [(32, False, 300_001), (64, False, 300_001), (32, True, 300_001),
+                                            (64, True, 300_001), (32, True, 1), (64, True, 8193)])
+def test_width_explicit_sorts(ctx, bits, payload, n):
+    """gprtBufferSort32/64 and gprtBufferSortPayload32/64 (include/gprt_hip_ext.h): the count is the
+    key buffer's size over the key width; duplicated keys keep their payload order (stable)."""
+    rng = np.random.default_rng(bits + n + payload)
+    kt = np.uint32 if bits == 32 else np.uint64
+    keys = rng.integers(0, 1 << 20, n, dtype=np.uint64).astype(kt) * kt(3 if bits == 32 else (1 << 40) + 1)
+    vals = np.arange(n, dtype=kt)
+    kb = ctx.device_buffer(keys)
+    vb = ctx.device_buffer(vals) if payload else None
+    fn = getattr(ctx.lib, f"gprtBufferSort{'Payload' if payload else ''}{bits}")
+    if payload:
+        fn(ctx.h, kb.h, vb.h, None)
+    else:
+        fn(ctx.h, kb.h, None)
+    assert api.last_error() is None, api.last_error()
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(kb.read(kt), keys[order])
+    if payload:
+        np.testing.assert_array_equal(vb.read(kt), vals[order])
+        vb.destroy()
+    kb.destroy()
