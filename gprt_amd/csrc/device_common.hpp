@@ -18,18 +18,21 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
-// Inclusive prefix sum across the 64 lanes (Hillis-Steele over ds_bpermute shuffles).
+// Inclusive prefix sum across the 64 lanes with DPP: shifts of 1, 2, 4 and 8 lanes within each
+// 16-lane row (lanes shifted in from outside the row read 0), then lane 15 of each row is added
+// to the next row and lane 31 to the upper half. Six VALU operations with no LDS traffic; the
+// whole wave must be active.
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
-  const uint32_t l = lane_id();
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t t = __shfl_up(v, o, kWave);
-    if (l >= (uint32_t)o) v += t;
-  }
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
   return v;
 }
 
-__device__ __forceinline__ uint32_t wave_read(uint32_t v, int lane) { return __shfl(v, lane, kWave); }
+__device__ __forceinline__ uint32_t wave_read(uint32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll

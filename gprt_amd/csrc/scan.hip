@@ -9,7 +9,11 @@
 //
 // MI355X design:
 //  * A tile is WAVES wave64s; each wave owns 64 lanes x K uint4 = 256K consecutive items, loaded
-//    with 16-byte coalesced vector loads and scanned with wave-level prefix sums.
+//    with 16-byte coalesced vector loads and scanned with DPP wave prefix sums. The sum streams
+//    its input and output with nontemporal loads and stores.
+//  * The sum's tiles are large (16 waves x 40 items per lane = 40,960 items, one workgroup per
+//    CU): a tile waits in its look-back for predecessors that are still loading, and that wait
+//    grows with the number of tiles in flight, so fewer, larger tiles spend less time waiting.
 //  * The look-back is done by a whole wave: lane l inspects predecessor tile (p-1-l), so 64
 //    predecessors are examined per round trip instead of one.
 //  * Status words are 64-bit {flag:32 | value:32}: values wrap modulo 2^32 with no 2^30 limit
@@ -27,6 +31,8 @@ namespace {
 
 constexpr uint64_t kAgg = 1ull << 32;
 constexpr uint64_t kInc = 2ull << 32;
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 // MODE 0: exclusive sum; 1: partition; 2: select.
 template <int WAVES, int K, int MODE>
@@ -56,7 +62,7 @@ __global__ void __launch_bounds__(WAVES * 64)
   for (int k = 0; k < K; ++k) {
     const size_t idx = waveBase + ((size_t)k * 64 + l) * 4;
     if (idx + 3 < n) {
-      uint4 q = *(const uint4 *)(in + idx);
+      const u32x4_t q = MODE == 0 ? __builtin_nontemporal_load((const u32x4_t *)(in + idx)) : *(const u32x4_t *)(in + idx);
       v[k][0] = q.x;
       v[k][1] = q.y;
       v[k][2] = q.z;
@@ -142,7 +148,7 @@ __global__ void __launch_bounds__(WAVES * 64)
     for (int c = 1; c < 4; ++c) r[c] = r[c - 1] + counted(k, c - 1);
     if (MODE == 0) {
       if (idx + 3 < n) {
-        *(uint4 *)(out + idx) = make_uint4(r[0], r[1], r[2], r[3]);
+        __builtin_nontemporal_store(u32x4_t{r[0], r[1], r[2], r[3]}, (u32x4_t *)(out + idx));
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
@@ -164,11 +170,11 @@ __global__ void __launch_bounds__(WAVES * 64)
   }
 }
 
-// Tiles: 16,384 items for the sum (8 waves x 32 items per lane: fewer tiles, so fewer tickets and
-// look-back steps per item), 4,096 for partition / select (4 waves x 16 items per lane), whose
-// flags and scattered writes need more registers.
-constexpr int kScanWavesSum = 8, kScanWavesSel = 4;
-constexpr int kScanKSum = 8, kScanKSel = 4;
+// Tiles: 40,960 items for the sum (16 waves x 40 items per lane, one workgroup per CU; about 110
+// VGPRs, within the 128 that 16 waves on a CU allow), 4,096 for partition / select (4 waves x 16
+// items per lane), whose flags and scattered writes need more registers.
+constexpr int kScanWavesSum = 16, kScanWavesSel = 4;
+constexpr int kScanKSum = 10, kScanKSel = 4;
 constexpr uint32_t kScanTileSum = kScanWavesSum * 64 * kScanKSum * 4;
 constexpr uint32_t kScanTileSel = kScanWavesSel * 64 * kScanKSel * 4;
 
@@ -179,7 +185,9 @@ constexpr uint32_t kScanTileSel = kScanWavesSel * 64 * kScanKSel * 4;
 // the name stands for a host-side shadow); it uses the device address hipGetSymbolAddress returns
 // for the current device, kept per device in g_noAllocState.
 constexpr size_t kNoAllocMaxItems = 1ull << 31;
-constexpr size_t kNoAllocWords = kNoAllocMaxItems / kScanTileSum + 2;  // 1 MiB + 16 B
+constexpr size_t kNoAllocWords = (1ull << 17) + 2;  // 1 MiB + 16 B
+static_assert(kNoAllocWords >= (kNoAllocMaxItems + kScanTileSum - 1) / kScanTileSum + 2,
+              "the static state holds every tile status of the largest allowed input");
 
 std::mutex g_noAllocMutex;
 std::vector<void *> g_noAllocState;  // device address of `scratch` per device ordinal
