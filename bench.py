@@ -65,8 +65,8 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 11961.7e6 + 69.8e6,
-    "mlp_infer_top1": 1219.3e6 + 16.8e6,  # one launch of 4,194,304 rays, 2,897 classes bound
+    "trace_closest": 28215.6e6 + 138.6e6,  # the plain trace of the workload's 8,388,608 rays
+    "mlp_infer_fused": 275.9e6 + 33.6e6,  # one launch of 8,388,608 rays, 2,897 classes bound
     "lbvh_sort_pass": 32.6e6 + 37.5e6,
     # per primitive, summed over the build kernels (init, bounds, keys, histogram scan, 4 sort
     # passes, sub-tile, region, top)
@@ -77,11 +77,11 @@ TRAFFIC_BYTES = {
     "trace_boxes": 14945.5e6 + 77.8e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
     "sort_onesweep": 599.1e6 + 617.4e6,  # one pass over 2^26 u32 keys + u32 payloads (8,192-key tiles)
-    "scan_lookback": 1084.5e6 + 1075.3e6,  # one launch over 2^28 u32
+    "scan_lookback": 1075.8e6 + 1078.1e6,  # one launch over 2^28 u32
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
 # commands and per-kernel table in profiles/cache_hit_rates_scene.md).
-TRACE_CACHE_HIT = {"l1": 0.819, "l2": 0.704}
+TRACE_CACHE_HIT = {"l1": 0.808, "l2": 0.684}
 
 
 def log(msg):
@@ -134,6 +134,49 @@ def _cpu_threads():
     from oracle import binding as orc
     return int(orc.lib().orc_max_threads())
 
+
+
+def proposal_fetch_split(lib, ctx, acc, prop, rb, n_rays, hits, n_prims):
+    """Inner nodes fetched per ray, plain and proposal-first without the gate, split by what the
+    proposal was: the ray's true treelet (the treelet of the triangle its plain trace hits), a
+    wrong treelet (or a ray that hits nothing), or the root class (no treelet walk). `best_gate`
+    is the mean over rays of the smaller of the two counts: what a per-ray gate that knew the
+    outcome could reach. Untimed; the counts come from the traversal kernel itself."""
+    from gprt_amd import api
+    cls_b = ctx.device_buffer(nbytes=n_rays * 4)
+    root_b = ctx.device_buffer(nbytes=4096 * 4)
+    pc_b = ctx.device_buffer(nbytes=n_prims * 4)
+    v0_b = ctx.device_buffer(nbytes=n_rays * 4)
+    v1_b = ctx.device_buffer(nbytes=n_rays * 4)
+    tmp_h = ctx.device_buffer(nbytes=n_rays * 16)
+    lib.gprtProposalInfer(ctx.h, prop.h, rb.h, n_rays, cls_b.h)
+    lib.gprtProposalExportTreelets(ctx.h, prop.h, root_b.h, pc_b.h)
+    lib.gprtAccelTraceClosestHitEx(ctx.h, acc.h, None, rb.h, tmp_h.h, n_rays, v0_b.h)
+    gate = float(lib.gprtProposalGetGate(prop.h))
+    lib.gprtProposalSetGate(ctx.h, prop.h, float("-inf"))
+    lib.gprtAccelTraceClosestHitEx(ctx.h, acc.h, prop.h, rb.h, tmp_h.h, n_rays, v1_b.h)
+    lib.gprtProposalSetGate(ctx.h, prop.h, gate)
+    cls = cls_b.read(np.int32)
+    roots = root_b.read(np.int32)
+    prim_class = pc_b.read(np.int32)
+    v0 = v0_b.read(np.uint32).astype(np.int64)
+    v1 = v1_b.read(np.uint32).astype(np.int64)
+    for b in (cls_b, root_b, pc_b, v0_b, v1_b, tmp_h):
+        b.destroy()
+    pid = hits["primID"]
+    true_cls = np.where(pid >= 0, prim_class[np.clip(pid, 0, n_prims - 1)], -1)
+    valid = (cls >= 0) & (cls < len(roots))
+    at_root = valid & (roots[np.clip(cls, 0, len(roots) - 1)] == 0)
+    right = valid & ~at_root & (cls == true_cls)
+    wrong = ~at_root & ~right
+    out = {"rays": int(n_rays), "plain": float(v0.mean()), "proposal_first": float(v1.mean()),
+           "best_gate": float(np.minimum(v0, v1).mean())}
+    for name, m in (("right_treelet", right), ("wrong_treelet_or_miss", wrong), ("root_class", at_root)):
+        k = int(m.sum())
+        out[name] = {"share": k / n_rays, "plain": float(v0[m].mean()) if k else 0.0,
+                     "proposal_first": float(v1[m].mean()) if k else 0.0,
+                     "cheaper_share": float((v1[m] < v0[m]).mean()) if k else 0.0}
+    return out
 
 def scene_workload(args):
     from gprt_amd import api, scenes
@@ -251,6 +294,9 @@ def scene_workload(args):
     mismatches = int((h1.view(np.uint32).reshape(-1, 4) != h2.view(np.uint32).reshape(-1, 4)).any(1).sum())
     hit_rate = float((h1["primID"] >= 0).mean())
 
+    fetch_split = proposal_fetch_split(lib, ctx, acc, prop, rb, n_rays, h1, N_TRIS)
+    log(f"proposal node fetches per ray: {json.dumps(fetch_split)}")
+
     sw_ms, sw_n = ms_of(per_kernel, "lbvh_sort_pass")
     sw_avg = sw_ms / max(sw_n, 1)
     roof_sort = {"kernel": "lbvh_sort_pass", "bound": "hbm", "achieved": N_TRIS * 16 / (sw_avg * 1e-3) / 1e9,
@@ -270,7 +316,8 @@ def scene_workload(args):
                 "achieved": out_rays * mlp_flop_per_ray(classes) / (out_avg * 1e-3) / 1e12,
                 "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "avg_launch_ms": out_avg,
                 "flop_per_ray": mlp_flop_per_ray(classes), "classes": classes,
-                "rays_per_launch": out_rays, "traffic": None}
+                "rays_per_launch": out_rays,
+                "traffic": TRAFFIC_BYTES["mlp_infer_fused"] * out_rays / 8388608}
     roof_mlp["frac"] = roof_mlp["achieved"] / BF16_PEAK_TFLOPS
 
     from oracle import binding as orc
@@ -352,7 +399,8 @@ def scene_workload(args):
                        "learned_vs_plain": plain_ms / learned_ms, "treelet_cut_ms": cut_ms,
                        "hit_mismatches_learned_vs_plain": mismatches, "hit_rate": hit_rate,
                        "train_s": train_s, "train_loss": loss, "train_top1": train_acc,
-                       "gate": gate, "gate_keep_fraction": gate_keep},
+                       "gate": gate, "gate_keep_fraction": gate_keep,
+                       "proposal_fetches": fetch_split},
         # The roofline of the kernel that takes the most time in the step.
         "roofline": dominant,
         "roofline_build": roof_build,
