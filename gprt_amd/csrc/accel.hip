@@ -613,7 +613,8 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
         std::sort(ms.begin(), ms.end());
         const float med = ms[reps / 2];
         char line[160];
-        snprintf(line, sizeof(line), "%s{\"digitBits\": %d, \"tileKeys\": %d, \"wavesPerCU\": %d, \"ms\": %.4f}",
+        // %.9g gives the float back exactly, so a reader's argmin agrees with the strict < below
+        snprintf(line, sizeof(line), "%s{\"digitBits\": %d, \"tileKeys\": %d, \"wavesPerCU\": %d, \"ms\": %.9g}",
                  idx ? ", " : "", b, t, w, med);
         js += line;
         if (med < bestMs) {
@@ -629,7 +630,7 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
   lbvh_build_batch(ctx, ctx->stream, accels.data(), count, true);
   ctx->timer.enabled = timer;
   char tail[200];
-  snprintf(tail, sizeof(tail), "], \"best\": {\"digitBits\": %d, \"tileKeys\": %d, \"wavesPerCU\": %d, \"ms\": %.4f}}",
+  snprintf(tail, sizeof(tail), "], \"best\": {\"digitBits\": %d, \"tileKeys\": %d, \"wavesPerCU\": %d, \"ms\": %.9g}}",
            bestCfg.digitBits, bestCfg.tileKeys, bestCfg.wavesPerCU, bestMs);
   js += tail;
   if (report && cap) {
