@@ -856,18 +856,18 @@ def boxes_workload(args):
     cpu = None
     if not args.no_cpu:
         from oracle import binding as orc
-        m = args.cpu_build_tris
-        t0 = time.perf_counter()
-        orc.lbvh_build_boxes(boxes[:m])
-        cpu_build_s = time.perf_counter() - t0
+        T = _cpu_threads()
+        m = min(args.cpu_build_tris, N_BOXES)
+        single = _median_time(lambda: orc.lbvh_build_boxes(boxes[:m]))
+        multi = _median_time(lambda: orc.par_lbvh_build_boxes(boxes, threads=T))
         sel = np.random.default_rng(0).choice(len(rays), 256, replace=False)
         sub = boxes[: 1 << 16]
-        t0 = time.perf_counter()
-        orc.trace_boxes_bruteforce(rays[sel], sub, threads=1)
-        bf_s = time.perf_counter() - t0
-        cpu = {"value": m / cpu_build_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
-               "sample": f"oracle LBVH build of the first {m} boxes, single thread; oracle brute-force trace of 256 "
-                         f"rays against the first 65,536 boxes ({256 * 65536 / bf_s / 1e6:.1f} M ray-box tests/s)"}
+        bf_s = _median_time(lambda: orc.trace_boxes_bruteforce(rays[sel], sub, threads=T), reps=3)
+        cpu = _cpu_baseline(
+            N_BOXES / multi / 1e6, "Mprims/s", T,
+            f"oracle OpenMP LBVH build of all {N_BOXES} boxes ({T} threads); OpenMP brute-force trace of 256 rays "
+            f"against the first 65,536 boxes ({256 * 65536 / bf_s / 1e6:.1f} M ray-box tests/s)",
+            m / single / 1e6, f"oracle LBVH build of the first {m} boxes, 1 thread")
     out = {
         "metric": "AABB-primitive LBVH build Mprims/s + closest-hit Mrays/s, 1 GPU",
         "value": N_BOXES / (build_ms * 1e3), "unit": "Mprims/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -943,13 +943,14 @@ def compute_workload(args):
     roof["traffic"] = TRAFFIC_BYTES["compute_bounds"]
     cpu = None
     if not args.no_cpu:
+        from oracle import binding as orc
+        T = _cpu_threads()
         m = 1 << 22
-        t0 = time.perf_counter()
-        for _ in range(3):
-            np.concatenate([centres[:m] - radii[:m, None], centres[:m] + radii[:m, None]], 1)
-        cpu_s = (time.perf_counter() - t0) / 3
-        cpu = {"value": m / cpu_s / 1e6, "unit": "Mprims/s", "cores": 1, "kind": "port",
-               "sample": f"numpy sphere bounds of the first {m} spheres, single thread"}
+        single = _median_time(lambda: orc.par_sphere_bounds(centres[:m], radii[:m], threads=1))
+        multi = _median_time(lambda: orc.par_sphere_bounds(centres, radii, threads=T))
+        cpu = _cpu_baseline(n / multi / 1e6, "Mprims/s", T,
+                            f"oracle OpenMP sphere bounds of all {n} spheres ({T} threads)",
+                            m / single / 1e6, f"oracle sphere bounds of the first {m} spheres, 1 thread")
     out = {
         "metric": "user compute program (ComputeBounds) Mprims/s through gprtComputeLaunch, 1 GPU",
         "value": n / (ms * 1e3), "unit": "Mprims/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,

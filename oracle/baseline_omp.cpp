@@ -152,12 +152,13 @@ uint32_t orc_par_exclusive_sum(const uint32_t *in, uint32_t *out, size_t n, int 
   return part[used];
 }
 
-// Bottom-up refit over a reference-layout tree: a leaf writes its triangle's box and climbs; at
+}  // extern "C"
+
+// Bottom-up refit over a reference-layout tree: a leaf writes its primitive's box and climbs; at
 // each inner node the first child to arrive stops (an atomic flag per node) and the second merges
 // both children's boxes and goes on. The result is order-free (min / max in the total order).
-void orc_par_refit(const float *verts, const uint32_t *idx, const int32_t *nodes, size_t n, float *aabbs,
-                   int threads) {
-  const int T = nthreads(threads);
+template <class LeafBox>
+static void par_refit_with(const int32_t *nodes, size_t n, float *aabbs, int T, LeafBox leafBox) {
   const long N = (long)n;
   std::vector<std::atomic<int>> flag(N > 1 ? (size_t)(N - 1) : 1);
 #pragma omp parallel for num_threads(T) schedule(static)
@@ -165,14 +166,7 @@ void orc_par_refit(const float *verts, const uint32_t *idx, const int32_t *nodes
 #pragma omp parallel for num_threads(T) schedule(static)
   for (long i = 0; i < N; ++i) {
     const long leaf = N - 1 + i;
-    const uint32_t prim = (uint32_t)nodes[4 * leaf + 3];
-    const float *p1 = verts + 3 * (size_t)idx[3 * prim], *p2 = verts + 3 * (size_t)idx[3 * prim + 1],
-                *p3 = verts + 3 * (size_t)idx[3 * prim + 2];
-    float *b = &aabbs[6 * leaf];
-    for (int a = 0; a < 3; ++a) {
-      b[a] = omin(p1[a], omin(p2[a], p3[a]));
-      b[3 + a] = omax(p1[a], omax(p2[a], p3[a]));
-    }
+    leafBox((uint32_t)nodes[4 * leaf + 3], &aabbs[6 * leaf]);
     for (long p = nodes[4 * leaf + 2]; p >= 0; p = nodes[4 * p + 2]) {
       if (flag[p].fetch_add(1, std::memory_order_acq_rel) == 0) break;
       const float *c0 = &aabbs[6 * (long)nodes[4 * p + 0]], *c1 = &aabbs[6 * (long)nodes[4 * p + 1]];
@@ -185,11 +179,26 @@ void orc_par_refit(const float *verts, const uint32_t *idx, const int32_t *nodes
   }
 }
 
-void orc_par_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint64_t *keysOut, int32_t *nodes,
-                        float *aabbs, int threads) {
-  const int T = nthreads(threads);
-  const long N = (long)n;
-  // Scene box: per-thread min / max in the total order (exact, order-free), then combined.
+extern "C" {
+
+void orc_par_refit(const float *verts, const uint32_t *idx, const int32_t *nodes, size_t n, float *aabbs,
+                   int threads) {
+  par_refit_with(nodes, n, aabbs, nthreads(threads), [&](uint32_t prim, float *b) {
+    const float *p1 = verts + 3 * (size_t)idx[3 * prim], *p2 = verts + 3 * (size_t)idx[3 * prim + 1],
+                *p3 = verts + 3 * (size_t)idx[3 * prim + 2];
+    for (int a = 0; a < 3; ++a) {
+      b[a] = omin(p1[a], omin(p2[a], p3[a]));
+      b[3 + a] = omax(p1[a], omax(p2[a], p3[a]));
+    }
+  });
+}
+
+}  // extern "C"
+
+// Scene box over `count` points (point(i, p) gives point i): per-thread min / max in the total
+// order (exact, order-free), then combined.
+template <class Point>
+static void par_bounds(long count, int T, float bounds[6], Point point) {
   std::vector<float> part((size_t)T * 6);
   int used = 1;
 #pragma omp parallel num_threads(T)
@@ -197,8 +206,8 @@ void orc_par_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint6
     const int t = omp_get_thread_num(), nt = omp_get_num_threads();
     float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
 #pragma omp for schedule(static)
-    for (long i = 0; i < 3 * N; ++i) {
-      const float *p = verts + 3 * (size_t)idx[i];
+    for (long i = 0; i < count; ++i) {
+      const float *p = point(i);
       for (int a = 0; a < 3; ++a) {
         b[a] = omin(b[a], p[a]);
         b[3 + a] = omax(b[3 + a], p[a]);
@@ -208,22 +217,33 @@ void orc_par_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint6
 #pragma omp single
     used = nt;
   }
-  float bounds[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int a = 0; a < 3; ++a) {
+    bounds[a] = INFINITY;
+    bounds[3 + a] = -INFINITY;
+  }
   for (int t = 0; t < used; ++t)
     for (int a = 0; a < 3; ++a) {
       bounds[a] = omin(bounds[a], part[(size_t)t * 6 + a]);
       bounds[3 + a] = omax(bounds[3 + a], part[(size_t)t * 6 + 3 + a]);
     }
-  // Morton codes of the centroids (the arithmetic of orc_morton_keys) with primID payloads.
+}
+
+// Morton codes of key points (keyPoint(i, c) writes point i's key point, the arithmetic of
+// orc_morton_keys), sorted with primID payloads into keysOut (morton << 32 | primID), then the
+// Karras tree: every inner node independently (each node is the only writer of its own fields
+// and of its children's parent fields).
+template <class KeyPoint>
+static void par_keys_karras(size_t n, const float bounds[6], int T, uint64_t *keysOut, int32_t *nodes,
+                            KeyPoint keyPoint) {
+  const long N = (long)n;
   std::vector<uint32_t> codes(n), ids(n);
 #pragma omp parallel for num_threads(T) schedule(static)
   for (long i = 0; i < N; ++i) {
-    const float *p1 = verts + 3 * (size_t)idx[3 * i], *p2 = verts + 3 * (size_t)idx[3 * i + 1],
-                *p3 = verts + 3 * (size_t)idx[3 * i + 2];
+    float c[3];
+    keyPoint(i, c);
     uint32_t q[3];
     for (int a = 0; a < 3; ++a) {
-      const float c = (p1[a] + p2[a] + p3[a]) / 3.f;
-      float pt = (c - bounds[a]) / (bounds[3 + a] - bounds[a]);
+      float pt = (c[a] - bounds[a]) / (bounds[3 + a] - bounds[a]);
       pt = std::fmin(std::fmax(pt * 1024.f, 0.f), 1023.f);
       q[a] = (uint32_t)pt;
     }
@@ -235,8 +255,6 @@ void orc_par_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint6
   par_radix(codes.data(), ids.data(), n, 0, 32, T);
 #pragma omp parallel for num_threads(T) schedule(static)
   for (long i = 0; i < N; ++i) keysOut[i] = ((uint64_t)codes[i] << 32) | ids[i];
-  // Karras: every inner node independently (each node is the only writer of its own fields and
-  // of its children's parent fields).
   auto delta = [&](long i, long j) -> int {
     if (j < 0 || j >= N) return -1;
     const uint64_t x = keysOut[i] ^ keysOut[j];
@@ -271,7 +289,54 @@ void orc_par_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint6
     nodes[4 * left + 2] = (int32_t)i;
     nodes[4 * right + 2] = (int32_t)i;
   }
+}
+
+extern "C" {
+
+void orc_par_lbvh_build(const float *verts, const uint32_t *idx, size_t n, uint64_t *keysOut, int32_t *nodes,
+                        float *aabbs, int threads) {
+  const int T = nthreads(threads);
+  float bounds[6];
+  par_bounds(3 * (long)n, T, bounds, [&](long i) { return verts + 3 * (size_t)idx[i]; });
+  par_keys_karras(n, bounds, T, keysOut, nodes, [&](long i, float *c) {
+    const float *p1 = verts + 3 * (size_t)idx[3 * i], *p2 = verts + 3 * (size_t)idx[3 * i + 1],
+                *p3 = verts + 3 * (size_t)idx[3 * i + 2];
+    for (int a = 0; a < 3; ++a) c[a] = (p1[a] + p2[a] + p3[a]) / 3.f;
+  });
   orc_par_refit(verts, idx, nodes, n, aabbs, threads);
+}
+
+// The LBVH over box primitives {lo.xyz, hi.xyz} (orc_lbvh_build_boxes): scene box over both
+// corners, key point the centre (lo + hi) * 0.5, leaf box the ordered corners.
+void orc_par_lbvh_build_boxes(const float *boxes, size_t n, uint64_t *keysOut, int32_t *nodes, float *aabbs,
+                              int threads) {
+  const int T = nthreads(threads);
+  float bounds[6];
+  par_bounds(2 * (long)n, T, bounds, [&](long i) { return boxes + 3 * (size_t)i; });
+  par_keys_karras(n, bounds, T, keysOut, nodes, [&](long i, float *c) {
+    const float *p = boxes + 6 * (size_t)i;
+    for (int a = 0; a < 3; ++a) c[a] = (p[a] + p[3 + a]) * 0.5f;
+  });
+  par_refit_with(nodes, n, aabbs, T, [&](uint32_t prim, float *b) {
+    const float *p = boxes + 6 * (size_t)prim;
+    for (int a = 0; a < 3; ++a) {
+      b[a] = omin(p[a], p[3 + a]);
+      b[3 + a] = omax(p[a], p[3 + a]);
+    }
+  });
+}
+
+// Sphere bounds (the compute sample's ComputeBounds): box i = {centre - r, centre + r}.
+void orc_par_sphere_bounds(const float *centres, const float *radii, size_t n, float *boxes, int threads) {
+#pragma omp parallel for num_threads(nthreads(threads)) schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    const float *c = centres + 3 * (size_t)i;
+    float *b = boxes + 6 * (size_t)i;
+    for (int a = 0; a < 3; ++a) {
+      b[a] = c[a] - radii[i];
+      b[3 + a] = c[a] + radii[i];
+    }
+  }
 }
 
 // sizes[m] triangles per mesh; mesh m's vertices start at vertex 3 * (its first triangle) and

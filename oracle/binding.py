@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
             "orc_par_radix_sort": (None, [P, P, sz, C.c_int, C.c_int, C.c_int]),
             "orc_par_exclusive_sum": (C.c_uint32, [P, P, sz, C.c_int]),
             "orc_par_lbvh_build": (None, [P, P, sz, P, P, P, C.c_int]),
+            "orc_par_lbvh_build_boxes": (None, [P, sz, P, P, P, C.c_int]),
+            "orc_par_sphere_bounds": (None, [P, P, sz, P, C.c_int]),
             "orc_par_refit": (None, [P, P, P, sz, P, C.c_int]),
             "orc_par_lbvh_build_many": (None, [P, P, P, sz, P, P, P, C.c_int]),
             "orc_trace_bvh_multi": (None, [P, sz, P, P, P, sz, P, P, P, P, C.c_int]),
@@ -308,6 +310,27 @@ def par_lbvh_build(verts, idx, threads: int = 0):
     aabbs = np.zeros((2 * n - 1, 6), dtype=np.float32)
     lib().orc_par_lbvh_build(_p(verts), _p(idx), n, _p(keys), _p(nodes), _p(aabbs), threads)
     return keys, nodes, aabbs
+
+
+def par_lbvh_build_boxes(boxes, threads: int = 0):
+    """All-cores form of lbvh_build_boxes (same keys, nodes and boxes)."""
+    b = _boxes(boxes)
+    n = len(b)
+    keys = np.zeros(n, dtype=np.uint64)
+    nodes = np.zeros((max(2 * n - 1, 0), 4), dtype=np.int32)
+    aabbs = np.zeros((max(2 * n - 1, 0), 6), dtype=np.float32)
+    if n:
+        lib().orc_par_lbvh_build_boxes(_p(b), n, _p(keys), _p(nodes), _p(aabbs), threads)
+    return keys, nodes, aabbs
+
+
+def par_sphere_bounds(centres, radii, threads: int = 0):
+    """Boxes {centre - r, centre + r} of spheres, all cores."""
+    c = np.ascontiguousarray(centres, dtype=np.float32).reshape(-1, 3)
+    r = np.ascontiguousarray(radii, dtype=np.float32)
+    out = np.zeros((len(c), 6), dtype=np.float32)
+    lib().orc_par_sphere_bounds(_p(c), _p(r), len(c), _p(out), threads)
+    return out
 
 
 def par_refit(verts, idx, nodes, threads: int = 0):

@@ -1,7 +1,7 @@
 """The all-cores CPU baseline (oracle/baseline_omp.cpp) computes exactly what the serial oracle
 computes: the parallel radix sort equals std::stable_sort (and numpy's stable argsort), the
-blocked scan equals the serial scan, the parallel LBVH build equals the serial build (keys,
-topology, every box), the per-mesh batch equals one build per mesh, and the multi-mesh trace
+blocked scan equals the serial scan, the parallel LBVH builds (triangles and boxes) equal the
+serial builds (keys, topology, every box), the sphere bounds equal numpy's, the per-mesh batch equals one build per mesh, and the multi-mesh trace
 equals the minimum of (t, mesh, primID) over per-mesh traces. Thread counts that do not divide
 the input are included."""
 import numpy as np
@@ -51,6 +51,32 @@ def test_par_lbvh_build_equals_serial(orc, n, kind):
     moved = (v + scenes.rand01(9, 0, v.size).reshape(v.shape).astype(np.float32) * np.float32(0.01)).astype(np.float32)
     np.testing.assert_array_equal(orc.par_refit(moved, idx, rn, threads=5).view(np.uint32),
                                   orc.refit(moved, idx, rn).view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 1000, 65_537])
+def test_par_box_lbvh_build_equals_serial(orc, n):
+    # Boxes given with some corners swapped (lo > hi on an axis): the leaf box is the ordered one.
+    rng = np.random.default_rng(n)
+    c = rng.random((n, 3), dtype=np.float32)
+    e = (rng.random((n, 3), dtype=np.float32) * np.float32(0.01)).astype(np.float32)
+    boxes = np.concatenate([c - e, c + e], 1).astype(np.float32)
+    boxes[::5, [0, 3]] = boxes[::5, [3, 0]]
+    rk, rn, ra = orc.lbvh_build_boxes(boxes)
+    for threads in (1, 6):
+        k, nodes, aabbs = orc.par_lbvh_build_boxes(boxes, threads=threads)
+        np.testing.assert_array_equal(k, rk)
+        np.testing.assert_array_equal(nodes, rn)
+        np.testing.assert_array_equal(aabbs.view(np.uint32), ra.view(np.uint32))
+
+
+def test_par_sphere_bounds_equals_numpy(orc):
+    rng = np.random.default_rng(3)
+    centres = (rng.random((10_001, 3), dtype=np.float32) * np.float32(100.0)).astype(np.float32)
+    radii = rng.random(10_001, dtype=np.float32)
+    ref = np.concatenate([centres - radii[:, None], centres + radii[:, None]], 1)
+    for threads in (1, 7):
+        np.testing.assert_array_equal(orc.par_sphere_bounds(centres, radii, threads=threads).view(np.uint32),
+                                      ref.view(np.uint32))
 
 
 def test_par_lbvh_build_duplicate_codes(orc):
