@@ -458,7 +458,7 @@ def batched_workload(args):
         lib.gprtAccelBuildBatch(ctx.h, ev, len(accs), params)
 
     per_mesh_ms = timed(per_mesh, args.steps) if not args.no_per_mesh else None
-    lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)  # the default configuration
+    lib.gprtSetSortConfig(ctx.h, 8, 0, 0)  # the default configuration
     batch_ms = timed(batch, args.steps)
     if args.no_autotune:
         tune, best, tuned_ms = {"best": None, "configs": []}, -1, batch_ms
@@ -1022,7 +1022,7 @@ def sort_workload(args):
     pass_ms = pk["sort_onesweep"]["total_ms"] / pk["sort_onesweep"]["launches"]
     # The same sort under each tile size of the sort configuration (8-bit digits), for comparison.
     sweep = {}
-    for tile in (2048, 4096, 8192):
+    for tile in (2048, 4096, 8192, 16384):
         lib.gprtSetSortConfig(ctx.h, 8, tile, 0)
         one()
         ctx.sync()
@@ -1033,7 +1033,7 @@ def sort_workload(args):
         ctx.sync()
         lib.gprtKernelTimerEnable(ctx.h, False)
         sweep[tile] = _device_ms(kernel_report(lib, ctx), SORT_KERNELS) / 5
-    lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+    lib.gprtSetSortConfig(ctx.h, 8, 0, 0)
     got_k, got_v = kb.read(np.uint32), vb.read(np.uint32)
     order = np.argsort(keys, kind="stable")
     ok = bool(np.array_equal(got_k, keys[order]) and np.array_equal(got_v, order.astype(np.uint32)))

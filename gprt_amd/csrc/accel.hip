@@ -565,7 +565,8 @@ GPRT_API void gprtAccelExportLBVH(GPRTContext _ctx, GPRTAccel _a, GPRTBuffer _no
 
 // Autotuner for batched builds: times the graph-captured batch build of `accels` for every
 // digit width {4, 6, 8} x tile {2048, 4096, 8192} keys x sort-pass occupancy {none, 8, 16} waves
-// per CU (median of `reps` graph launches, HIP events on the context's stream), keeps the
+// per CU, and the 16384-key tile (one workgroup per CU, no cap) per digit width: 30 configurations
+// (median of `reps` graph launches, HIP events on the context's stream), keeps the
 // fastest configuration in the context and leaves the accels built with it. Results do not depend
 // on the configuration. Writes a JSON report into `report` (up to `cap` bytes); returns the index
 // of the best configuration.
@@ -583,7 +584,7 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
   // An accel keeps the build mode of its last build (default parameters if it has none).
   for (auto *a : accels)
     if (a->mode == GPRT_BUILD_MODE_UNINITIALIZED) a->mode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
-  const int bits[3] = {4, 6, 8}, tiles[3] = {2048, 4096, 8192}, waves[3] = {0, 8, 16};
+  const int bits[3] = {4, 6, 8}, tiles[4] = {2048, 4096, 8192, 16384}, waves[3] = {0, 8, 16};
   const bool timer = ctx->timer.enabled;
   ctx->timer.enabled = false;
   hipEvent_t e0, e1;
@@ -596,6 +597,7 @@ GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext _ctx, GPRTAccel *_acce
   for (int b : bits)
     for (int t : tiles)
       for (int w : waves) {
+        if (t == 16384 && w != 0) continue;  // one workgroup per CU already: a cap changes nothing
         SortConfig cfg = ctx->sortCfg;
         cfg.digitBits = b;
         cfg.tileKeys = t;

@@ -63,7 +63,8 @@ struct KernelTimer {
 // Radix sort tuning knobs (the autotuner sweeps them).
 struct SortConfig {
   int digitBits = 8;     // 4, 6 or 8
-  int tileKeys = 8192;   // keys per tile: 256 threads x {8, 16} (x2 for 512 threads); 8192 measured fastest
+  int tileKeys = 0;      // keys per tile: 256 threads x {8, 16}, 512 x 16 or 1024 x 16; 0: automatic (8192, or
+                         // 16384 for accel builds averaging at least 2^20 primitives per mesh)
   int threads = 256;     // 256 or 512
   int wavesPerCU = 0;    // LBVH sort passes: occupancy cap through LDS padding (0 = none)
 };

@@ -1290,6 +1290,8 @@ void launch_sort_passes(Context *ctx, hipStream_t s, BVHStore *st, const BuildAr
     else if (st->cfg.tileKeys == 4096)
       launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 256, 16>, dim3(st->tiles), dim3(256),
              sort_lds_pad<BITS, 256, 16>(wpc), a, p);
+    else if (st->cfg.tileKeys == 16384)  // one 130 KB workgroup per CU
+      launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 1024, 16>, dim3(st->tiles), dim3(1024), 0, a, p);
     else
       launch(ctx, s, "lbvh_sort_pass", k_onesweep<BITS, 512, 16>, dim3(st->tiles), dim3(512),
              sort_lds_pad<BITS, 512, 16>(wpc), a, p);
@@ -1497,9 +1499,20 @@ void accel_detach(Accel *a) {
   }
 }
 
+// The automatic tile size: 16384-key tiles (one 1024-thread sort workgroup per CU) for builds
+// averaging at least 2^20 primitives per mesh, whose sort passes then run a single short chain of
+// tiles (4,194,304 triangles: 33 instead of 38 us per pass); 8192 otherwise, where more tiles
+// per mesh keep the look-backs short.
+static int auto_tile_keys(Accel **accels, uint32_t count) {
+  uint64_t prims = 0;
+  for (uint32_t m = 0; m < count; ++m) prims += accels[m]->geom->icount;
+  return prims >= (uint64_t)count << 20 ? 16384 : 8192;
+}
+
 bool lbvh_build_batch(Context *ctx, hipStream_t s, Accel **accels, uint32_t count, bool useGraph,
                       bool layoutOnly) {
-  const SortConfig cfg = ctx->sortCfg;
+  SortConfig cfg = ctx->sortCfg;
+  if (cfg.tileKeys == 0) cfg.tileKeys = auto_tile_keys(accels, count);
   const uint32_t tileKeys = (uint32_t)cfg.tileKeys;
   std::vector<MeshDesc> desc(count);
   std::vector<uint32_t> tileMesh;

@@ -665,8 +665,8 @@ GPRT_API void gprtSetSortConfig(GPRTContext _ctx, uint32_t digitBits, uint32_t t
     fail("gprtSetSortConfig: digitBits %u is not one of 4, 6, 8", digitBits);
     return;
   }
-  if (tileKeys != 2048 && tileKeys != 4096 && tileKeys != 8192) {
-    fail("gprtSetSortConfig: tileKeys %u is not one of 2048, 4096, 8192", tileKeys);
+  if (tileKeys != 0 && tileKeys != 2048 && tileKeys != 4096 && tileKeys != 8192 && tileKeys != 16384) {
+    fail("gprtSetSortConfig: tileKeys %u is not one of 0 (automatic), 2048, 4096, 8192, 16384", tileKeys);
     return;
   }
   if (wavesPerCU > 64) {

@@ -415,12 +415,14 @@ bool run_bits(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt
   const bool wide = sizeof(KT) == 8 || VB == 64;
   if (cfg.tileKeys == 2048) {
     run_typed<KT, VB, BITS, 256, 8>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
-  } else if (cfg.tileKeys == 8192 && !wide) {
+  } else if ((cfg.tileKeys == 8192 || cfg.tileKeys == 0) && !wide) {  // 0: automatic, as 8192
     run_typed<KT, VB, BITS, 512, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
-  } else if (cfg.tileKeys == 4096 || cfg.tileKeys == 8192) {
+  } else if (cfg.tileKeys == 16384 && !wide) {
+    run_typed<KT, VB, BITS, 1024, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
+  } else if (cfg.tileKeys == 4096 || cfg.tileKeys == 8192 || cfg.tileKeys == 16384 || cfg.tileKeys == 0) {
     run_typed<KT, VB, BITS, 256, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
   } else {
-    fail("sort: tileKeys %d is not one of 2048, 4096, 8192", cfg.tileKeys);
+    fail("sort: tileKeys %d is not one of 2048, 4096, 8192, 16384", cfg.tileKeys);
     return false;
   }
   return true;
@@ -440,7 +442,8 @@ bool run_kv(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt, 
 int tile_keys_effective(const SortConfig &cfg, int keyBits, int valBits) {
   const bool wide = keyBits == 64 || valBits == 64;
   if (cfg.tileKeys == 2048) return 2048;
-  if (cfg.tileKeys == 8192 && !wide) return 8192;
+  if ((cfg.tileKeys == 8192 || cfg.tileKeys == 0) && !wide) return 8192;
+  if (cfg.tileKeys == 16384 && !wide) return 16384;
   return 4096;
 }
 

@@ -36,7 +36,7 @@ def batch_meshes(orc):
 
 # Every configuration the autotuner may pick: digit width x tile x sort-pass occupancy cap.
 @pytest.mark.parametrize("bits", [8, 6, 4])
-@pytest.mark.parametrize("tile", [2048, 4096, 8192])
+@pytest.mark.parametrize("tile", [2048, 4096, 8192, 16384])
 @pytest.mark.parametrize("waves", [0, 8, 16])
 def test_batch_equals_oracle_per_mesh(ctx, orc, batch_meshes, bits, tile, waves):
     ctx.lib.gprtSetSortConfig(ctx.h, bits, tile, waves)
@@ -56,7 +56,7 @@ def test_batch_equals_oracle_per_mesh(ctx, orc, batch_meshes, bits, tile, waves)
         for a in accs:
             a.destroy()
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 0, 0)
 
 
 def test_batch_graph_replays_with_moving_vertices(ctx, orc):
@@ -158,9 +158,10 @@ def test_autotune_small_batch(ctx, orc):
         assert api.last_error() is None, api.last_error()
         r = json.loads(rep.value.decode())
         cfgs = r["configs"]
-        assert len(cfgs) == 27
+        assert len(cfgs) == 30
         assert {(c["digitBits"], c["tileKeys"], c["wavesPerCU"]) for c in cfgs} == {
-            (b, t, w) for b in (4, 6, 8) for t in (2048, 4096, 8192) for w in (0, 8, 16)}
+            (b, t, w) for b in (4, 6, 8) for t in (2048, 4096, 8192) for w in (0, 8, 16)} | {
+            (b, 16384, 0) for b in (4, 6, 8)}
         ms = [c["ms"] for c in cfgs]
         assert best == int(np.argmin(ms))
         for k in ("digitBits", "tileKeys", "wavesPerCU"):
@@ -175,6 +176,6 @@ def test_autotune_small_batch(ctx, orc):
         assert "twice" in (api.last_error() or "")
         ctx.lib.gprtClearLastError()
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 0, 0)
         for a in accs:
             a.destroy()

@@ -157,7 +157,7 @@ def test_batched_4096_meshes_autotuned(ctx, orc):
         best = ctx.lib.gprtAccelBuildBatchAutotune(ctx.h, arr, len(accs), 1, rep, len(rep))
         assert api.last_error() is None, api.last_error()
         r = json.loads(rep.value.decode())
-        assert len(r["configs"]) == 27 and best == int(np.argmin([c["ms"] for c in r["configs"]]))
+        assert len(r["configs"]) == 30 and best == int(np.argmin([c["ms"] for c in r["configs"]]))
         got = [C.c_uint32(0) for _ in range(3)]
         ctx.lib.gprtGetSortConfig(ctx.h, *[C.byref(g) for g in got])
         assert [g.value for g in got] == [r["best"][k] for k in ("digitBits", "tileKeys", "wavesPerCU")]
@@ -189,7 +189,7 @@ def test_batched_4096_meshes_autotuned(ctx, orc):
             np.testing.assert_array_equal(aabbs.view(np.uint32), ra.view(np.uint32), err_msg=f"mesh {m}")
         assert len(bad) == 0, f"{len(bad)} root boxes differ from the boxes of their vertices"
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 0, 0)
         for a in accs:
             a.destroy()
 

@@ -155,7 +155,7 @@ def test_sort_t02_payload_unique_shuffled(ctx):
 
 
 @pytest.mark.parametrize("bits", [4, 6, 8])
-@pytest.mark.parametrize("tile", [2048, 4096, 8192])
+@pytest.mark.parametrize("tile", [2048, 4096, 8192, 16384])
 def test_sort_stable_all_configs(ctx, orc, bits, tile):
     ctx.lib.gprtSetSortConfig(ctx.h, bits, tile, 0)
     try:
@@ -168,7 +168,7 @@ def test_sort_stable_all_configs(ctx, orc, bits, tile):
         np.testing.assert_array_equal(k, rk)
         np.testing.assert_array_equal(v, rv)
     finally:
-        ctx.lib.gprtSetSortConfig(ctx.h, 8, 8192, 0)
+        ctx.lib.gprtSetSortConfig(ctx.h, 8, 0, 0)
 
 
 @pytest.mark.parametrize("kdt,vdt", [(np.uint32, None), (np.uint64, None), (np.uint64, np.uint32),
