@@ -36,7 +36,6 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr int kMortonBits = 30;
-constexpr int kThreads = 256;  // non-sort kernels
 
 struct Subtree {  // a finished subtree passed up a level: {box lo, ref}, {box hi, first leaf}
   float4 a, b;
@@ -275,8 +274,10 @@ __global__ void __launch_bounds__(kBoundsThreads) k_bounds(BuildArgs a, uint32_t
 // Morton keys of triangle centroids (gprt_lbvh.hlsl:139-146, :262-277) + per-mesh digit
 // histograms of every sort pass (only for meshes spanning more than one tile).
 // ------------------------------------------------------------------------------------------------
+constexpr int kKeysThreads = 1024;  // one workgroup per tile
+
 template <int BITS>
-__global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBase) {
+__global__ void __launch_bounds__(kKeysThreads) k_keys(BuildArgs a, uint32_t tileBase) {
   constexpr int R = 1 << BITS;
   extern __shared__ uint32_t s_h[];  // [passes][R]
   const uint32_t t = tileBase + blockIdx.x;
@@ -285,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
   const TileRange r = tile_range(a, d, t, m);
   const bool multi = d.ntiles > 1;
   if (multi) {
-    for (int i = threadIdx.x; i < a.passes * R; i += kThreads) s_h[i] = 0;
+    for (int i = threadIdx.x; i < a.passes * R; i += kKeysThreads) s_h[i] = 0;
     __syncthreads();
   }
   const uint32_t *h = (const uint32_t *)a.headers + (size_t)m * 16;
@@ -294,11 +295,11 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
   // kKeysBatch keys per thread at a time: the centroid loads of a batch are issued together (the
   // code stores would otherwise keep the next key's loads behind them).
   constexpr int kKeysBatch = 8;
-  for (uint32_t i0 = r.lo + threadIdx.x; i0 < r.hi; i0 += kThreads * kKeysBatch) {
+  for (uint32_t i0 = r.lo + threadIdx.x; i0 < r.hi; i0 += kKeysThreads * kKeysBatch) {
     float cx[kKeysBatch], cy[kKeysBatch], cz[kKeysBatch];
 #pragma unroll
     for (int u = 0; u < kKeysBatch; ++u) {
-      const uint32_t i = min(i0 + u * kThreads, r.hi - 1);  // past the end: repeats a valid one
+      const uint32_t i = min(i0 + u * kKeysThreads, r.hi - 1);  // past the end: repeats a valid one
       const float *cs = a.centroids + 3 * ((size_t)d.primOffset + i);
       cx[u] = cs[0];
       cy[u] = cs[1];
@@ -306,7 +307,7 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
     }
 #pragma unroll
     for (int u = 0; u < kKeysBatch; ++u) {
-      const uint32_t i = i0 + u * kThreads;
+      const uint32_t i = i0 + u * kKeysThreads;
       if (i >= r.hi) break;
       const float qx = fminf(fmaxf((cx[u] - lox) / ex * 1024.0f, 0.0f), 1023.0f);
       const float qy = fminf(fmaxf((cy[u] - loy) / ey * 1024.0f, 0.0f), 1023.0f);
@@ -314,10 +315,19 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
       const uint32_t code = morton30((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
       a.codes[d.primOffset + i] = code;
       if (multi) {
+        // Keys in primitive order are spatially coherent, so a wave's high digits mostly agree:
+        // then one lane adds the wave's count instead of 64 atomics queueing on one LDS word.
+        const uint64_t active = __ballot(true);
         for (int ps = 0; ps < a.passes; ++ps) {
           const int shift = ps * BITS;
           const int bits = min(BITS, kMortonBits - shift);
-          atomicAdd(&s_h[ps * R + ((code >> shift) & ((1u << bits) - 1u))], 1u);
+          const uint32_t dig = (code >> shift) & ((1u << bits) - 1u);
+          const uint32_t lead = __builtin_amdgcn_readfirstlane(dig);
+          if (__ballot(dig == lead) == active) {
+            if (__lane_id() == (uint32_t)__builtin_ctzll(active)) atomicAdd(&s_h[ps * R + lead], (uint32_t)__popcll(active));
+          } else {
+            atomicAdd(&s_h[ps * R + dig], 1u);
+          }
         }
       }
     }
@@ -325,7 +335,7 @@ __global__ void __launch_bounds__(kThreads) k_keys(BuildArgs a, uint32_t tileBas
   if (multi) {
     __syncthreads();
     uint32_t *gh = a.hist + (size_t)m * a.passes * R;
-    for (int i = threadIdx.x; i < a.passes * R; i += kThreads) {
+    for (int i = threadIdx.x; i < a.passes * R; i += kKeysThreads) {
       const uint32_t c = s_h[i];
       if (c) atomicAdd(&gh[i], c);
     }
@@ -1309,17 +1319,17 @@ void enqueue_build(Context *ctx, hipStream_t s, BVHStore *st) {
   const size_t histLds = (size_t)st->passes * R * 4;
   switch (st->cfg.digitBits) {
     case 4:
-      launch(ctx, s, "lbvh_keys", k_keys<4>, dim3(st->tiles), dim3(kThreads), histLds, a, 0u);
+      launch(ctx, s, "lbvh_keys", k_keys<4>, dim3(st->tiles), dim3(kKeysThreads), histLds, a, 0u);
       launch(ctx, s, "lbvh_hist_scan", k_hist_scan<4>, dim3(st->meshes), dim3(256), 0, a);
       launch_sort_passes<4>(ctx, s, st, a);
       break;
     case 6:
-      launch(ctx, s, "lbvh_keys", k_keys<6>, dim3(st->tiles), dim3(kThreads), histLds, a, 0u);
+      launch(ctx, s, "lbvh_keys", k_keys<6>, dim3(st->tiles), dim3(kKeysThreads), histLds, a, 0u);
       launch(ctx, s, "lbvh_hist_scan", k_hist_scan<6>, dim3(st->meshes), dim3(256), 0, a);
       launch_sort_passes<6>(ctx, s, st, a);
       break;
     default:
-      launch(ctx, s, "lbvh_keys", k_keys<8>, dim3(st->tiles), dim3(kThreads), histLds, a, 0u);
+      launch(ctx, s, "lbvh_keys", k_keys<8>, dim3(st->tiles), dim3(kKeysThreads), histLds, a, 0u);
       launch(ctx, s, "lbvh_hist_scan", k_hist_scan<8>, dim3(st->meshes), dim3(256), 0, a);
       launch_sort_passes<8>(ctx, s, st, a);
       break;
