@@ -242,7 +242,9 @@ def scene_workload(args):
         return float(lib.gprtEndProfile(ctx.h))
 
     # Overlap of inference and traversal: a short sweep of chunk counts and traversal workgroups
-    # per CU (results do not depend on it); the fastest is used for the timed steps.
+    # per CU (results do not depend on it) on the ungated trace, where the inference always runs;
+    # the fastest is used from then on.
+    lib.gprtProposalSetGate(ctx.h, prop.h, float("-inf"))
     overlap_sweep = {}
     for chunks, blocks in [(0, 0)] + [(c, b) for c in args.overlap_chunks for b in args.overlap_blocks]:
         lib.gprtProposalSetOverlap(ctx.h, prop.h, chunks, blocks)
@@ -251,7 +253,8 @@ def scene_workload(args):
     best_overlap = min(overlap_sweep, key=overlap_sweep.get)
     oc, ob = (int(x) for x in best_overlap.split("x"))
     lib.gprtProposalSetOverlap(ctx.h, prop.h, oc, ob)
-    log(f"overlap sweep (learned trace ms): {overlap_sweep}; using {best_overlap}")
+    lib.gprtProposalSetGate(ctx.h, prop.h, gate)
+    log(f"overlap sweep (ungated learned trace ms): {overlap_sweep}; using {best_overlap}")
     learned_wall = []
 
     def step():
@@ -281,8 +284,11 @@ def scene_workload(args):
             f"{rec['total_ms'] / rec['launches']:9.4f} ms/launch")
     build_ms = ms_of(per_kernel, "lbvh_")[0] / K
     cut_ms = (ms_of(per_kernel, "mlp_treelet")[0] + ms_of(per_kernel, "mlp_leaf_class")[0]) / K
-    trace_ms_tot, trace_n = ms_of(per_kernel, "trace_closest")
-    plain_ms = (trace_ms_tot - ms_of(per_kernel, "trace_closest_proposal")[0]) / K
+    # The plain traversal kernel's time per launch: when the gate admits no ray, the learned trace
+    # launches that same kernel (and no inference), so launches are averaged rather than counted
+    # per step.
+    plain_rec = per_kernel.get("trace_closest", {"launches": 0, "total_ms": 0.0})
+    plain_ms = plain_rec["total_ms"] / max(plain_rec["launches"], 1)
     prop_trace_ms = ms_of(per_kernel, "trace_closest_proposal")[0] / K
     # Inference = the fused encoding / hidden / output-layer / argmax kernel (per step, summed over
     # its chunks); the learned trace's time is its device wall time (inference and traversal may
@@ -293,6 +299,29 @@ def scene_workload(args):
     h1, h2 = hb.read(api.HIT_DTYPE), hb2.read(api.HIT_DTYPE)
     mismatches = int((h1.view(np.uint32).reshape(-1, 4) != h2.view(np.uint32).reshape(-1, 4)).any(1).sum())
     hit_rate = float((h1["primID"] >= 0).mean())
+
+    # The ungated learned trace (every ray walks its proposed treelet first; the inference always
+    # runs), after the timed steps: its wall time, its kernels' times and its hits.
+    lib.gprtProposalSetGate(ctx.h, prop.h, float("-inf"))
+    learned_trace()
+    ctx.sync()
+    ungated_wall = []
+    lib.gprtKernelTimerReset(ctx.h)
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    ungated_runs = max(1, min(K, 5))
+    for _ in range(ungated_runs):
+        ungated_wall.append(learned_trace())
+    ctx.sync()
+    lib.gprtKernelTimerEnable(ctx.h, False)
+    ungated_kernels = kernel_report(lib, ctx)
+    lib.gprtProposalSetGate(ctx.h, prop.h, gate)
+    ungated_ms = float(np.median(ungated_wall))
+    ungated_infer_ms = ms_of(ungated_kernels, "mlp_infer")[0] / ungated_runs
+    ungated_traverse_ms = ms_of(ungated_kernels, "trace_closest_proposal")[0] / ungated_runs
+    h3 = hb2.read(api.HIT_DTYPE)
+    mismatches_ungated = int((h1.view(np.uint32).reshape(-1, 4) != h3.view(np.uint32).reshape(-1, 4)).any(1).sum())
+    log(f"learned trace: gated {learned_ms:.3f} ms (inference {infer_ms:.3f} ms/step), ungated {ungated_ms:.3f} ms "
+        f"(inference {ungated_infer_ms:.3f} + traversal {ungated_traverse_ms:.3f}); plain {plain_ms:.3f} ms")
 
     fetch_split = proposal_fetch_split(lib, ctx, acc, prop, rb, n_rays, h1, N_TRIS)
     log(f"proposal node fetches per ray: {json.dumps(fetch_split)}")
@@ -309,9 +338,9 @@ def scene_workload(args):
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_prim": LBVH_BYTES_PER_PRIM,
                   "traffic": TRAFFIC_BYTES["build_per_prim"] * N_TRIS}
     roof_build["frac"] = roof_build["achieved"] / HBM_PEAK_GBS
-    out_ms_tot, out_n = ms_of(per_kernel, "mlp_infer_fused")
+    out_ms_tot, out_n = ms_of(ungated_kernels, "mlp_infer_fused")
     out_avg = out_ms_tot / max(out_n, 1)
-    out_rays = n_rays * K / max(out_n, 1)  # rays per launch
+    out_rays = n_rays * ungated_runs / max(out_n, 1)  # rays per launch
     roof_mlp = {"kernel": "mlp_infer_fused (encoding, hidden layers, output layer + argmax)", "bound": "mfma",
                 "achieved": out_rays * mlp_flop_per_ray(classes) / (out_avg * 1e-3) / 1e12,
                 "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "avg_launch_ms": out_avg,
@@ -398,6 +427,11 @@ def scene_workload(args):
                        "overlap_sweep_ms": overlap_sweep,
                        "learned_vs_plain": plain_ms / learned_ms, "treelet_cut_ms": cut_ms,
                        "hit_mismatches_learned_vs_plain": mismatches, "hit_rate": hit_rate,
+                       "ungated_trace_ms_learned": ungated_ms, "ungated_infer_ms": ungated_infer_ms,
+                       "ungated_traverse_ms": ungated_traverse_ms,
+                       "ungated_serial_sum_ms": ungated_infer_ms + ungated_traverse_ms,
+                       "ungated_learned_vs_plain": plain_ms / ungated_ms,
+                       "hit_mismatches_ungated_vs_plain": mismatches_ungated,
                        "train_s": train_s, "train_loss": loss, "train_top1": train_acc,
                        "gate": gate, "gate_keep_fraction": gate_keep,
                        "proposal_fetches": fetch_split},

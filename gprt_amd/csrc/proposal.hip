@@ -504,6 +504,9 @@ void trace_detail(const char *fn, Context *ctx, Accel *a, Proposal *p, Buffer *r
     return;
   }
   if (count == 0) return;
+  // A gate of +infinity admits no ray (every top logit is below it), so the classes would all be
+  // discarded: the trace runs without the proposal and without its inference. Hits are the same.
+  if (p && p->gate == INFINITY) p = nullptr;
   hipStream_t s = ctx->stream;
   int32_t *cls = nullptr;
   uint32_t *vis = visits ? (uint32_t *)visits->d : nullptr;

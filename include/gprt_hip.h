@@ -391,8 +391,9 @@ GPRT_API void gprtProposalExportTreelets(GPRTContext context, GPRTProposal propo
                                          GPRTBuffer primClass);
 /* Confidence gate of proposal-first traces: a ray whose top logit is below `threshold` is traced
  * without its proposal (for such a ray the treelet walk more likely costs than saves node fetches).
- * -INFINITY (the default) keeps the proposal of every ray, +INFINITY drops all. Hits never depend
- * on it. gprtProposalInfer is not gated. */
+ * -INFINITY (the default) keeps the proposal of every ray; +INFINITY drops all, and the trace then
+ * skips the inference too (it is the plain trace). Hits never depend on it. gprtProposalInfer is
+ * not gated. */
 GPRT_API void gprtProposalSetGate(GPRTContext context, GPRTProposal proposal, float threshold);
 GPRT_API float gprtProposalGetGate(GPRTProposal proposal);
 /* Sets the gate from measured cost on a sample of rays (use rays of the workload's distribution,

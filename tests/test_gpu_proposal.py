@@ -376,7 +376,19 @@ def test_confidence_gate(ctx, scene):
     h_all, v_all = _trace(ctx, acc, test, prop)
     _same(h_all, plain)
     lib.gprtProposalSetGate(ctx.h, prop.h, np.inf)
+    # A gate that admits no ray skips the inference as well: the trace is the plain one.
+    lib.gprtKernelTimerEnable(ctx.h, True)
+    lib.gprtKernelTimerReset(ctx.h)
     h_none, v_none = _trace(ctx, acc, test, prop)
+    ctx.sync()
+    launches = C.c_uint32(0)
+    lib.gprtKernelTimerQuery(ctx.h, b"mlp_infer", C.byref(launches))
+    assert launches.value == 0
+    lib.gprtKernelTimerQuery(ctx.h, b"trace_closest_proposal", C.byref(launches))
+    assert launches.value == 0
+    lib.gprtKernelTimerQuery(ctx.h, b"trace_closest", C.byref(launches))
+    assert launches.value == 1
+    lib.gprtKernelTimerEnable(ctx.h, False)
     _same(h_none, plain)
     np.testing.assert_array_equal(v_none, v_plain)
     # gprtProposalInfer ignores the gate.
