@@ -310,7 +310,8 @@ GPRT_API void gprtAccelBuildBatch(GPRTContext context, GPRTAccel *accels, uint32
                                   GPRTBuildParams buildParams GPRT_IF_CPP(= GPRTBuildParams()));
 
 /* Project-written autotuner for batched builds: times the graph-captured batch build for every
- * digitBits {4,6,8} x tileKeys {2048,4096,8192} x wavesPerCU {0,8,16} (median of `reps` launches),
+ * digitBits {4,6,8} x tileKeys {2048,4096,8192} x wavesPerCU {0,8,16}, and tileKeys 16384 (one
+ * workgroup per CU, so without a cap) for each digit width: 30 configurations (median of `reps` launches),
  * keeps the fastest configuration in the context (gprtGetSortConfig reads it back), leaves the
  * accels built with it and writes a JSON report. Returns the index of the best configuration. */
 GPRT_API uint32_t gprtAccelBuildBatchAutotune(GPRTContext context, GPRTAccel *accels, uint32_t count, uint32_t reps,
