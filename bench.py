@@ -573,10 +573,13 @@ def dynamic_workload(args):
     instance accel on top of them, then 1024^2 pinhole primary rays and 1024^2 bounce rays from
     their hits (2,097,152 rays) are traced plainly and with the learned proposal (classes = the
     instance accel's 4,096 top-level leaf groups). Every 16 frames the proposal is fine-tuned on
-    that frame's rays with the plain traversal's labels (all 2,097,152 by default). Frame times
+    that frame's rays with the plain traversal's labels (all 2,097,152 by default) and its
+    confidence gate calibrated on all of them. Frame times
     come from the library's per-kernel HIP events: plain frame = perturb + builds + bounce
-    generation + plain traces; learned frame = the same with inference + proposal-first traces;
-    the fine-tune is timed on its own (wall clock)."""
+    generation + plain traces; learned frame = the same with the learned traces in place of the
+    plain ones (gated as calibrated at the last fine-tune; their device wall time between two
+    events); the ungated learned traces of the last frame are timed apart;
+    the fine-tune (with the calibration) is timed on its own (wall clock)."""
     from gprt_amd import api
     from gprt_amd.dynamic import DynamicScene
 
@@ -607,19 +610,32 @@ def dynamic_workload(args):
     mism = 0
     hit_rates = []
     t0 = time.perf_counter()
+    gate_keeps = []
     for f in range(frames):
+        # The frame of DynamicScene.frame, step by step: the plain part under the kernel timer, the
+        # learned traces timed whole between two events (with the gate admitting no ray they launch
+        # the plain traversal kernel, so kernel names cannot tell them apart).
         lib.gprtKernelTimerReset(ctx.h)
         lib.gprtKernelTimerEnable(ctx.h, True)
         timing["on"] = True
-        tuned = ds.frame(f, tune_rays=tune_rays, epochs=args.tune_epochs, lr=args.lr, on_tune=on_tune)
+        ds.rebuild(f)
+        ds.trace_plain(f)
+        tuned = None
+        if f % 16 == 0:
+            on_tune(True)
+            tuned = ds.fine_tune(tune_rays, args.tune_epochs, args.lr)
+            on_tune(False)
+            gate_keeps.append(ds.gate_keep)
         ctx.sync()
         lib.gprtKernelTimerEnable(ctx.h, False)
         timing["on"] = False
         pk = kernel_report(lib, ctx)
+        lib.gprtBeginProfile(ctx.h)
+        ds.trace_learned()
+        tl = float(lib.gprtEndProfile(ctx.h))
         common = ms_of(pk, "synth_")[0] + ms_of(pk, "lbvh_")[0] + ms_of(pk, "tlas_")[0] + ms_of(pk, "fill_u32")[0] \
             + ms_of(pk, "scan_")[0]
         tp = ms_of(pk, "trace_closest_instances")[0]
-        tl = ms_of(pk, "trace_closest_proposal_instances")[0] + ms_of(pk, "mlp_infer")[0]
         plain_ms.append(common + tp)
         learned_ms.append(common + tl)
         build_ms.append(ms_of(pk, "lbvh_")[0] + ms_of(pk, "tlas_")[0])
@@ -631,6 +647,16 @@ def dynamic_workload(args):
             mism += ds.mismatches()
             hit_rates.append(ds.proposal_hit_rate())
     wall = time.perf_counter() - t0
+    # The last frame's learned traces ungated (every ray walks its proposed leaf group first), for
+    # comparison; the calibrated gate is put back afterwards.
+    gate = float(lib.gprtProposalGetGate(ds.prop.h))
+    lib.gprtProposalSetGate(ctx.h, ds.prop.h, float("-inf"))
+    ds.trace_learned()
+    lib.gprtBeginProfile(ctx.h)
+    ds.trace_learned()
+    ungated_ms = float(lib.gprtEndProfile(ctx.h))
+    mism += ds.mismatches()
+    lib.gprtProposalSetGate(ctx.h, ds.prop.h, gate)
     pct = lambda a, q: float(np.percentile(np.asarray(a), q))
     rays = ds.rays_per_frame
     # Node fetches per ray of the last frame's plain traces (untimed), for the traversal's
@@ -698,6 +724,9 @@ def dynamic_workload(args):
             "trace_ms_plain_p50": pct(tplain, 50), "trace_mrays_s_plain_p50": rays / (pct(tplain, 50) * 1e3),
             "trace_ms_learned_p50": pct(tlearn, 50), "trace_mrays_s_learned_p50": rays / (pct(tlearn, 50) * 1e3),
             "learned_vs_plain_trace": pct(tplain, 50) / pct(tlearn, 50),
+            "gate_keep_by_tune": gate_keeps,
+            "ungated_trace_ms_learned": ungated_ms,
+            "ungated_learned_vs_plain_trace": pct(tplain, 50) / ungated_ms,
             "fine_tune_wall_ms_mean": float(np.mean(tune_ms)) if tune_ms else None,
             "proposal_hit_rate_by_check": hit_rates, "hit_mismatches_learned_vs_plain": mism},
         "roofline": roof,

@@ -618,16 +618,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 // ------------------------------------------------------------------------------------------------
 // Treelets
 // ------------------------------------------------------------------------------------------------
-// One workgroup: breadth-first over the top `depth` levels. classRoot[c] = node code (inner
-// index >= 0, or ~leaf); innerClass[node] = c for inner roots; leafClass[leaf] = c for leaves
-// that are themselves roots (shallower than `depth`, or at it).
+// One workgroup: breadth-first over the top `depth` levels (depth <= 12, so a level holds at most
+// 4096 nodes). classRoot[c] = node code (inner index >= 0, or ~leaf); innerClass[node] = c for
+// inner roots; leafClass[leaf] = c for leaves that are themselves roots (shallower than `depth`, or
+// at it). Classes are numbered breadth-first, left to right within a level: each level's frontier
+// is compacted by a workgroup prefix sum, never by atomics, so the numbering depends on the tree
+// alone and a network trained against it stays valid when the proposal is bound again (in another
+// process, after deserialisation, or after a rebuild that yields the same tree).
 __global__ void __launch_bounds__(1024) k_treelet_cut(const BVHNode *__restrict__ nodes, uint32_t nprims, int depth,
                                                       int32_t *__restrict__ classRoot, int32_t *__restrict__ innerClass,
                                                       int32_t *__restrict__ leafClass, uint32_t *__restrict__ numClasses) {
+  constexpr int kPer = 4;  // frontier entries per thread: 1024 x 4 = 4096
   __shared__ int32_t s_front[2][4096];
+  __shared__ uint32_t s_wsum[1024 / 64];
   __shared__ uint32_t s_n[2];
   __shared__ uint32_t s_cls;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) {
     s_n[0] = 1;
     s_cls = 0;
@@ -636,24 +642,49 @@ __global__ void __launch_bounds__(1024) k_treelet_cut(const BVHNode *__restrict_
   __syncthreads();
   for (int d = 0; d <= depth; ++d) {
     const int cur = d & 1, nxt = cur ^ 1;
-    if (tid == 0) s_n[nxt] = 0;
+    const uint32_t n = s_n[cur], clsBase = s_cls;
+    // Thread t takes entries [4t, 4t + 4). An entry is a class (a leaf, or a node at the cut
+    // depth) or passes its two children to the next level; packed counts: classes in the high 16
+    // bits, children in the low 16 (each total is at most 4096).
+    int32_t code[kPer];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const uint32_t i = (uint32_t)(kPer * tid + j);
+      code[j] = i < n ? s_front[cur][i] : 0;
+      if (i < n) mine += (code[j] < 0 || d == depth) ? (1u << 16) : 2u;
+    }
+    const uint32_t incl = wave_incl_sum(mine);
+    if (lane == 63) s_wsum[wave] = incl;
     __syncthreads();
-    const uint32_t n = s_n[cur];
-    for (uint32_t i = tid; i < n; i += blockDim.x) {
-      const int32_t code = s_front[cur][i];
-      if (code < 0 || d == depth) {
-        const uint32_t c = atomicAdd(&s_cls, 1u);
-        classRoot[c] = code;
-        if (code < 0)
-          leafClass[~code] = (int32_t)c;
+    uint32_t off = incl - mine, total = 0;
+    for (int w = 0; w < 1024 / 64; ++w) {
+      const uint32_t x = s_wsum[w];
+      if (w < wave) off += x;
+      total += x;
+    }
+    uint32_t c = clsBase + (off >> 16), o = off & 0xFFFFu;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if ((uint32_t)(kPer * tid + j) >= n) break;
+      if (code[j] < 0 || d == depth) {
+        classRoot[c] = code[j];
+        if (code[j] < 0)
+          leafClass[~code[j]] = (int32_t)c;
         else
-          innerClass[code] = (int32_t)c;
+          innerClass[code[j]] = (int32_t)c;
+        ++c;
       } else {
-        const int4 dd = nodes[code].d;
-        const uint32_t o = atomicAdd(&s_n[nxt], 2u);
+        const int4 dd = nodes[code[j]].d;
         s_front[nxt][o] = dd.x;
         s_front[nxt][o + 1] = dd.y;
+        o += 2;
       }
+    }
+    __syncthreads();  // this level's counts and frontier are read before they change
+    if (tid == 0) {
+      s_n[nxt] = total & 0xFFFFu;
+      s_cls = clsBase + (total >> 16);
     }
     __syncthreads();
   }

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "bvh.hpp"
+#include "device_common.hpp"
 #include "internal.hpp"
 #include "mlp.hpp"
 
@@ -30,6 +31,21 @@ __global__ void __launch_bounds__(256) k_compact(const GPRTRay *__restrict__ ray
   const uint32_t o = offs[i];
   outRays[o] = rays[i];
   outLabels[o] = labels[i];
+}
+
+// Gate calibration: each ray's top logit becomes a sort key that orders logits from the largest
+// down (its order-preserving bits inverted; -0 taken as +0, which compares equal to it, and NaN as
+// -infinity, which no threshold keeps), and its node-fetch difference (proposal-first minus plain,
+// two's complement) the payload.
+__global__ void __launch_bounds__(256) k_gate_keys(uint32_t *__restrict__ logitKey, uint32_t *__restrict__ v0Delta,
+                                                   const uint32_t *__restrict__ v1, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f = __uint_as_float(logitKey[i]);
+  if (f != f) f = -INFINITY;
+  if (f == 0.0f) f = 0.0f;
+  logitKey[i] = ~f2ord(f);
+  v0Delta[i] = v1[i] - v0Delta[i];
 }
 
 __global__ void __launch_bounds__(256) k_inst_of_hits(const GPRTHit *__restrict__ hits, uint32_t n,
@@ -620,14 +636,16 @@ GPRT_API float gprtProposalCalibrateGate(GPRTContext _ctx, GPRTProposal _p, GPRT
   if (count == 0) return 0.f;
   hipStream_t s = ctx->stream;
   const size_t n = count;
+  const SortConfig cfg = ctx->sortCfg;
+  const size_t sortBytes = sort_scratch_bytes(n, 32, 32, 0, 32, cfg);
+  const size_t main = (n * (4 + 4 + 4 + 4 + sizeof(GPRTHit)) + 255) & ~(size_t)255;
   char *blk = nullptr;
-  const size_t bytes = n * (4 + 4 + 4 + 4 + sizeof(GPRTHit));
-  if (!hip_ok(hipMalloc((void **)&blk, bytes), "hipMalloc(calibration)")) return NAN;
+  if (!hip_ok(hipMalloc((void **)&blk, main + sortBytes), "hipMalloc(calibration)")) return NAN;
   int32_t *cls = (int32_t *)blk;
-  float *logit = (float *)(blk + 4 * n);
-  uint32_t *v0 = (uint32_t *)(blk + 8 * n), *v1 = (uint32_t *)(blk + 12 * n);
-  void *hits = blk + 16 * n;
-  mlp_infer(ctx, s, p, rays->d, count, cls, logit, -INFINITY);
+  uint32_t *logit = (uint32_t *)(blk + 4 * n);  // the top logits, then their sort keys
+  uint32_t *v0 = (uint32_t *)(blk + 8 * n), *v1 = (uint32_t *)(blk + 12 * n);  // v0: then the differences
+  char *hits = blk + 16 * n;  // the traces' hits, then the sort's ping-pong buffers
+  mlp_infer(ctx, s, p, rays->d, count, cls, (float *)logit, -INFINITY);
   if (a->kind == AccelKind::Instances) {
     tlas_trace(ctx, s, a, rays->d, hits, count, nullptr, 0, nullptr, nullptr, v0);
     tlas_trace(ctx, s, a, rays->d, hits, count, cls, p->numClasses, nullptr, nullptr, v1);
@@ -635,28 +653,36 @@ GPRT_API float gprtProposalCalibrateGate(GPRTContext _ctx, GPRTProposal _p, GPRT
     trace_closest_ex(ctx, s, a, rays->d, hits, count, nullptr, nullptr, 0, v0);
     trace_closest_ex(ctx, s, a, rays->d, hits, count, cls, p->classRoot, p->numClasses, v1);
   }
-  std::vector<float> lg(n);
-  std::vector<uint32_t> h0(n), h1(n);
-  bool ok = hip_ok(hipMemcpyAsync(lg.data(), logit, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
-            hip_ok(hipMemcpyAsync(h0.data(), v0, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
-            hip_ok(hipMemcpyAsync(h1.data(), v1, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
+  // Rays from the largest top logit down, with their fetch differences, sorted on the device.
+  launch(ctx, s, "proposal_gate_keys", k_gate_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, logit, v0,
+         (const uint32_t *)v1, (uint32_t)n);
+  std::vector<uint32_t> key(n), delta(n);
+  bool ok = sort_run(ctx, s, logit, v0, hits, hits + 4 * n, n, 32, 32, 0, 32, blk + main, cfg) &&
+            hip_ok(hipMemcpyAsync(key.data(), logit, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
+            hip_ok(hipMemcpyAsync(delta.data(), v0, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
             stream_wait(ctx, s, fn);
   hipFree(blk);
   if (!ok) return NAN;
-  std::vector<uint32_t> order(n);
-  for (size_t i = 0; i < n; ++i) order[i] = (uint32_t)i;
-  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return lg[x] > lg[y]; });
+  // Proposing to the rays with the k largest logits costs the sum of their first k differences;
+  // the cheapest k wins. A cut between equal logits cannot be expressed by a threshold.
   int64_t cum = 0, best = 0;
   size_t keep = 0;
   for (size_t k = 0; k < n; ++k) {
-    cum += (int64_t)h1[order[k]] - (int64_t)h0[order[k]];
-    // A cut between equal logits cannot be expressed by a threshold.
-    if (cum < best && (k + 1 == n || lg[order[k + 1]] < lg[order[k]])) {
+    cum += (int32_t)delta[k];
+    if (cum < best && (k + 1 == n || key[k + 1] != key[k])) {
       best = cum;
       keep = k + 1;
     }
   }
-  p->gate = keep == 0 ? INFINITY : (keep == n ? -INFINITY : lg[order[keep - 1]]);
+  float gate = -INFINITY;
+  if (keep == 0) {
+    gate = INFINITY;
+  } else if (keep < n) {
+    const uint32_t u = ~key[keep - 1];  // back from the order-preserving bits to the logit
+    const uint32_t bits = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    std::memcpy(&gate, &bits, 4);
+  }
+  p->gate = gate;
   return (float)keep / (float)n;
 }
 

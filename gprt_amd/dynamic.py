@@ -9,7 +9,10 @@ Every frame f:
     leaf), side^2 cosine-weighted bounce rays are generated from their hits
     (gprtSynthBounceRays) and traced the same way;
   * every `tune_every` frames the proposal is fine-tuned on that frame's rays, labelled by the
-    plain traversal's top-level leaves (gprtProposalTrainLabels);
+    plain traversal's top-level leaves (gprtProposalTrainLabels), and its confidence gate is
+    calibrated on the same rays (gprtProposalCalibrateGate: the logit threshold that minimises
+    their node fetches; +infinity when proposing to no ray is cheapest, and the learned traces then
+    skip the inference);
   * both ray sets are traced again with the proposal first (instances and hits written to the
     `*_learned` buffers), which must give the plain traversal's hits.
 
@@ -45,6 +48,8 @@ class DynamicScene:
         self.ip, self.ib, self.lp, self.lb, self.ip2, self.ib2 = (ctx.device_buffer(nbytes=4 * n) for _ in range(6))
         self.prop = api.Proposal(ctx, max_batch=max_batch, seed=proposal_seed)
         self.classes = self.prop.bind(self.tlas)
+        self.cal = None  # the frame's primary and bounce rays together, for the gate calibration
+        self.gate_keep = None  # share of the calibration rays the gate kept at the last fine-tune
         self.leaves_per_class = int(self.lib.gprtAccelGetPrimCount(self.tlas.h)) // self.classes
 
     @property
@@ -65,12 +70,21 @@ class DynamicScene:
         api.bounce_rays(ctx, self.tlas, self.pb, self.hp, self.ip, n, self.bb, n, seed=7000 + f)
         api.trace_detail(ctx, self.tlas, self.bb, self.hb, n, self.ib, self.lb)
 
-    def fine_tune(self, rays: int | None = None, epochs: int = 2, lr: float = 1e-3):
+    def fine_tune(self, rays: int | None = None, epochs: int = 2, lr: float = 1e-3, calibrate: bool = True):
         """Minibatch Adam over the frame's primary and bounce rays (the first `rays` / 2 of each;
-        all of them by default), labelled by the plain traversal's top-level leaves."""
+        all of them by default), labelled by the plain traversal's top-level leaves; then (with
+        `calibrate`) the confidence gate from all of the frame's rays."""
         k = self.n if rays is None else min(self.n, rays // 2)
         l1 = self.prop.train_labels(self.pb, self.lp, k, epochs=epochs, lr=lr)
         l2 = self.prop.train_labels(self.bb, self.lb, k, epochs=epochs, lr=lr)
+        if calibrate:
+            n = self.n
+            if self.cal is None:
+                self.cal = self.ctx.device_buffer(nbytes=2 * 32 * n)
+            self.lib.gprtBufferCopy(self.ctx.h, self.pb.h, self.cal.h, 0, 0, 32, n, 0, 0)
+            self.lib.gprtBufferCopy(self.ctx.h, self.bb.h, self.cal.h, 0, n, 32, n, 0, 0)
+            self.gate_keep = float(self.lib.gprtProposalCalibrateGate(self.ctx.h, self.prop.h, self.tlas.h,
+                                                                      self.cal.h, 2 * n))
         return l1, l2
 
     def trace_learned(self):
@@ -79,7 +93,7 @@ class DynamicScene:
         api.trace_detail(ctx, self.tlas, self.bb, self.hb2, n, self.ib2, proposal=self.prop)
 
     def frame(self, f: int, tune_every: int = 16, tune_rays: int | None = None, epochs: int = 2,
-              lr: float = 1e-3, on_tune=None):
+              lr: float = 1e-3, on_tune=None, calibrate: bool = True):
         """One frame; returns the fine-tune's (loss, top-1) pairs in frames that have one.
         `on_tune(before: bool)` is called around the fine-tune (the benchmark pauses its kernel
         timer there)."""
@@ -89,7 +103,7 @@ class DynamicScene:
         if tune_every and f % tune_every == 0:
             if on_tune:
                 on_tune(True)
-            tuned = self.fine_tune(tune_rays, epochs, lr)
+            tuned = self.fine_tune(tune_rays, epochs, lr, calibrate)
             if on_tune:
                 on_tune(False)
         self.trace_learned()
@@ -116,8 +130,9 @@ class DynamicScene:
 
     def destroy(self):
         for b in (self.pb, self.bb, self.hp, self.hb, self.hp2, self.hb2, self.ip, self.ib, self.lp, self.lb,
-                  self.ip2, self.ib2, self.cur):
-            b.destroy()
+                  self.ip2, self.ib2, self.cur, self.cal):
+            if b is not None:
+                b.destroy()
         self.prop.destroy()
         self.tlas.destroy()
         for a in self.blas:

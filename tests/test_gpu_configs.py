@@ -245,5 +245,9 @@ def test_dynamic_scene_32_frames(ctx, orc):
     assert [f for f, _ in tuned] == [0, 16]
     for f, ((l1, a1), (l2, a2)) in tuned:
         assert np.isfinite(l1) and np.isfinite(l2)
-    _log(f"dynamic: fine-tunes {tuned}; proposal hit rate {ds.proposal_hit_rate():.3f}")
+    # The gate was calibrated at each fine-tune (learned hits matched the plain ones above whatever
+    # it kept).
+    assert ds.gate_keep is not None and 0.0 <= ds.gate_keep <= 1.0
+    assert not np.isnan(ctx.lib.gprtProposalGetGate(ds.prop.h))
+    _log(f"dynamic: fine-tunes {tuned}; proposal hit rate {ds.proposal_hit_rate():.3f}; gate keeps {ds.gate_keep:.4f}")
     ds.destroy()

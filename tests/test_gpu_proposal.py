@@ -186,6 +186,16 @@ def test_treelet_cut(ctx, scene):
     for r in ref_ids:
         assert depth[r] == 12 or (r >= n - 1 and depth[r] <= 12)
     assert len(set(ref_ids)) == nc
+    # Classes are numbered breadth-first, left to right within a level (the queue order above), so
+    # the numbering depends on the tree alone: binding again gives the same table.
+    assert ref_ids == [x for x in order if depth[x] == 12 or (x >= n - 1 and depth[x] <= 12)]
+    again = api.Proposal(ctx, max_batch=256, seed=2)
+    assert again.bind(acc, 12) == nc
+    crb2, pcb2 = ctx.device_buffer(nbytes=4), ctx.device_buffer(nbytes=4)
+    ctx.lib.gprtProposalExportTreelets(ctx.h, again.h, crb2.h, pcb2.h)
+    np.testing.assert_array_equal(crb2.read(np.int32, nc), roots)
+    np.testing.assert_array_equal(pcb2.read(np.int32, len(idx)), pclass)
+    again.destroy()
     # every primitive's treelet root is an ancestor-or-self of its leaf
     leaf_of = np.empty(n, dtype=np.int64)
     leaf_of[nodes[n - 1:, 3]] = np.arange(n - 1, 2 * n - 1)
