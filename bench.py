@@ -65,12 +65,12 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 28221.6e6 + 138.6e6,  # the plain trace of the workload's 8,388,608 rays
+    "trace_closest": 28218.3e6 + 138.6e6,  # the plain trace of the workload's 8,388,608 rays
     "mlp_infer_fused": 275.9e6 + 33.6e6,  # one launch of 8,388,608 rays, 2,897 classes bound
     "lbvh_sort_pass": 30.6e6 + 34.9e6,  # one pass over 4,194,304 keys on 16,384-key tiles
     # per primitive, summed over the build kernels (init, bounds, keys, histogram scan, 4 sort
     # passes, sub-tile, region, top)
-    "build_per_prim": 0.3 + 60.1 + 16.2 + 4 * (7.3 + 8.3) + 263.7 + 1.2 + 0.1,
+    "build_per_prim": 0.3 + 60.1 + 16.2 + 4 * (7.3 + 8.3) + 263.6 + 1.2 + 0.1,
     "batched_build_per_prim": 0.7 + 60.1 + 16.5 + 0.1 + 4 * (7.61 + 9.18) + 220.1 + 1.4 + 0.04,
     "boxes_build_per_prim": 0.3 + 36.1 + 16.2 + 4 * (7.3 + 8.4) + 262.3 + 1.1 + 0.04,
     "refit_per_prim": 300.2 + 1.2 + 0.1,  # sub-tile, region, top

@@ -349,6 +349,8 @@ typedef struct GPRTProposal_T *GPRTProposal;
 GPRT_API GPRTProposal gprtProposalCreate(GPRTContext context, uint32_t maxBatch, uint64_t seed);
 GPRT_API void gprtProposalDestroy(GPRTContext context, GPRTProposal proposal);
 /* Cuts the accel's tree at cutDepth; returns the number of treelets. Rebind after a rebuild.
+ * Treelets are numbered from the tree alone (breadth-first, left to right within a level), so a
+ * trained network bound again to the same tree, in this process or another, keeps its classes.
  * For an instance accel the classes are its top-level leaves (cutDepth is ignored; at most
  * 4096). */
 GPRT_API uint32_t gprtProposalBindAccel(GPRTContext context, GPRTProposal proposal, GPRTAccel accel,
