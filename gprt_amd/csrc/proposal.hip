@@ -659,8 +659,9 @@ GPRT_API float gprtProposalCalibrateGate(GPRTContext _ctx, GPRTProposal _p, GPRT
   std::vector<uint32_t> key(n), delta(n);
   bool ok = sort_run(ctx, s, logit, v0, hits, hits + 4 * n, n, 32, 32, 0, 32, blk + main, cfg) &&
             hip_ok(hipMemcpyAsync(key.data(), logit, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
-            hip_ok(hipMemcpyAsync(delta.data(), v0, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
-            stream_wait(ctx, s, fn);
+            hip_ok(hipMemcpyAsync(delta.data(), v0, 4 * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  // Whatever failed, the work already enqueued ends before its memory is freed.
+  ok = stream_wait(ctx, s, fn) && ok;
   hipFree(blk);
   if (!ok) return NAN;
   // Proposing to the rays with the k largest logits costs the sum of their first k differences;
