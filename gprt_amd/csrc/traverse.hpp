@@ -48,6 +48,15 @@ struct LaneStack {
   uint32_t *spill;  // per-lane global entries [kLdsStack, maxStack)
 };
 
+// Reads stack entry `sp` (the first kLdsStack in LDS, the rest in the lane's spill area). The spill
+// read is a relaxed wave-scope atomic load, an ordinary global load in the code, so that the
+// compiler does not fold the two reads into one flat load behind an address select: that costs
+// about seven more vector instructions per pop and waits for every outstanding memory operation.
+__device__ __forceinline__ uint32_t stack_read(const LaneStack &st, int sp) {
+  if (sp < kLdsStack) return st.lds[sp][st.tid];
+  return __hip_atomic_load(&st.spill[sp - kLdsStack], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // Tests one triangle record ({v0, primID} {v1} {v2}) of instance `inst`; `tag` is recorded with a hit.
 __device__ __forceinline__ void intersect_record(const RayPre &r, const TriRecord &tr, uint32_t inst, int tag,
                                                  HitState &h) {
@@ -156,7 +165,7 @@ __device__ __forceinline__ uint32_t walk(const RayPre &r, const BVHNode *__restr
       if (sp == sp0) break;
       --sp;
       if (sp >= MaxStack) break;  // unreachable within the depth bounds above
-      node = (int)(sp < kLdsStack ? st.lds[sp][st.tid] : st.spill[sp - kLdsStack]);
+      node = (int)stack_read(st, sp);
     }
   }
   return visits;
@@ -178,17 +187,18 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
                                                  LeafFn &&leaf) {
   constexpr int kDone = INT_MIN;
   if (nprims == 0) return 0;  // an empty tree has no nodes
-  int q0 = -1, q1 = -1, q2 = -1, q3 = -1, nq = 0;  // queued leaf positions (q0 is the next tested)
-  auto enqueue = [&](int lf) __attribute__((always_inline)) {
+  // Queued leaves as child codes (~position, negative; q0 is the next tested): decoded when tested.
+  int q0 = -1, q1 = -1, q2 = -1, q3 = -1, nq = 0;
+  auto enqueue = [&](int code) __attribute__((always_inline)) {
     q3 = q2;
     q2 = q1;
     q1 = q0;
-    q0 = lf;
+    q0 = code;
     ++nq;
   };
   int node = start;
   if (start < 0) {
-    enqueue(~start);
+    enqueue(start);
     node = kDone;
   }
   int sp = sp0;
@@ -210,11 +220,11 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
         h0 = h0 && c0 != skip;
         h1 = h1 && c1 != skip;
         if (h0 && c0 < 0) {
-          enqueue(~c0);
+          enqueue(c0);
           h0 = false;
         }
         if (h1 && c1 < 0) {
-          enqueue(~c1);
+          enqueue(c1);
           h1 = false;
         }
         if (h0 && h1) {
@@ -234,7 +244,7 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
           node = kDone;
         } else {
           --sp;
-          node = (int)(sp < kLdsStack ? st.lds[sp][st.tid] : st.spill[sp - kLdsStack]);
+          node = (int)stack_read(st, sp);
         }
       }
     }
@@ -244,12 +254,12 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
     if (__ballot(stuck) || 2 * __popcll(waiting) >= __popcll(active)) {
       // One call site for the triangle test: lanes drain their queues together.
       while (nq > 0) {
-        const int lf = q0;
+        const int code = q0;
         q0 = q1;
         q1 = q2;
         q2 = q3;
         --nq;
-        leaf(lf, sp);
+        leaf(~code, sp);
       }
     }
     if (node == kDone && nq == 0) break;
