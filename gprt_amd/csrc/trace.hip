@@ -27,7 +27,9 @@ namespace gprt {
 
 namespace {
 
-template <bool kBoxes>
+// kPlain: no proposal, no per-ray fetch counts and fewer than kSmallTreePrims primitives (the
+// common trace, with fewer instructions per node step); otherwise the general form.
+template <bool kBoxes, bool kPlain>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const BVHNode *__restrict__ nodes, const TriRecord *__restrict__ tris, uint32_t nprims,
             const GPRTRay *__restrict__ rays, GPRTHit *__restrict__ hits, uint32_t count,
@@ -50,27 +52,32 @@ __global__ void __launch_bounds__(kTraceBlock)
       const RayPre r = ray_setup(ray.ox, ray.oy, ray.oz, ray.tmin, ray.dx, ray.dy, ray.dz);
       HitState h;
       hit_init(h, ray.tmax);
-      uint32_t visits = 0;
-      int skip = kNoSkip;
-      if (proposal) {
-        const int32_t c = proposal[ri];
-        if (c >= 0 && (uint32_t)c < numClasses) {
-          const int32_t root = classRoot[c];
-          // A treelet rooted at the tree's root (cut depth 0) would only repeat the full pass.
-          if (root != 0) {
-            visits += traverse<kMaxStack, kBoxes>(r, nodes, tris, nprims, root, h, st, 0, 0u, -1);
-            skip = root;  // searched completely: the full pass does not enter it again
+      if constexpr (kPlain) {
+        traverse<kMaxStack, kBoxes, true>(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, st, 0, 0u, -1);
+      } else {
+        uint32_t visits = 0;
+        int skip = kNoSkip;
+        if (proposal) {
+          const int32_t c = proposal[ri];
+          if (c >= 0 && (uint32_t)c < numClasses) {
+            const int32_t root = classRoot[c];
+            // A treelet rooted at the tree's root (cut depth 0) would only repeat the full pass.
+            if (root != 0) {
+              visits += traverse<kMaxStack, kBoxes>(r, nodes, tris, nprims, root, h, st, 0, 0u, -1);
+              skip = root;  // searched completely: the full pass does not enter it again
+            }
           }
         }
+        visits +=
+            traverse<kMaxStack, kBoxes>(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, st, 0, 0u, -1, skip);
+        if (visitsOut) visitsOut[ri] = visits;
       }
-      visits += traverse<kMaxStack, kBoxes>(r, nodes, tris, nprims, nprims == 1 ? ~0 : 0, h, st, 0, 0u, -1, skip);
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
       o.primID = h.prim;
       o.u = h.u;
       o.v = h.v;
       store_hit_stream(hits + ri, o);
-      if (visitsOut) visitsOut[ri] = visits;
     }
   }
 }
@@ -103,14 +110,16 @@ bool trace_closest_ex(Context *ctx, hipStream_t s, Accel *a, const void *rays, v
                                                                : (unsigned)kBlocksPerCU);
   fill_u32(ctx, s, work, 0u, 1);
   const char *name = proposal ? "trace_closest_proposal" : "trace_closest";
+  const bool plain = !proposal && !visits && a->n < kSmallTreePrims;
+  auto go = [&](auto kernel) {
+    launch(ctx, s, name, kernel, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
+           (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
+           classRoot, numClasses, visits);
+  };
   if (a->kind == AccelKind::AABBs)
-    launch(ctx, s, name, k_trace<true>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
-           (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
-           classRoot, numClasses, visits);
+    plain ? go(k_trace<true, true>) : go(k_trace<true, false>);
   else
-    launch(ctx, s, name, k_trace<false>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)a->nodes,
-           (const TriRecord *)a->tris, a->n, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
-           classRoot, numClasses, visits);
+    plain ? go(k_trace<false, true>) : go(k_trace<false, false>);
   return true;
 }
 

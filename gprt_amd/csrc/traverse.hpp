@@ -96,14 +96,24 @@ __device__ __forceinline__ void intersect_box_record(const RayPre &r, const TriR
 }
 
 // kBoxes: the leaves are box primitives rather than triangles.
-template <bool kBoxes = false>
+// kSmall: the tree has fewer than 2^26 primitives, so node and record byte offsets fit in 32 bits
+// (an address is then one shift from the array's base instead of 64-bit arithmetic).
+constexpr uint32_t kSmallTreePrims = 1u << 26;
+
+template <class T>
+__device__ __forceinline__ const T &at32(const T *base, uint32_t i) {
+  return *(const T *)((const char *)base + i * (uint32_t)sizeof(T));
+}
+
+template <bool kBoxes = false, bool kSmall = false>
 __device__ __forceinline__ void intersect_leaf(const RayPre &r, const TriRecord *__restrict__ tris, int leaf,
                                                uint32_t nprims, uint32_t inst, int tag, HitState &h) {
   if ((uint32_t)leaf >= nprims) return;
+  const TriRecord &rec = kSmall ? at32(tris, (uint32_t)leaf) : tris[(uint32_t)leaf];
   if (kBoxes)
-    intersect_box_record(r, tris[leaf], inst, tag, h);
+    intersect_box_record(r, rec, inst, tag, h);
   else
-    intersect_record(r, tris[leaf], inst, tag, h);
+    intersect_record(r, rec, inst, tag, h);
 }
 
 // Depth-first walk of the subtree rooted at `start` (an inner node index, or ~leaf position) of a
@@ -181,7 +191,9 @@ __device__ __forceinline__ uint32_t walk(const RayPre &r, const BVHNode *__restr
 // result independent of the order of the tests. Returns the number of inner nodes fetched.
 constexpr int kLeafQueue = 4;
 
-template <int MaxStack, class LeafFn>
+// kPlain: no subtree is skipped and fetches are not counted (the walk returns 0), and the tree has
+// fewer than kSmallTreePrims primitives: the form of a plain trace, with fewer instructions per node.
+template <int MaxStack, class LeafFn, bool kPlain = false>
 __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode *__restrict__ nodes, uint32_t nprims,
                                                  int start, const HitState &h, const LaneStack &st, int sp0, int skip,
                                                  LeafFn &&leaf) {
@@ -211,14 +223,17 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
       if ((uint32_t)node >= nprims - 1) {
         node = kDone;  // only a malformed tree
       } else {
-        const BVHNode nd = nodes[node];
-        ++visits;
+        // node >= 0 here: no sign extension
+        const BVHNode nd = kPlain ? at32(nodes, (uint32_t)node) : nodes[(uint32_t)node];
+        if (!kPlain) ++visits;
         float tn0, tn1;
         bool h0 = box_hit(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, h.t, tn0);
         bool h1 = box_hit(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, h.t, tn1);
         const int c0 = nd.d.x, c1 = nd.d.y;
-        h0 = h0 && c0 != skip;
-        h1 = h1 && c1 != skip;
+        if (!kPlain) {
+          h0 = h0 && c0 != skip;
+          h1 = h1 && c1 != skip;
+        }
         if (h0 && c0 < 0) {
           enqueue(c0);
           h0 = false;
@@ -269,15 +284,15 @@ __device__ __forceinline__ uint32_t walk_batched(const RayPre &r, const BVHNode 
 
 // Primitive-accel walk: leaves are triangles (or, with kBoxes, box primitives) of instance `inst`;
 // `tag` is recorded with a hit.
-template <int MaxStack, bool kBoxes = false>
+template <int MaxStack, bool kBoxes = false, bool kPlain = false>
 __device__ __forceinline__ uint32_t traverse(const RayPre &r, const BVHNode *__restrict__ nodes,
                                              const TriRecord *__restrict__ tris, uint32_t nprims, int start,
                                              HitState &h, const LaneStack &st, int sp0, uint32_t inst, int tag,
                                              int skip = kNoSkip) {
-  return walk_batched<MaxStack>(r, nodes, nprims, start, h, st, sp0, skip,
-                                [&](int leaf, int) __attribute__((always_inline)) {
-                                  intersect_leaf<kBoxes>(r, tris, leaf, nprims, inst, tag, h);
-                                });
+  auto leafTest = [&](int leaf, int) __attribute__((always_inline)) {
+    intersect_leaf<kBoxes, kPlain>(r, tris, leaf, nprims, inst, tag, h);
+  };
+  return walk_batched<MaxStack, decltype(leafTest) &, kPlain>(r, nodes, nprims, start, h, st, sp0, skip, leafTest);
 }
 
 }  // namespace gprt
