@@ -32,8 +32,7 @@ using gprt_device::comp;
 using gprt_device::fmax_ord;
 using gprt_device::fmin_ord;
 using gprt_device::ford;
-using gprt_device::kFarScale;
-using gprt_device::kNearSlack;
+using gprt_device::kBoxSlack;
 using gprt_device::ray_setup;
 using gprt_device::RayPre;
 using gprt_device::tri_hit;

@@ -125,12 +125,13 @@ __host__ __device__ __forceinline__ uint32_t ford(float f) {
 __host__ __device__ __forceinline__ float fmin_ord(float a, float b) { return ford(b) < ford(a) ? b : a; }
 __host__ __device__ __forceinline__ float fmax_ord(float a, float b) { return ford(b) > ford(a) ? b : a; }
 
-/* Conservative slab test of a node's child box. The far distance is widened by 1+2*gamma(3)
- * (Ize 2013) and the near distance is compared against tbest widened by a relative slack that
- * covers the rounding of the triangle test, so a primitive the triangle test accepts is never
- * culled by its box. */
-constexpr float kFarScale = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
-constexpr float kNearSlack = 1.0f / 16384.0f;
+/* Conservative slab test of a node's child box. The box is accepted unless the later of its entry
+ * distance and tmin exceeds the earlier of its exit distance and tbest by more than
+ * kBoxSlack * (|tnear| + |tfar|). That margin (2^-13 relative) is far above the rounding of the
+ * slab distances (Ize 2013 bounds it by 2*gamma(3) of the far distance) and of the triangle test,
+ * so a primitive the triangle test accepts is never culled by its box; a NaN difference (from
+ * overflowing distances) accepts. */
+constexpr float kBoxSlack = 1.0f / 8192.0f;
 
 __device__ __forceinline__ bool box_hit(const RayPre &r, float lox, float loy, float loz, float hix, float hiy,
                                         float hiz, float tbest, float &tnear) {
@@ -140,11 +141,9 @@ __device__ __forceinline__ bool box_hit(const RayPre &r, float lox, float loy, f
   const float t0z = (loz - r.oz) * r.iz, t1z = (hiz - r.oz) * r.iz;
   const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
   const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-  const float slack = kNearSlack * (fabsf(tn) + fabsf(tf));
   tnear = tn;
-  const float lo = fmaxf(tn - slack, r.tmin);
-  const float hi = fminf(tf * kFarScale + slack, tbest);
-  return lo <= hi;
+  const float slack = kBoxSlack * (fabsf(tn) + fabsf(tf));
+  return !(fmaxf(tn, r.tmin) - fminf(tf, tbest) > slack);
 }
 
 /* Box primitive (AABB geometry) as a closed surface: the entry distance when the ray enters the
