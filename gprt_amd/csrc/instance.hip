@@ -316,7 +316,9 @@ __global__ void __launch_bounds__(256) k_tlas_tag(TriRecord *__restrict__ topTri
   topTris[pos] = r;
 }
 
-template <bool kCheck>
+// kPlain: no proposal, no per-ray fetch counts and fewer than kSmallTreePrims top-level leaves
+// (the common trace: no skip compares or fetch counter, 32-bit node and record offsets).
+template <bool kCheck, bool kPlain>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace_tlas(const BVHNode *__restrict__ topNodes, const TriRecord *__restrict__ topTris, uint32_t leaves,
                  const TlasInst *__restrict__ insts,
@@ -343,7 +345,7 @@ __global__ void __launch_bounds__(kTraceBlock)
       uint32_t visits = 0;
       // With a proposal: the proposed class (a subtree of one instance's own tree) first, whose hit
       // tightens t; then the top-level walk, which finds that hit again and records its leaf.
-      if (proposal) {
+      if (!kPlain && proposal) {
         const int32_t c = proposal[ri];
         if (c >= 0 && (uint32_t)c < numClasses) {
           const int2 cn = classNodes[c];
@@ -360,12 +362,11 @@ __global__ void __launch_bounds__(kTraceBlock)
           }
         }
       }
-      visits += walk_batched<kMaxStack>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h, st, 0, kNoSkip,
-                                [&](int pos, int) __attribute__((always_inline)) {
+      auto topLeaf = [&](int pos, int) __attribute__((always_inline)) {
                                   // A top-level leaf: one triangle of one instance (a copy of its
                                   // record), intersected in the instance's space.
                                   if ((uint32_t)pos >= leaves) return;
-                                  const TriRecord tr = topTris[pos];
+                                  const TriRecord tr = kPlain ? at32(topTris, (uint32_t)pos) : topTris[pos];
                                   const int code = __float_as_int(tr.v1.w);
                                   if (code < 0) return;
                                   const uint32_t inst = (uint32_t)code & kLeafInstMask;
@@ -377,7 +378,9 @@ __global__ void __launch_bounds__(kTraceBlock)
                                     const RayPre ro = object_ray(insts[inst], ray);
                                     intersect_record(ro, tr, inst, k, h);
                                   }
-                                });
+                                };
+      visits += walk_batched<kMaxStack, decltype(topLeaf) &, kPlain>(rw, topNodes, leaves, leaves == 1 ? ~0 : 0, h,
+                                                                     st, 0, kNoSkip, topLeaf);
       GPRTHit o;
       o.t = h.prim >= 0 ? h.t : ray.tmax;
       o.primID = h.prim;
@@ -386,7 +389,7 @@ __global__ void __launch_bounds__(kTraceBlock)
       store_hit_stream(hits + ri, o);
       if (instOut) instOut[ri] = h.prim >= 0 ? (int32_t)h.inst : -1;
       if (leafOut) leafOut[ri] = h.prim >= 0 ? h.leaf : -1;
-      if (visitsOut) visitsOut[ri] = visits;
+      if (!kPlain && visitsOut) visitsOut[ri] = visits;
     }
   }
 }
@@ -757,16 +760,17 @@ bool tlas_trace(Context *ctx, hipStream_t s, Accel *a, const void *rays, void *h
   // destroyed or moved since the instance accel was built.
   const bool checked = t->builtEpoch != ctx->blasEpoch;
   const char *name = proposal ? "trace_closest_proposal_instances" : "trace_closest_instances";
+  const bool plain = !proposal && !visits && t->leaves < kSmallTreePrims;
+  auto go = [&](auto kernel) {
+    launch(ctx, s, name, kernel, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
+           (const TriRecord *)t->top.tris, t->leaves, (const TlasInst *)t->insts, (const BlasDesc *)ctx->blasPool,
+           (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal, (const int2 *)t->classNodes, nc,
+           instOut, leafOut, visits);
+  };
   if (checked)
-    launch(ctx, s, name, k_trace_tlas<true>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
-           (const TriRecord *)t->top.tris, t->leaves, (const TlasInst *)t->insts,
-           (const BlasDesc *)ctx->blasPool, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
-           (const int2 *)t->classNodes, nc, instOut, leafOut, visits);
+    plain ? go(k_trace_tlas<true, true>) : go(k_trace_tlas<true, false>);
   else
-    launch(ctx, s, name, k_trace_tlas<false>, dim3(blocks), dim3(kTraceBlock), 0, (const BVHNode *)t->top.nodes,
-           (const TriRecord *)t->top.tris, t->leaves, (const TlasInst *)t->insts,
-           (const BlasDesc *)ctx->blasPool, (const GPRTRay *)rays, (GPRTHit *)hits, count, work, work + 64, proposal,
-           (const int2 *)t->classNodes, nc, instOut, leafOut, visits);
+    plain ? go(k_trace_tlas<false, true>) : go(k_trace_tlas<false, false>);
   return true;
 }
 
