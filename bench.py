@@ -65,23 +65,23 @@ LBVH_BYTES_PER_PRIM = 368
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain 1 GiB copy gives): read + write.
 # The commands and per-kernel tables are in profiles/hbm_bytes_build_trace.md.
 TRAFFIC_BYTES = {
-    "trace_closest": 28218.3e6 + 138.6e6,  # the plain trace of the workload's 8,388,608 rays
+    "trace_closest": 28060.8e6 + 138.6e6,  # the plain trace of the workload's 8,388,608 rays
     "mlp_infer_fused": 275.9e6 + 33.6e6,  # one launch of 8,388,608 rays, 2,897 classes bound
-    "lbvh_sort_pass": 30.6e6 + 34.9e6,  # one pass over 4,194,304 keys on 16,384-key tiles
+    "lbvh_sort_pass": 30.9e6 + 35.3e6,  # one pass over 4,194,304 keys on 16,384-key tiles (mean of 4)
     # per primitive, summed over the build kernels (init, bounds, keys, histogram scan, 4 sort
     # passes, sub-tile, region, top)
-    "build_per_prim": 0.3 + 60.1 + 16.2 + 4 * (7.3 + 8.3) + 263.6 + 1.2 + 0.1,
+    "build_per_prim": 0.3 + 60.1 + 16.2 + (4.4 + 8.6) + 3 * (8.4 + 8.4) + 263.8 + 1.1,
     "batched_build_per_prim": 0.7 + 60.1 + 16.5 + 0.1 + 4 * (7.61 + 9.18) + 220.1 + 1.4 + 0.04,
-    "boxes_build_per_prim": 0.3 + 36.1 + 16.2 + 4 * (7.3 + 8.4) + 262.3 + 1.1 + 0.04,
-    "refit_per_prim": 300.2 + 1.2 + 0.1,  # sub-tile, region, top
-    "trace_boxes": 14917.2e6 + 69.4e6,  # one launch of 4,194,304 rays
+    "boxes_build_per_prim": 0.3 + 36.1 + 16.2 + (4.3 + 8.3) + 3 * (8.4 + 8.4) + 262.0 + 1.1,
+    "refit_per_prim": 370.0 + 1.1,  # sub-tile (two random gathers), region, top
+    "trace_boxes": 14894.1e6 + 69.4e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
     "sort_onesweep": 599.1e6 + 617.4e6,  # one pass over 2^26 u32 keys + u32 payloads (8,192-key tiles)
     "scan_lookback": 1075.8e6 + 1078.1e6,  # one launch over 2^28 u32
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
 # commands and per-kernel table in profiles/cache_hit_rates_scene.md).
-TRACE_CACHE_HIT = {"l1": 0.808, "l2": 0.684}
+TRACE_CACHE_HIT = {"l1": 0.807, "l2": 0.687}
 
 
 def log(msg):
