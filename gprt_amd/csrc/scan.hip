@@ -11,7 +11,7 @@
 //  * A tile is WAVES wave64s; each wave owns 64 lanes x K uint4 = 256K consecutive items, loaded
 //    with 16-byte coalesced vector loads and scanned with DPP wave prefix sums. The sum streams
 //    its input and output with nontemporal loads and stores.
-//  * The sum's tiles are large (16 waves x 40 items per lane = 40,960 items, one workgroup per
+//  * The sum's tiles are large (16 waves x 44 items per lane = 45,056 items, one workgroup per
 //    CU): a tile waits in its look-back for predecessors that are still loading, and that wait
 //    grows with the number of tiles in flight, so fewer, larger tiles spend less time waiting.
 //  * The look-back is done by a whole wave: lane l inspects predecessor tile (p-1-l), so 64
@@ -170,11 +170,11 @@ __global__ void __launch_bounds__(WAVES * 64)
   }
 }
 
-// Tiles: 40,960 items for the sum (16 waves x 40 items per lane, one workgroup per CU; about 110
+// Tiles: 45,056 items for the sum (16 waves x 44 items per lane, one workgroup per CU; 113
 // VGPRs, within the 128 that 16 waves on a CU allow), 4,096 for partition / select (4 waves x 16
 // items per lane), whose flags and scattered writes need more registers.
 constexpr int kScanWavesSum = 16, kScanWavesSel = 4;
-constexpr int kScanKSum = 10, kScanKSel = 4;
+constexpr int kScanKSum = 11, kScanKSel = 4;
 constexpr uint32_t kScanTileSum = kScanWavesSum * 64 * kScanKSum * 4;
 constexpr uint32_t kScanTileSel = kScanWavesSel * 64 * kScanKSel * 4;
 

@@ -45,7 +45,10 @@ def test_exclusive_sum_t03(ctx, orc, rnd):
     np.testing.assert_array_equal(out, np.arange(n, dtype=np.uint32))
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 63, 64, 65, 4095, 4096, 4097, 8191, 8192, 8193, 1000003])
+# Edge sizes around a lane's 4-item group, a wave (64 lanes), a wave's 2,816-item chunk and the
+# sum's 45,056-item tile (16 waves x 64 lanes x 44 items).
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 63, 64, 65, 2815, 2816, 2817, 4095, 4096, 4097, 8191, 8192, 8193,
+                               45055, 45056, 45057, 2 * 45056 + 1, 1000003])
 def test_exclusive_sum_random(ctx, orc, n):
     rng = np.random.default_rng(n)
     x = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)  # wraps modulo 2^32
