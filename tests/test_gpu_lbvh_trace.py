@@ -228,3 +228,53 @@ def test_refit_follows_moved_vertex_buffer(ctx, orc):
     np.testing.assert_array_equal(aabbs2.view(np.uint32), ref.view(np.uint32))
     acc.destroy()
     other.destroy()
+
+
+def test_plain_and_general_trace_forms_agree(ctx):
+    """A plain trace (no proposal, no fetch counts) runs a specialised kernel; asking for fetch
+    counts runs the general one. Both must give bit-identical hits, on a triangle accel, an AABB
+    accel and an instance accel with transformed instances."""
+    from test_oracle_boxes import random_boxes
+
+    def both(acc, rays):
+        n = len(rays)
+        rb = ctx.device_buffer(rays)
+        h1, h2 = ctx.device_buffer(nbytes=16 * n), ctx.device_buffer(nbytes=16 * n)
+        i1, i2 = ctx.device_buffer(nbytes=4 * n), ctx.device_buffer(nbytes=4 * n)
+        vb = ctx.device_buffer(nbytes=4 * n)
+        api.trace_detail(ctx, acc, rb, h1, n, i1)              # plain form
+        api.trace_detail(ctx, acc, rb, h2, n, i2, visits_buf=vb)  # general form
+        assert api.last_error() is None, api.last_error()
+        a, b = h1.read(api.HIT_DTYPE), h2.read(api.HIT_DTYPE)
+        _assert_hits_equal(a, b)
+        np.testing.assert_array_equal(i1.read(np.int32), i2.read(np.int32))
+        assert (vb.read(np.uint32) > 0).any()
+        hit = float((a["primID"] >= 0).mean())
+        for b_ in (rb, h1, h2, i1, i2, vb):
+            b_.destroy()
+        return hit
+
+    verts, idx = scenes.blob_scene(100_000, seed=12)
+    tri = _build(ctx, verts, idx)
+    prim = scenes.pinhole_rays(160, 120)
+    ph = _trace(ctx, tri, prim)
+    rays = np.concatenate([prim, scenes.bounce_rays(prim, ph, verts, idx, 20000, seed=13)])
+    hits = [both(tri, rays)]
+
+    boxes = random_boxes(20000, 9)
+    box = api.AABBAccel(ctx, boxes)
+    box.build()
+    hits.append(both(box, scenes.random_rays(30000, seed=14)))
+
+    meshes = [scenes.uniform_scene(n, seed=70 + k) for k, n in enumerate([3000, 5000, 800])]
+    blas = [_build(ctx, v, ix) for v, ix in meshes]
+    moves = [np.eye(3, 4, dtype=np.float32) for _ in blas]
+    for k, m in enumerate(moves):
+        m[:, 3] = np.float32(0.15 * k)  # translated instances: traced in object space
+    tlas = api.InstanceAccel(ctx, api.instance_records(blas, transforms=moves))
+    tlas.build()
+    hits.append(both(tlas, scenes.random_rays(30000, seed=15)))
+    print(f"hit rates (triangles, boxes, instances): {hits}", flush=True)
+    assert min(hits) > 0.02  # the comparison covers hits, not only misses
+    for a in [tlas, box, tri] + blas:
+        a.destroy()
