@@ -8,8 +8,9 @@
 //
 // Kernel shape: persistent workgroups of 128 lanes; each wave takes 64 rays at a time from a
 // global counter (load balance across rays of very different cost). One node fetch decides both
-// children (64-byte node with both child boxes); leaf children are intersected at once; the near
-// inner child is descended and the far one pushed on a per-lane stack whose first kLdsStack
+// children (64-byte node with both child boxes); leaf children are queued and tested in batches
+// across the wave (the triangle test then runs for many lanes at once); the near inner child is
+// descended and the far one pushed on a per-lane stack whose first kLdsStack
 // entries live in LDS and the rest in a per-lane global spill area. A binary radix tree over
 // distinct 64-bit keys is at most 64 levels deep, so 64 entries always suffice.
 //

@@ -1,7 +1,8 @@
 // Device-side closest-hit traversal shared by the triangle-accel and instance-accel trace kernels.
 //
 // One node fetch decides both children (64-byte node with both child boxes); leaf children are
-// handed to a leaf callback at once; the near inner child is descended and the far one pushed on
+// handed to a leaf callback (at once by walk, in batches across the wave by walk_batched); the
+// near inner child is descended and the far one pushed on
 // a per-lane stack whose first kLdsStack entries live in LDS and the rest in a per-lane global
 // spill area. The stack is shared by nested traversals (instance accels: the bottom-level walk
 // starts above the top-level entries), so a traversal takes the stack slot it starts at.
@@ -15,7 +16,8 @@ namespace gprt {
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 20;
-constexpr int kBlocksPerCU = 14;  // 28 waves per CU: the register-limited occupancy at ~60 VGPRs
+// 14 workgroups (28 waves) per CU: 16, the most the LDS stacks allow, measured no faster.
+constexpr int kBlocksPerCU = 14;
 // A binary radix tree over distinct 64-bit keys is at most 64 levels deep, so a single-level walk
 // needs at most 64 entries and a two-level walk at most 128.
 constexpr int kMaxStack = 64;
