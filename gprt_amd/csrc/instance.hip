@@ -289,22 +289,25 @@ __device__ __forceinline__ RayPre object_ray(const TlasInst &ti, const GPRTRay &
 
 // kCheck: the instance table may be stale (a triangle accel was destroyed or moved since this
 // instance accel was built), so every use of an instance first checks its descriptor.
-// After the top-level build, each top-level leaf record (in key order) is replaced by a copy of
-// its triangle record in the instance's own space: {v0, primID} {v1, instance | kLeafIdentity, or
-// -1 when disabled} {v2, top-level leaf index}. Only the parent nodes' boxes steer the top-level
-// walk, so a leaf test then reads one record instead of three dependent loads (leaf -> instance ->
+// The top-level build writes no leaf records of its own: after it, top-level leaf position `pos`
+// (key order, sorted primitive id k) gets a copy of its triangle record in the instance's own
+// space: {v0, primID} {v1, instance | kLeafIdentity, or -1 when disabled} {v2, k}; a disabled leaf
+// gets zero geometry, which no walk reads. Only the parent nodes' boxes steer the top-level walk,
+// so a leaf test then reads one record instead of three dependent loads (leaf -> instance ->
 // triangle). An instance accel thus traces the triangles its instances held when it was built.
 constexpr uint32_t kLeafIdentity = 0x40000000u;
 constexpr uint32_t kLeafInstMask = 0x3FFFFFFFu;
 
 __global__ void __launch_bounds__(256) k_tlas_tag(TriRecord *__restrict__ topTris, uint32_t leaves,
+                                                  const uint32_t *__restrict__ sortedIds,
                                                   const int2 *__restrict__ subroots,
                                                   const TlasInst *__restrict__ insts) {
   const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= leaves) return;
-  TriRecord r = topTris[pos];
-  const uint32_t k = __float_as_uint(r.v0.w);
+  const uint32_t k = sortedIds[pos];
   const int2 sr = k < leaves ? subroots[k] : make_int2(-1, 0);
+  TriRecord r;
+  r.v0 = r.v1 = r.v2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int code = -1;
   if (sr.x >= 0) {
     const TlasInst &ti = insts[sr.x];
@@ -706,6 +709,7 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
     t->top.ctx = ctx;
     t->top.geom = &t->geom;
     t->top.mode = GPRT_BUILD_MODE_FAST_TRACE_NO_UPDATE;
+    t->top.recordsExternal = true;
     t->leaves = 0;
   }
   if (L != t->leaves) {
@@ -732,7 +736,8 @@ bool tlas_build(Context *ctx, hipStream_t s, Accel *a) {
   Accel *top = &t->top;
   if (!lbvh_build_batch(ctx, s, &top, 1, true)) return false;
   launch(ctx, s, "tlas_tag", k_tlas_tag, dim3((L + 255) / 256), dim3(256), 0, (TriRecord *)top->tris, L,
-         (const int2 *)t->subroots, (const TlasInst *)t->insts);
+         (const uint32_t *)(top->store->ids + top->store->desc[top->mesh].primOffset), (const int2 *)t->subroots,
+         (const TlasInst *)t->insts);
   a->nodes = top->nodes;
   a->tris = top->tris;
   a->header = top->header;

@@ -152,6 +152,7 @@ struct MeshDesc {
   uint32_t tileOffset;   // first tile of this mesh
   uint32_t ntiles;       // ceil(ntri / tileKeys)
   uint32_t kind;         // PrimKind: what a primitive is and where it is read from
+  uint32_t noRecords;    // 1: the build writes no leaf records (the caller fills them afterwards)
 };
 
 // Primitive kinds of the LBVH builder. Triangles: uint3 indices at idx + i * istride into float3
@@ -217,6 +218,9 @@ struct Accel {
   unsigned flags = 0;
   GPRTBuildMode mode = GPRT_BUILD_MODE_UNINITIALIZED;
   bool built = false;
+  // The build leaves the leaf records unwritten: the owner fills them from the sorted primitive
+  // ids (an instance accel's top-level tree, whose records are copies of its instances' records).
+  bool recordsExternal = false;
   BVHStore *store = nullptr;  // shared by every accel of one batch build
   uint32_t mesh = 0;          // this accel's index in the store
   uint32_t n = 0;             // triangles

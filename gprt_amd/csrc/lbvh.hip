@@ -712,7 +712,7 @@ __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t t
       rec.v1 = make_float4(q.x, q.y, q.z, 0.0f);
       rec.v2 = make_float4(s.x, s.y, s.z, 0.0f);
     }
-    a.tris[d.primOffset + k] = rec;
+    if (!d.noRecords) a.tris[d.primOffset + k] = rec;
     for (int c = 0; c < 6; ++c) s_leaf[tid][c] = box[c];
   }
   lds_barrier();
@@ -1469,6 +1469,7 @@ void geom_fields(const Accel *acc, MeshDesc &d) {
   d.vstride = g->vstride;
   d.nverts = g->vcount;
   d.ntri = g->icount;
+  d.noRecords = acc->recordsExternal ? 1u : 0u;
   if (acc->kind == AccelKind::AABBs || acc->kind == AccelKind::Points) {
     d.kind = acc->kind == AccelKind::AABBs ? kPrimBoxes : kPrimPoints;  // read at verts + i * vstride
     d.idx = nullptr;
