@@ -64,7 +64,8 @@ struct KernelTimer {
 struct SortConfig {
   int digitBits = 8;     // 4, 6 or 8
   int tileKeys = 0;      // keys per tile: 256 threads x {8, 16}, 512 x 16 or 1024 x 16; 0: automatic (8192, or
-                         // 16384 for accel builds averaging at least 2^20 primitives per mesh)
+                         // 16384 for accel builds averaging at least 2^20 primitives per mesh and for
+                         // buffer sorts of at least 2^22 keys)
   int threads = 256;     // 256 or 512
   int wavesPerCU = 0;    // LBVH sort passes: occupancy cap through LDS padding (0 = none)
 };

@@ -421,13 +421,7 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const uint32_t dj = digit(k[j]);
-    uint64_t mm = ~0ull;
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-      const bool bit = (dj >> b) & 1u;
-      const uint64_t bb = wave_ballot(bit);
-      mm &= bit ? bb : ~bb;
-    }
+    const uint64_t mm = match_digit<BITS>(dj, ~0ull);
     const uint32_t rnk = mask_rank(mm);
     const uint32_t c = (uint32_t)__popcll(mm);
     const uint32_t pre = s_whist[w][dj];

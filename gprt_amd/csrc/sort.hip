@@ -219,13 +219,7 @@ __global__ void __launch_bounds__(THREADS)
   // Wave multisplit: peers with the same digit, rank among them, wave-private running counts.
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    uint64_t m = ~0ull;
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-      const bool bit = (d[j] >> b) & 1u;
-      const uint64_t bb = wave_ballot(bit);
-      m &= bit ? bb : ~bb;
-    }
+    const uint64_t m = match_digit<BITS>(d[j], ~0ull);
     const uint32_t r = mask_rank(m);
     const uint32_t c = (uint32_t)__popcll(m);
     const uint32_t pre = s_whist[w][d[j]];
@@ -380,6 +374,20 @@ __global__ void __launch_bounds__(256) k_final_copy(KT *__restrict__ keys0, cons
 // ------------------------------------------------------------------------------------------------
 inline int passes_for(int loBit, int hiBit, int bits) { return (hiBit - loBit + bits - 1) / bits; }
 
+// Keys per tile of one sort: 2,048 when asked for; 4,096 for 64-bit keys or payloads (their
+// larger tiles do not fit the LDS); automatically 16,384 (1,024 threads x 16) for at least 2^22
+// narrow keys and 8,192 below (fewer, larger tiles cost fewer look-back steps once every CU has
+// tiles to rank); otherwise the configured size.
+constexpr size_t kAutoLargeTileKeys = (size_t)1 << 22;
+
+int tile_keys_effective(const SortConfig &cfg, size_t n, int keyBits, int valBits) {
+  const bool wide = keyBits == 64 || valBits == 64;
+  if (cfg.tileKeys == 2048) return 2048;
+  if (wide) return 4096;  // 256 threads x 16 keys of 64 bits fit the LDS; 4,096 to 16,384 asked: 4,096
+  if (cfg.tileKeys == 0) return n >= kAutoLargeTileKeys ? 16384 : 8192;
+  return cfg.tileKeys;
+}
+
 template <typename KT, int VB, int BITS, int THREADS, int ITEMS>
 void run_typed(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt, void *valsAlt, size_t n, int loBit,
                int hiBit, void *scratch) {
@@ -412,20 +420,25 @@ void run_typed(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAl
 template <typename KT, int VB, int BITS>
 bool run_bits(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt, void *valsAlt, size_t n, int loBit,
               int hiBit, void *scratch, const SortConfig &cfg) {
-  const bool wide = sizeof(KT) == 8 || VB == 64;
-  if (cfg.tileKeys == 2048) {
-    run_typed<KT, VB, BITS, 256, 8>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
-  } else if ((cfg.tileKeys == 8192 || cfg.tileKeys == 0) && !wide) {  // 0: automatic, as 8192
-    run_typed<KT, VB, BITS, 512, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
-  } else if (cfg.tileKeys == 16384 && !wide) {
-    run_typed<KT, VB, BITS, 1024, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
-  } else if (cfg.tileKeys == 4096 || cfg.tileKeys == 8192 || cfg.tileKeys == 16384 || cfg.tileKeys == 0) {
-    run_typed<KT, VB, BITS, 256, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
-  } else {
-    fail("sort: tileKeys %d is not one of 2048, 4096, 8192, 16384", cfg.tileKeys);
-    return false;
+  constexpr bool wide = sizeof(KT) == 8 || VB == 64;
+  switch (tile_keys_effective(cfg, n, (int)sizeof(KT) * 8, VB)) {
+    case 2048: run_typed<KT, VB, BITS, 256, 8>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch); return true;
+    case 4096: run_typed<KT, VB, BITS, 256, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch); return true;
+    case 8192:
+      if constexpr (!wide) {
+        run_typed<KT, VB, BITS, 512, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
+        return true;
+      }
+      break;
+    case 16384:
+      if constexpr (!wide) {
+        run_typed<KT, VB, BITS, 1024, 16>(ctx, s, keys, vals, keysAlt, valsAlt, n, loBit, hiBit, scratch);
+        return true;
+      }
+      break;
   }
-  return true;
+  fail("sort: tileKeys %d is not one of 0, 2048, 4096, 8192, 16384", cfg.tileKeys);
+  return false;
 }
 
 template <typename KT, int VB>
@@ -439,20 +452,12 @@ bool run_kv(Context *ctx, hipStream_t s, void *keys, void *vals, void *keysAlt, 
   }
 }
 
-int tile_keys_effective(const SortConfig &cfg, int keyBits, int valBits) {
-  const bool wide = keyBits == 64 || valBits == 64;
-  if (cfg.tileKeys == 2048) return 2048;
-  if ((cfg.tileKeys == 8192 || cfg.tileKeys == 0) && !wide) return 8192;
-  if (cfg.tileKeys == 16384 && !wide) return 16384;
-  return 4096;
-}
-
 }  // namespace
 
 size_t sort_scratch_bytes(size_t n, int keyBits, int valBits, int loBit, int hiBit, const SortConfig &cfg) {
   const int R = 1 << cfg.digitBits;
   const int passes = passes_for(loBit, hiBit, cfg.digitBits);
-  const size_t tile = (size_t)tile_keys_effective(cfg, keyBits, valBits);
+  const size_t tile = (size_t)tile_keys_effective(cfg, n, keyBits, valBits);
   const size_t tiles = (n + tile - 1) / tile;
   return ((size_t)passes * R + 32 + (size_t)passes * tiles * R) * 4;
 }

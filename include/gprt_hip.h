@@ -293,7 +293,8 @@ GPRT_API uint32_t gprtBufferExclusiveSumNoAlloc(GPRTContext context, GPRTBuffer 
 /* Radix sort knobs used by every sort and accel build on this context (the autotuner's search
  * space): digitBits in {4, 6, 8}, tileKeys (keys per workgroup tile) in {2048, 4096, 8192, 16384}
  * or 0 (automatic: 16384 for accel builds averaging at least 2^20 primitives per mesh, else 8192;
- * 8192 for the buffer sorts), wavesPerCU (0 = no cap) caps the resident waves of the accel
+ * for the buffer sorts 16384 from 2^22 keys, else 8192; 64-bit keys or payloads use 4096 unless
+ * 2048 is asked for), wavesPerCU (0 = no cap) caps the resident waves of the accel
  * builds' sort passes through LDS padding (no effect at 16384 keys, whose 1024-thread tiles
  * already run one per CU). Results do not depend on them. Default: 8, 0, 0. */
 GPRT_API void gprtSetSortConfig(GPRTContext context, uint32_t digitBits, uint32_t tileKeys, uint32_t wavesPerCU);

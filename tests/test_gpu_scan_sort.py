@@ -225,6 +225,21 @@ def test_sort_partial_count(ctx, orc):
     np.testing.assert_array_equal(k[6000:], keys[6000:])
 
 
+@pytest.mark.parametrize("payload", [False, True])
+@pytest.mark.parametrize("n", [(1 << 22) - 1, (1 << 22) + 16_383])
+def test_sort_automatic_tile_threshold(ctx, orc, payload, n):
+    """Buffer sorts take 16,384-key tiles from 2^22 keys and 8,192-key tiles below: both sides of
+    the threshold, ragged last tiles, keys only and with payloads, duplicates in the masked digits."""
+    rng = np.random.default_rng(n + payload)
+    keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32) & np.uint32(0xF0F0FFFF)
+    vals = np.arange(n, dtype=np.uint32) if payload else None
+    k, v = _sort_kv(ctx, keys, vals)
+    rk, rv = orc.stable_sort(keys, vals)
+    np.testing.assert_array_equal(k, rk)
+    if payload:
+        np.testing.assert_array_equal(v, rv)
+
+
 def test_sort_full_size_microbench(ctx):
     # 2^26 u32 keys uniform over 32 bits with u32 payload = index (sort microbenchmark size)
     n = 1 << 26
