@@ -18,22 +18,24 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
-// The lanes of `lanes` whose BITS-bit digit equals this lane's: one ballot per digit bit, and each
-// bit keeps the lanes that agree with it, lanes & ~(ballot ^ s) with s this lane's bit
-// sign-extended to 0 or ~0. Written on 32-bit halves so that the compiler fuses the update into
-// three-input bit operations (v_bitop3_b32, v_or3_b32): about 5 vector instructions per bit,
-// where selecting ballot or ~ballot per lane as 64-bit values took about 10.
+// The lanes of `lanes` whose BITS-bit digit equals this lane's; every lane of `lanes` must be
+// active. One ballot per digit bit; a lane differs from this one where the ballot's bit differs
+// from this lane's bit s (sign-extended to 0 or ~0), so the complement of the result collects
+// ballot ^ s over the bits: one v_bitop3_b32 per 32-lane half and bit (truth table 0xF6 =
+// S0 | (S1 ^ S2), index S0 * 4 + S1 * 2 + S2), after a v_bfe_i32 and the ballot's compare. About
+// 4 vector instructions per bit, where selecting ballot or ~ballot per lane as 64-bit values
+// compiled to about 10.
 template <int BITS>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t lanes) {
-  uint32_t lo = (uint32_t)lanes, hi = (uint32_t)(lanes >> 32);
+  uint32_t lo = ~(uint32_t)lanes, hi = ~(uint32_t)(lanes >> 32);
 #pragma unroll
   for (int b = 0; b < BITS; ++b) {
-    const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
-    const uint64_t bb = __ballot(s != 0u);
-    lo &= ~((uint32_t)bb ^ s);
-    hi &= ~((uint32_t)(bb >> 32) ^ s);
+    const uint32_t s = (uint32_t)__builtin_amdgcn_sbfe((int)d, (unsigned)b, 1u);
+    const uint64_t bb = __builtin_amdgcn_uicmp(s, 0u, 33);  // 33: not equal; inactive lanes read 0
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)bb, s, 0xF6);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(bb >> 32), s, 0xF6);
   }
-  return ((uint64_t)hi << 32) | lo;
+  return ~(((uint64_t)hi << 32) | lo);
 }
 
 // Inclusive prefix sum across the 64 lanes with DPP: shifts of 1, 2, 4 and 8 lanes within each
