@@ -519,17 +519,23 @@ __global__ void __launch_bounds__(THREADS) k_onesweep(BuildArgs a, int pass) {
     s_vbuf[pos[j]] = v[j];
   }
   __syncthreads();
-  // Runs of equal digits are consecutive in the tile order, so the writes coalesce per run.
+  // Runs of equal digits are consecutive in the tile order, so the writes coalesce per run. Every
+  // position of the tile holds a key and payload (the padding included): all LDS reads are issued
+  // before the first write, so the wave waits once instead of twice per key.
+  uint32_t sk[ITEMS], sv[ITEMS], gg[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    const uint32_t i = (uint32_t)(j * THREADS + tid);
-    if (i < valid) {
-      const uint32_t key = s_buf[i];
-      const uint32_t g = s_gbase[digit(key)] + i;
-      if (g >= d.primOffset && g < segEnd) {  // guards the write; always true when consistent
-        dst[g] = key;
-        vdst[g] = s_vbuf[i];
-      }
+    sk[j] = s_buf[j * THREADS + tid];
+    sv[j] = s_vbuf[j * THREADS + tid];
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) gg[j] = s_gbase[digit(sk[j])] + (uint32_t)(j * THREADS + tid);
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    // The range test guards the write; it always holds for the tile's own keys when consistent.
+    if ((uint32_t)(j * THREADS + tid) < valid && gg[j] >= d.primOffset && gg[j] < segEnd) {
+      dst[gg[j]] = sk[j];
+      vdst[gg[j]] = sv[j];
     }
   }
 }

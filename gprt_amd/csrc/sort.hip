@@ -318,19 +318,20 @@ __global__ void __launch_bounds__(THREADS)
   }
   __syncthreads();
 
-  uint32_t gaddr[VB ? ITEMS : 1];
+  // The sorted keys back from LDS (every position of the tile holds a key, the padding included)
+  // and their global positions: all LDS reads are issued before the first write, so the wave waits
+  // once instead of twice per key.
+  KT sk[ITEMS];
+  uint32_t gaddr[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) sk[j] = s_keys[j * THREADS + tid];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) gaddr[j] = s_gbase[digit_of(sk[j], shift, dmask)] + (uint32_t)(j * THREADS + tid);
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    const uint32_t i = (uint32_t)(j * THREADS + tid);
-    if (VB) gaddr[VB ? j : 0] = 0xFFFFFFFFu;
-    if (i < valid) {
-      const KT key = s_keys[i];
-      const uint32_t g = s_gbase[digit_of(key, shift, dmask)] + i;
-      if (g < n) {  // always true when histogram and look-back agree; guards the write
-        dst[g] = key;
-        if (VB) gaddr[VB ? j : 0] = g;
-      }
-    }
+    // gaddr < n always holds for the tile's own keys when histogram and look-back agree; it guards
+    // the write.
+    if ((uint32_t)(j * THREADS + tid) < valid && gaddr[j] < n) dst[gaddr[j]] = sk[j];
   }
 
   if (VB) {
@@ -346,11 +347,12 @@ __global__ void __launch_bounds__(THREADS)
       }
     }
     __syncthreads();
+    VT sv[ITEMS];
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t i = (uint32_t)(j * THREADS + tid);
-      if (i < valid && gaddr[VB ? j : 0] < n) vdst[gaddr[VB ? j : 0]] = s_vals[i];
-    }
+    for (int j = 0; j < ITEMS; ++j) sv[j] = s_vals[j * THREADS + tid];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+      if ((uint32_t)(j * THREADS + tid) < valid && gaddr[j] < n) vdst[gaddr[j]] = sv[j];
   }
 }
 
