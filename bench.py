@@ -76,7 +76,7 @@ TRAFFIC_BYTES = {
     "refit_per_prim": 370.0 + 1.1,  # sub-tile (two random gathers), region, top
     "trace_boxes": 14894.1e6 + 69.4e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
-    "sort_onesweep": 556.5e6 + 565.9e6,  # one pass over 2^26 u32 keys + u32 payloads (16,384-key tiles)
+    "sort_onesweep": 556.5e6 + 566.3e6,  # one pass over 2^26 u32 keys + u32 payloads (16,384-key tiles)
     "scan_lookback": 1075.5e6 + 1078.1e6,  # one launch over 2^28 u32
 }
 # Vector L1 and L2 hit rates of the traversal kernel on this workload (rocprofv3 --pmc TCP_* / TCC_*;
