@@ -73,7 +73,7 @@ TRAFFIC_BYTES = {
     "build_per_prim": 0.3 + 60.1 + 16.2 + (4.4 + 8.6) + 3 * (8.4 + 8.4) + 263.8 + 1.1,
     "batched_build_per_prim": 0.7 + 60.1 + 16.5 + 0.1 + 4 * (7.61 + 9.18) + 220.1 + 1.4 + 0.04,
     "boxes_build_per_prim": 0.3 + 36.1 + 16.2 + (4.3 + 8.3) + 3 * (8.4 + 8.4) + 262.0 + 1.1,
-    "refit_per_prim": 370.0 + 1.1,  # sub-tile (two random gathers), region, top
+    "refit_per_prim": 12.0 + 263.4 + 1.2,  # identity check, sub-tile (vertex gather), region + top
     "trace_boxes": 14894.1e6 + 69.4e6,  # one launch of 4,194,304 rays
     "compute_bounds": 268.5e6 + 402.7e6,  # one launch over 16,777,216 spheres
     "sort_onesweep": 556.5e6 + 566.3e6,  # one pass over 2^26 u32 keys + u32 payloads (16,384-key tiles)
@@ -810,7 +810,7 @@ def refit_workload(args):
     lib.gprtKernelTimerEnable(ctx.h, False)
     K = args.steps
     pk = kernel_report(lib, ctx)
-    refit_ms = ms_of(pk, "lbvh_build")[0] / K
+    refit_ms = (ms_of(pk, "lbvh_build")[0] + ms_of(pk, "lbvh_ident")[0]) / K
     trace_refit_ms = ms_of(pk, "trace_closest")[0] / K
     # The same vertices rebuilt from scratch: build and trace times for comparison.
     lib.gprtKernelTimerReset(ctx.h)
@@ -823,7 +823,8 @@ def refit_workload(args):
     pk2 = kernel_report(lib, ctx)
     build_ms = ms_of(pk2, "lbvh_")[0] / K
     trace_rebuilt_ms = ms_of(pk2, "trace_closest")[0] / K
-    roof = _roof_hbm("lbvh_build_local + region + top (refit over the kept topology)", N_TRIS * REFIT_BYTES_PER_PRIM,
+    roof = _roof_hbm("lbvh_ident + build_local + region + top (refit over the kept topology)",
+                     N_TRIS * REFIT_BYTES_PER_PRIM,
                      refit_ms, bytes_per_prim=REFIT_BYTES_PER_PRIM)
     roof["traffic"] = TRAFFIC_BYTES["refit_per_prim"] * N_TRIS
     cpu = None

@@ -180,6 +180,46 @@ __device__ __forceinline__ float3 prim_centroid(const MeshDesc &d, float3 p, flo
 }
 
 // ------------------------------------------------------------------------------------------------
+// Identity-index check of one triangle mesh for a refit, which runs no bounds pass: the mesh's
+// identity word (set to 1 before the launch) is cleared by any tile holding a triangle i that does
+// not index vertices 3i, 3i+1, 3i+2, as the bounds pass does during a build. Reads the index
+// triples only (12 B per triangle, in order), so the refit's sub-tile level can skip the dependent
+// index gather for triangle soups.
+// ------------------------------------------------------------------------------------------------
+constexpr int kIdentThreads = 1024;
+
+__global__ void __launch_bounds__(64) k_ident_reset(float *__restrict__ headers, uint32_t m) {
+  if (threadIdx.x == 0) ((uint32_t *)headers)[(size_t)m * 16 + kIdentityWord] = 1u;
+}
+
+__global__ void __launch_bounds__(kIdentThreads) k_ident(BuildArgs a, uint32_t tileBase, uint32_t m) {
+  const uint32_t t = tileBase + blockIdx.x;
+  const MeshDesc d = a.desc[m];
+  const TileRange r = tile_range(a, d, t, m);
+  bool ident = true;
+  // kIdentBatch triples per thread in flight at a time (past the end a valid one is repeated).
+  constexpr int kIdentBatch = 8;
+  for (uint32_t i0 = r.lo + threadIdx.x; i0 < r.hi; i0 += kIdentThreads * kIdentBatch) {
+    uint32_t v[kIdentBatch][3];
+#pragma unroll
+    for (int u = 0; u < kIdentBatch; ++u) {
+      const uint32_t i = min(i0 + u * kIdentThreads, r.hi - 1);
+      const auto *ix = global_ptr<uint32_t>(d.idx + (size_t)i * d.istride);
+      v[u][0] = ix[0];
+      v[u][1] = ix[1];
+      v[u][2] = ix[2];
+    }
+#pragma unroll
+    for (int u = 0; u < kIdentBatch; ++u) {
+      const uint32_t i = min(i0 + u * kIdentThreads, r.hi - 1);
+      ident &= v[u][0] == 3 * i && v[u][1] == 3 * i + 1 && v[u][2] == 3 * i + 2;
+    }
+  }
+  if (!__syncthreads_and(ident ? 1 : 0) && threadIdx.x == 0)
+    atomicAnd((uint32_t *)a.headers + (size_t)m * 16 + kIdentityWord, 0u);
+}
+
+// ------------------------------------------------------------------------------------------------
 // Scene box per mesh over all vertices of its triangles (gprt_lbvh.hlsl:121-131, :215-226). The
 // triangles' centroids are stored on the way, so the key kernel reads 12 bytes per triangle
 // instead of gathering the triangle again.
@@ -626,8 +666,8 @@ __device__ __forceinline__ void put_entry(Subtree *p, const float b[6], int ref,
   p->b = make_float4(b[3], b[4], b[5], __uint_as_float(first));
 }
 
-// identOk: this launch follows the same build's bounds pass, so the mesh's identity-index word is
-// current (a refit leaves it as the last build set it and gathers the indices).
+// identOk: the mesh's identity-index word is current (set by this build's bounds pass, or by the
+// refit's own check of the index buffer).
 __global__ void __launch_bounds__(kBuildSub) k_build_sub(BuildArgs a, uint32_t tileBase, uint32_t subPerTile,
                                                          uint32_t identOk) {
   __shared__ int8_t s_delta[kBuildSub + 4];        // delta(lo - 1 + q), in [-1, 63]
@@ -1641,7 +1681,14 @@ bool lbvh_refit(Context *ctx, hipStream_t s, Accel *acc) {
   }
   const BuildArgs a = args_of(st);
   if (acc->slot != 0xFFFFFFFFu) ++ctx->blasContentEpoch;
-  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u, false);
+  // The index buffer may have changed since the build: its identity layout is checked again (the
+  // word reset by a kernel rather than a memset, so a captured refit replays correctly).
+  const bool tris = d.kind == kPrimTriangles;
+  if (tris) {
+    launch(ctx, s, "lbvh_ident", k_ident_reset, dim3(1), dim3(64), 0, st->headers, acc->mesh);
+    launch(ctx, s, "lbvh_ident", k_ident, dim3(d.ntiles), dim3(kIdentThreads), 0, a, d.tileOffset, acc->mesh);
+  }
+  launch_refit(ctx, s, st, a, d.tileOffset, d.ntiles, acc->mesh, 1u, tris);
   return true;
 }
 

@@ -230,6 +230,31 @@ def test_refit_follows_moved_vertex_buffer(ctx, orc):
     other.destroy()
 
 
+def test_refit_rechecks_index_layout(ctx, orc):
+    """A refit runs no bounds pass, so it checks again whether every triangle i indexes vertices
+    3i, 3i+1, 3i+2 (a triangle soup, whose vertices it then reads without gathering the indices):
+    after the index buffer leaves that layout and returns to it, the refit boxes equal the
+    oracle's refit of the triangles the current indices name."""
+    verts, idx = scenes.blob_scene(70_000, seed=9)
+    assert np.array_equal(idx.reshape(-1), np.arange(idx.size))  # the blob scene is a soup
+    acc = _build(ctx, verts, idx, api.GPRT_BUILD_MODE_FAST_BUILD_AND_UPDATE)
+    _, nodes0, _ = acc.export()
+    idx2 = np.ascontiguousarray(np.roll(idx, -1, axis=0))  # triangle i takes triangle i+1's vertices
+    acc.ibuf.write(idx2)
+    acc.update()
+    assert api.last_error() is None, api.last_error()
+    _, nodes1, aabbs1 = acc.export()
+    np.testing.assert_array_equal(nodes1, nodes0)
+    np.testing.assert_array_equal(aabbs1.view(np.uint32), orc.refit(verts, idx2, nodes0).view(np.uint32))
+    moved = (verts + np.float32(1e-3)).astype(np.float32)
+    acc.ibuf.write(idx)
+    acc.vbuf.write(moved)
+    acc.update()
+    _, _, aabbs2 = acc.export()
+    np.testing.assert_array_equal(aabbs2.view(np.uint32), orc.refit(moved, idx, nodes0).view(np.uint32))
+    acc.destroy()
+
+
 def test_plain_and_general_trace_forms_agree(ctx):
     """A plain trace (no proposal, no fetch counts) runs a specialised kernel; asking for fetch
     counts runs the general one. Both must give bit-identical hits, on a triangle accel, an AABB
